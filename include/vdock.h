@@ -1,0 +1,163 @@
+/*
+ * vdock.h -- C ABI of libvdock, the MI355X (gfx950) docking hot path.
+ *
+ * Plain C types and pointers only.  Every call returns VD_OK (0) or a negative vd_status and
+ * never throws; the message of the last failure is available from vd_last_error().  A context
+ * owns one HIP stream on one device and must be used from one host thread at a time.
+ * Device buffers are owned by the library; host buffers belong to the caller and are copied.
+ *
+ * The reference path this replaces (paths under the reference root, V11 = Vina-CUDA_v1.1):
+ *   kernel_grid   V11/inc/cuda/kernel1.cuh:30-41   -> vd_grid_populate
+ *   kernel_monte  V11/inc/cuda/kernel2.cuh:65-79   -> vd_dock / vd_dock_batch
+ *   main_procedure_cl V11/lib/main_procedure_cu.h:2-4 (host driver)  -> the calls below in sequence
+ *   cache::populate   V11/lib/cache.h:73            -> vd_grid_populate
+ *   model::eval_deriv V11/lib/model.cpp:1324-1334   -> vd_eval
+ */
+#ifndef VDOCK_H
+#define VDOCK_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vd_ctx vd_ctx;
+
+enum {
+  VD_OK = 0,
+  VD_ERR_ARG = -1,        /* invalid argument or call order */
+  VD_ERR_CAPACITY = -2,   /* input beyond a supported size (atoms, torsions, pairs, box) */
+  VD_ERR_HIP = -3,        /* HIP runtime failure; see vd_last_error */
+  VD_ERR_PARSE = -4,      /* malformed PDBQT input */
+  VD_ERR_NO_DEVICE = -5   /* no usable GPU */
+};
+
+/* Library version string. */
+const char* vd_version(void);
+
+/* Context on HIP device `device` (replaces cudaSetDevice + stream setup,
+ * V11/lib/main_procedure_cu.cpp:219-228).  Default scoring weights are installed. */
+int vd_create(int device, vd_ctx** out);
+void vd_destroy(vd_ctx* ctx);
+const char* vd_last_error(const vd_ctx* ctx);
+
+/* Scoring weights w[0..5] = gauss1, gauss2, repulsion, hydrophobic, hydrogen bond, rot
+ * (V11/main/main.cpp:649-654, 856-862) -> FP64 look-up tables (precalculate,
+ * V11/lib/precalculate.h:255-283) uploaded as FP32 (replaces the pre_cl upload,
+ * V11/lib/main_procedure_cu.cpp:300-356). */
+int vd_set_weights(vd_ctx* ctx, const double w[6]);
+/* FP64 tables of type pair tp (0..152): smooth value, smooth derivative/r, fast (2051 each). */
+int vd_get_lut(vd_ctx* ctx, int tp, double* smooth_e, double* smooth_dor, double* fast);
+
+/* Receptor atoms: coordinates xyz[3n] and XS types (17 = untyped, ignored)
+ * (replaces the pa_cl upload, V11/lib/main_procedure_cu.cpp:246-272). */
+int vd_set_receptor(vd_ctx* ctx, int n, const float* xyz, const int* xs);
+/* Same from a rigid receptor PDBQT text (parse_receptor_pdbqt, V11/lib/parse_pdbqt.cpp:643-652). */
+int vd_set_receptor_pdbqt(vd_ctx* ctx, const char* text, int* n_atoms);
+/* Copies up to max receptor atoms; returns the count (>= 0) or an error. */
+int vd_get_receptor(vd_ctx* ctx, int max, float* xyz, int* xs);
+
+/* Search box as Vina builds it: n = ceil(size/granularity) intervals per axis, centred
+ * (V11/main/main.cpp:864-876).  Maps have n+1 points per axis. */
+typedef struct {
+  double begin[3];
+  double end[3];
+  int n[3];
+} vd_box;
+int vd_box_from_center(const double center[3], const double size[3], double granularity, vd_box* box);
+
+/* All 17 XS maps over the box on the GPU (replaces kernel_grid, V11/inc/cuda/kernel1.cuh:30-41,
+ * and cache::populate, V11/lib/cache.cpp:148-208).  Maps stay resident on the device in the
+ * plain FP32 layout map[t][z][y][x]. */
+int vd_grid_populate(vd_ctx* ctx, const vd_box* box);
+/* Kernel time of the last vd_grid_populate in milliseconds. */
+double vd_grid_last_ms(const vd_ctx* ctx);
+/* Host copies of the resident maps: 17*(n0+1)*(n1+1)*(n2+1) floats. */
+int vd_grid_download(vd_ctx* ctx, float* maps);
+int vd_grid_upload(vd_ctx* ctx, const vd_box* box, const float* maps);
+
+/* Ligands (parse_ligand_pdbqt, V11/lib/parse_pdbqt.cpp:618-627, flattened for the device as
+ * V11/lib/main_procedure_cu.cpp:551-763 does).  Returns the ligand id. */
+int vd_ligand_add_pdbqt(vd_ctx* ctx, const char* text, int* id);
+typedef struct {
+  int n_atoms;      /* movable atoms, hydrogens included */
+  int n_torsions;   /* rigid nodes - 1 = degrees of freedom beyond 6 */
+  int n_pairs;      /* intramolecular interacting pairs */
+  int n_heavy;      /* non-hydrogen atoms (the pose coordinates reported per chain) */
+  int torsdof;      /* TORSDOF record */
+  int tree_depth;   /* depth of the rigid-node tree */
+} vd_ligand_info;
+int vd_ligand_info_get(vd_ctx* ctx, int id, vd_ligand_info* info);
+/* Flattened model: per atom xs, element class (0 = H), node, node-local xyz; per node parent,
+ * atom range [begin, end), rel_origin, rel_axis; pairs as (type pair, a, b) triples. */
+int vd_ligand_export(vd_ctx* ctx, int id, int* xs, int* el, int* node, float* local, int* parent,
+                     int* begin, int* end, float* rel_origin, float* rel_axis, int* pairs);
+int vd_ligand_count(const vd_ctx* ctx);
+
+/* Energy and gradient of n conformations of ligand `id` (model::eval_deriv,
+ * V11/lib/model.cpp:1324-1334; device form V11/inc/cuda/quasi_newton.cuh:1294-1339).
+ * conf[n][7+T] = position, quaternion (w,x,y,z), torsions; grad[n][6+T] = force, torque,
+ * torsion derivatives; e[n].  Curl caps v_intra / v_inter and the out-of-box slope as the
+ * reference's hunt_cap (10) and slope (1e6).  device_ptrs = 1: the three pointers are device
+ * pointers (no copies).  chains_per_wave: 1, 2 or 4 (0 = pick). */
+int vd_eval(vd_ctx* ctx, int id, long long n, const float* conf, float* e, float* grad,
+            float v_intra, float v_inter, float slope, int chains_per_wave, int device_ptrs);
+/* Kernel time of the last vd_eval in milliseconds. */
+double vd_eval_last_ms(const vd_ctx* ctx);
+
+/* Docking parameters (monte_carlo, V11/lib/monte_carlo.h:60-70; kernel_monte arguments). */
+enum { VD_LS_BFGS = 0, VD_LS_RILC_BFGS = 1 };
+enum { VD_MC_VINA = 0, VD_MC_QVINA2 = 1 };
+typedef struct {
+  int chains;            /* MC chains per ligand (reference: --thread) */
+  int search_depth;      /* MC steps per chain; 0 = heuristic of V11/main/main.cpp:354-380 */
+  int bfgs_steps;        /* local-search iterations; 0 = heuristic of V11/main/main.cpp:383-391 */
+  int local_search;      /* VD_LS_RILC_BFGS (reference --rilc_bfgs 1) or VD_LS_BFGS */
+  int policy;            /* VD_MC_VINA or VD_MC_QVINA2 */
+  int mutation_quirk;    /* 1: mutate one of 1+T entities like the reference GPU host tables */
+  int chains_per_wave;   /* 1, 2, 4; 0 = autotuned per ligand size bucket */
+  uint32_t seed;
+  float temperature;     /* 1.2 */
+  float amplitude;       /* mutation amplitude 2 */
+  float v_intra, v_inter;/* curl caps during search: hunt_cap (10, 10) */
+  float slope;           /* out-of-box penalty slope 1e6 */
+} vd_dock_params;
+int vd_dock_params_default(vd_dock_params* p);
+/* search_depth and bfgs_steps heuristics of V11/main/main.cpp:354-391 for a ligand. */
+int vd_dock_heuristics(int n_atoms, int n_torsions, int rilc, int* search_depth, int* bfgs_steps);
+
+typedef struct {
+  double kernel_ms;            /* device time of the docking kernel(s) */
+  unsigned long long evals;    /* energy/gradient evaluations performed */
+  int chains_per_wave;         /* used by the last launch (plain mode) */
+  int blocks;                  /* workgroups launched */
+} vd_stats;
+
+/* Plain per-ligand launch (replaces kernel_monte, V11/inc/cuda/kernel2.cuh:65-79).
+ * Per chain results: e[chains], conf[chains][7+T], heavy[chains][n_heavy][3] (host pointers,
+ * any may be NULL).  The box is the one of the resident maps. */
+int vd_dock(vd_ctx* ctx, int id, const vd_dock_params* p, float* e, float* conf, float* heavy, vd_stats* st);
+
+/* Many ligands of uneven size in one persistent launch fed by a device work queue.
+ * Results stay on the device; fetch them per ligand with vd_batch_results. */
+int vd_dock_batch(vd_ctx* ctx, int n, const int* ids, const vd_dock_params* p, vd_stats* st);
+/* Same work as vd_dock_batch issued as one plain launch per ligand (the per-ligand baseline). */
+int vd_dock_batch_plain(vd_ctx* ctx, int n, const int* ids, const vd_dock_params* p, vd_stats* st);
+/* Results of ligand k (0..n-1) of the last batch, host pointers (any may be NULL). */
+int vd_batch_results(vd_ctx* ctx, int k, float* e, float* conf, float* heavy);
+/* Best energy per ligand of the last batch, best_e[n]. */
+int vd_batch_best(vd_ctx* ctx, float* best_e);
+
+/* Output pose list: add_to_output_container over chains in order (min_rmsd, max_size;
+ * V11/lib/coords.cpp:47-62).  Writes kept chain indices sorted by energy; returns the count. */
+int vd_cluster(int n, int n_heavy, const float* e, const float* heavy, double min_rmsd, int max_size, int* kept);
+
+/* Synthetic inputs of the benchmark recipes: Poisson-disk receptor with a central cavity,
+ * random acyclic ligand with n_rot rotatable bonds (PDBQT text; returns its length, or the
+ * needed size when cap is too small). */
+int vd_synth_receptor(int n_atoms, double cube, double cavity_radius, uint64_t seed, float* xyz, int* xs);
+int vd_synth_ligand(int n_heavy, int n_rot, uint64_t seed, char* out, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

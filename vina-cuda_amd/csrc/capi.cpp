@@ -1,0 +1,774 @@
+// capi.cpp -- the C ABI of include/vdock.h on top of host_model and the HIP kernels.
+#include "../../include/vdock.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <functional>
+#include <tuple>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "host_model.h"
+#include "kernels.h"
+
+namespace {
+
+struct HipError : std::runtime_error {
+  explicit HipError(const std::string& m) : std::runtime_error(m) {}
+};
+struct ArgError : std::runtime_error {
+  int code;
+  ArgError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+#define HIP_CHECK(x)                                                                              \
+  do {                                                                                            \
+    hipError_t e_ = (x);                                                                          \
+    if (e_ != hipSuccess) throw HipError(std::string(#x) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void ensure(size_t count) {
+    if (count <= n && p) return;
+    if (p) HIP_CHECK(hipFree(p));
+    p = nullptr; n = 0;
+    HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(count, 1) * sizeof(T)));
+    n = std::max<size_t>(count, 1);
+  }
+  void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+};
+
+struct BatchState {   // results of the last batch, kept on the device
+  std::vector<int> ids;
+  std::vector<longlong4> off;      // per slot: e, conf, heavy offsets
+  std::vector<int> T, H, chains;
+  DevBuf<float> e, conf, heavy;
+};
+
+}  // namespace
+
+struct vd_ctx {
+  int device = 0;
+  bool host_only = false;      // device == -1: host model only, no HIP calls
+  hipStream_t stream = nullptr;
+  std::string err;
+  double weights[6] = {-0.035579, -0.005156, 0.840245, -0.035069, -0.587439, 0.05846};
+  vd::Precalculated pre;
+  DevBuf<float> lut_rows, lutq;
+  std::vector<float> rec_xyz;
+  std::vector<int> rec_xs;
+  // maps
+  bool maps_ready = false;
+  vd_box box{};
+  vd_grid_desc gd{};
+  DevBuf<float> maps;
+  DevBuf<float4> grid_atoms;
+  DevBuf<int2> grid_cells;
+  double grid_ms = 0, eval_ms = 0;
+  // ligands
+  std::vector<vd::Ligand> ligs;
+  // per-launch buffers
+  DevBuf<float4> blob;
+  DevBuf<vd_lig_header> headers;
+  DevBuf<uint32_t> streams;
+  DevBuf<int2> items;
+  DevBuf<longlong4> offs;
+  DevBuf<int> queue;
+  DevBuf<unsigned long long> evals;
+  DevBuf<float> io_a, io_b, io_c;
+  BatchState batch;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int n_cu = 0;
+  std::map<int, int> tuned;       // size bucket -> chains per wave
+};
+
+namespace {
+
+int guard(vd_ctx* ctx, const std::function<void()>& f) {
+  try {
+    f();
+    return VD_OK;
+  } catch (ArgError& e) {
+    if (ctx) ctx->err = e.what();
+    return e.code;
+  } catch (vd::parse_error& e) {
+    if (ctx) ctx->err = e.what();
+    return VD_ERR_PARSE;
+  } catch (HipError& e) {
+    if (ctx) ctx->err = e.what();
+    return VD_ERR_HIP;
+  } catch (std::exception& e) {
+    if (ctx) ctx->err = e.what();
+    return VD_ERR_ARG;
+  }
+}
+
+void upload_lut(vd_ctx* c) {
+  c->pre.build(c->weights);
+  if (c->host_only) return;
+  std::vector<float> rows, quads;
+  c->pre.grid_rows(rows);
+  c->pre.pair_quads(quads);
+  c->lut_rows.ensure(rows.size());
+  c->lutq.ensure(quads.size());
+  HIP_CHECK(hipMemcpy(c->lut_rows.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(c->lutq.p, quads.data(), quads.size() * 4, hipMemcpyHostToDevice));
+}
+
+vd_grid_desc make_desc(const vd_box& b) {
+  // grid::init (V11/lib/grid.cpp:29-43), computed in double and stored in float
+  vd_grid_desc g{};
+  for (int i = 0; i < 3; ++i) {
+    const int dim = b.n[i] + 1;
+    const double range = b.end[i] - b.begin[i];
+    const double dfm1 = dim - 1.0, factor = dfm1 / range;
+    g.init[i] = (float)b.begin[i];
+    g.factor[i] = (float)factor;
+    g.factor_inv[i] = (float)(1.0 / factor);
+    g.dim_fl_minus_1[i] = (float)dfm1;
+    g.dim[i] = dim;
+  }
+  g.plane = (long long)g.dim[0] * g.dim[1] * g.dim[2];
+  return g;
+}
+
+void check_box(const vd_box* b) {
+  if (!b) throw ArgError(VD_ERR_ARG, "box is NULL");
+  for (int i = 0; i < 3; ++i) {
+    if (b->n[i] < 1 || !(b->end[i] > b->begin[i])) throw ArgError(VD_ERR_ARG, "empty search box");
+    if (b->n[i] > 4096) throw ArgError(VD_ERR_CAPACITY, "search box too large");
+  }
+}
+
+void need_device(vd_ctx* c) {
+  if (c->host_only) throw ArgError(VD_ERR_NO_DEVICE, "host-only context (created with device -1)");
+  HIP_CHECK(hipSetDevice(c->device));
+}
+
+const vd::Ligand& lig_at(vd_ctx* c, int id) {
+  if (id < 0 || id >= (int)c->ligs.size()) throw ArgError(VD_ERR_ARG, "unknown ligand id " + std::to_string(id));
+  return c->ligs[id];
+}
+
+void check_params(const vd_dock_params* p) {
+  if (!p) throw ArgError(VD_ERR_ARG, "params is NULL");
+  if (p->chains < 1) throw ArgError(VD_ERR_ARG, "chains must be >= 1");
+  if (p->search_depth < 0 || p->bfgs_steps < 0) throw ArgError(VD_ERR_ARG, "negative search_depth or bfgs_steps");
+  if (p->local_search != VD_LS_BFGS && p->local_search != VD_LS_RILC_BFGS) throw ArgError(VD_ERR_ARG, "bad local_search");
+  if (p->policy != VD_MC_VINA && p->policy != VD_MC_QVINA2) throw ArgError(VD_ERR_ARG, "bad policy");
+  if (p->chains_per_wave != 0 && p->chains_per_wave != 1 && p->chains_per_wave != 2 && p->chains_per_wave != 4)
+    throw ArgError(VD_ERR_ARG, "chains_per_wave must be 0, 1, 2 or 4");
+}
+
+int heur_depth(int A, int T, int rilc) {
+  double s = (0.24 * A + 0.29 * T - 3.41) * (rilc ? 2.0 : 2.4);
+  if (s < 1) s = 1;
+  return (int)s;
+}
+int heur_bfgs(int A, int rilc) { return (unsigned)((rilc ? 30 : 25) + A) / 3; }
+
+// Group width G (lanes per chain) that a ligand allows: torsion lanes 1..T and the six rigid
+// lanes G-6..G-1 must not overlap, and every node needs a lane.
+int min_group(const vd::Ligand& L) {
+  const int T = L.n_torsions();
+  if (T + 7 <= 16) return 16;
+  if (T + 7 <= 32) return 32;
+  if (T + 7 <= 64) return 64;
+  return 0;
+}
+int size_bucket(const vd::Ligand& L) {
+  int a = (int)L.atoms.size(), t = L.n_torsions();
+  return (std::min(a / 16, 15) << 8) | std::min(t / 4, 15);
+}
+int pick_group(vd_ctx* c, const vd::Ligand& L, int cpw) {
+  const int gmin = min_group(L);
+  if (gmin == 0) throw ArgError(VD_ERR_CAPACITY, "ligand has too many torsions");
+  int G = 64;
+  if (cpw == 2) G = 32;
+  else if (cpw == 4) G = 16;
+  else if (cpw == 0) {
+    auto it = c->tuned.find(size_bucket(L));
+    if (it != c->tuned.end()) G = it->second;
+    else G = std::max(gmin, (int)L.atoms.size() + 4 <= 32 ? 32 : 64);
+  }
+  if (G < gmin) G = gmin;
+  return G;
+}
+
+vd_mc_consts make_consts(vd_ctx* c, const vd_dock_params* p, int depth, int bfgs) {
+  vd_mc_consts m{};
+  for (int i = 0; i < 3; ++i) { m.box_lo[i] = (float)c->box.begin[i]; m.box_hi[i] = (float)c->box.end[i]; }
+  m.v_intra = p->v_intra; m.v_inter = p->v_inter; m.slope = p->slope;
+  m.temperature = p->temperature; m.amplitude = p->amplitude;
+  m.search_depth = depth; m.bfgs_steps = bfgs; m.local_search = p->local_search;
+  m.mutation_quirk = p->mutation_quirk; m.policy = p->policy; m.seed = p->seed; m.chains = p->chains;
+  return m;
+}
+
+vdk::EvalK make_evalk(vd_ctx* c, float v_intra, float v_inter, float slope) {
+  vdk::EvalK k;
+  k.maps = c->maps.p; k.gd = c->gd; k.lutq = reinterpret_cast<const float4*>(c->lutq.p);
+  k.v_intra = v_intra; k.v_inter = v_inter; k.slope = slope;
+  return k;
+}
+
+struct Packed {
+  std::vector<float> blob;
+  std::vector<vd_lig_header> headers;
+  int Amax = 0, Nmax = 0;
+};
+Packed pack(vd_ctx* c, const std::vector<int>& ids, const std::vector<int>* depth = nullptr,
+            const std::vector<int>* bfgs = nullptr) {
+  Packed pk;
+  for (size_t k = 0; k < ids.size(); ++k) {
+    const vd::Ligand& L = lig_at(c, ids[k]);
+    pk.headers.push_back(vd::pack_ligand(L, pk.blob));
+    if (depth) pk.headers.back().search_depth = (*depth)[k];
+    if (bfgs) pk.headers.back().bfgs_steps = (*bfgs)[k];
+    pk.Amax = std::max(pk.Amax, (int)L.atoms.size());
+    pk.Nmax = std::max(pk.Nmax, (int)L.nodes.size());
+  }
+  return pk;
+}
+
+double elapsed_ms(vd_ctx* c) {
+  float ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  return ms;
+}
+
+// Runs one MC launch over the given ligand slots (all with the same G) and work items.
+void run_mc(vd_ctx* c, int G, const Packed& pk, const std::vector<uint32_t>& streams, const std::vector<int2>& items,
+            const std::vector<longlong4>& off, const vd_mc_consts& P, float* d_e, float* d_conf, float* d_heavy,
+            bool persistent, vd_stats* st, bool timed) {
+  c->blob.ensure(pk.blob.size() / 4 + 1);
+  c->headers.ensure(pk.headers.size());
+  c->streams.ensure(streams.size());
+  c->items.ensure(items.size());
+  c->offs.ensure(off.size());
+  c->queue.ensure(1);
+  c->evals.ensure(1);
+  HIP_CHECK(hipMemcpyAsync(c->blob.p, pk.blob.data(), pk.blob.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->headers.p, pk.headers.data(), pk.headers.size() * sizeof(vd_lig_header), hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->streams.p, streams.data(), streams.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->items.p, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->offs.p, off.data(), off.size() * sizeof(longlong4), hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemsetAsync(c->evals.p, 0, sizeof(unsigned long long), c->stream));
+  McArgs a{};
+  a.blob = c->blob.p; a.headers = c->headers.p; a.streams = c->streams.p; a.items = c->items.p;
+  a.n_items = (int)items.size();
+  a.K = make_evalk(c, P.v_intra, P.v_inter, P.slope);
+  a.P = P;
+  a.out_e = d_e; a.out_conf = d_conf; a.out_heavy = d_heavy; a.out_off = c->offs.p;
+  a.eval_count = c->evals.p;
+  a.Amax = pk.Amax; a.Nmax = pk.Nmax;
+  const int lds = vdl::mc_lds_bytes(G, pk.Amax, pk.Nmax, P.local_search == 0);
+  if (lds > 160 * 1024) throw ArgError(VD_ERR_CAPACITY, "ligand too large for the LDS layout");
+  int blocks = (int)items.size();
+  if (persistent) {
+    a.queue = c->queue.p;
+    HIP_CHECK(vdl::reset_counter(c->queue.p, c->stream));
+    const int per_cu = std::max(1, vdl::max_resident_mc_blocks(G, lds));
+    blocks = std::min<long long>((long long)per_cu * c->n_cu, (long long)items.size());
+    blocks = std::max(blocks, 1);
+  }
+  if (timed) HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+  HIP_CHECK(vdl::launch_mc(G, a, blocks, c->stream));
+  if (timed) HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+  if (st) {
+    unsigned long long ev = 0;
+    HIP_CHECK(hipMemcpyAsync(&ev, c->evals.p, sizeof(ev), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    st->evals += ev;
+    st->blocks += blocks;
+    st->chains_per_wave = 64 / G;
+    if (timed) st->kernel_ms += elapsed_ms(c);
+  } else {
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  }
+}
+
+struct LigPlan { int id, G, depth, bfgs, T, H, chains; };
+LigPlan plan_for(vd_ctx* c, int id, const vd_dock_params* p) {
+  const vd::Ligand& L = lig_at(c, id);
+  LigPlan pl;
+  pl.id = id; pl.T = L.n_torsions(); pl.H = L.n_heavy; pl.chains = p->chains;
+  pl.G = pick_group(c, L, p->chains_per_wave);
+  const int rilc = p->local_search == VD_LS_RILC_BFGS;
+  pl.depth = p->search_depth > 0 ? p->search_depth : heur_depth((int)L.atoms.size(), pl.T, rilc);
+  pl.bfgs = p->bfgs_steps > 0 ? p->bfgs_steps : heur_bfgs((int)L.atoms.size(), rilc);
+  pl.bfgs = std::max(pl.bfgs, 1);
+  return pl;
+}
+
+// Prepares the batch result buffers for the ligands in `ids`.
+void prepare_batch(vd_ctx* c, const std::vector<LigPlan>& plans) {
+  BatchState& b = c->batch;
+  b.ids.clear(); b.off.clear(); b.T.clear(); b.H.clear(); b.chains.clear();
+  long long oe = 0, oc = 0, oh = 0;
+  for (const LigPlan& pl : plans) {
+    b.ids.push_back(pl.id); b.T.push_back(pl.T); b.H.push_back(pl.H); b.chains.push_back(pl.chains);
+    b.off.push_back(make_longlong4(oe, oc, oh, 0));
+    oe += pl.chains; oc += (long long)pl.chains * (7 + pl.T); oh += (long long)pl.chains * pl.H * 3;
+  }
+  b.e.ensure(oe); b.conf.ensure(oc); b.heavy.ensure(oh);
+}
+
+}  // namespace
+
+// =============================================================================================
+extern "C" {
+
+const char* vd_version(void) { return "vdock 0.1 (gfx950)"; }
+
+int vd_create(int device, vd_ctx** out) {
+  if (!out) return VD_ERR_ARG;
+  *out = nullptr;
+  if (device == -1) {
+    vd_ctx* c = new vd_ctx();
+    c->host_only = true;
+    int rc = guard(c, [&]() { upload_lut(c); });
+    if (rc != VD_OK) { delete c; return rc; }
+    *out = c;
+    return VD_OK;
+  }
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return VD_ERR_NO_DEVICE;
+  vd_ctx* c = new vd_ctx();
+  int rc = guard(c, [&]() {
+    c->device = device;
+    HIP_CHECK(hipSetDevice(device));
+    HIP_CHECK(hipStreamCreate(&c->stream));
+    HIP_CHECK(hipEventCreate(&c->ev0));
+    HIP_CHECK(hipEventCreate(&c->ev1));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, device));
+    c->n_cu = prop.multiProcessorCount;
+    upload_lut(c);
+  });
+  if (rc != VD_OK) { delete c; return rc; }
+  *out = c;
+  return VD_OK;
+}
+
+void vd_destroy(vd_ctx* c) {
+  if (!c) return;
+  if (c->host_only) { delete c; return; }
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->lut_rows.release(); c->lutq.release(); c->maps.release(); c->grid_atoms.release(); c->grid_cells.release();
+  c->blob.release(); c->headers.release(); c->streams.release(); c->items.release(); c->offs.release();
+  c->queue.release(); c->evals.release(); c->io_a.release(); c->io_b.release(); c->io_c.release();
+  c->batch.e.release(); c->batch.conf.release(); c->batch.heavy.release();
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* vd_last_error(const vd_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int vd_set_weights(vd_ctx* c, const double w[6]) {
+  if (!c || !w) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    if (!c->host_only) HIP_CHECK(hipSetDevice(c->device));
+    std::memcpy(c->weights, w, sizeof(c->weights));
+    upload_lut(c);
+    c->maps_ready = false;
+  });
+}
+
+int vd_get_lut(vd_ctx* c, int tp, double* se, double* sd, double* fast) {
+  if (!c) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    if (tp < 0 || tp >= VD_NUM_TYPE_PAIRS) throw ArgError(VD_ERR_ARG, "type pair out of range");
+    const size_t b = (size_t)tp * VD_LUT_BINS;
+    for (int i = 0; i < VD_LUT_BINS; ++i) {
+      if (se) se[i] = c->pre.smooth_e[b + i];
+      if (sd) sd[i] = c->pre.smooth_dor[b + i];
+      if (fast) fast[i] = c->pre.fast[b + i];
+    }
+  });
+}
+
+int vd_set_receptor(vd_ctx* c, int n, const float* xyz, const int* xs) {
+  if (!c) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    if (n < 0 || (n > 0 && (!xyz || !xs))) throw ArgError(VD_ERR_ARG, "bad receptor arrays");
+    for (int i = 0; i < 3 * n; ++i) if (!std::isfinite(xyz[i])) throw ArgError(VD_ERR_ARG, "non-finite receptor coordinate");
+    for (int i = 0; i < n; ++i) if (xs[i] < 0 || xs[i] > VD_XS_SIZE) throw ArgError(VD_ERR_ARG, "bad receptor XS type");
+    c->rec_xyz.assign(xyz, xyz + 3 * (size_t)n);
+    c->rec_xs.assign(xs, xs + n);
+    c->maps_ready = false;
+  });
+}
+
+int vd_set_receptor_pdbqt(vd_ctx* c, const char* text, int* n_atoms) {
+  if (!c || !text) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    std::vector<float> xyz; std::vector<int> xs;
+    vd::parse_receptor_pdbqt(text, xyz, xs);
+    c->rec_xyz.swap(xyz); c->rec_xs.swap(xs);
+    c->maps_ready = false;
+    if (n_atoms) *n_atoms = (int)c->rec_xs.size();
+  });
+}
+
+int vd_get_receptor(vd_ctx* c, int max, float* xyz, int* xs) {
+  if (!c) return VD_ERR_ARG;
+  const int n = (int)c->rec_xs.size();
+  for (int i = 0; i < std::min(n, max); ++i) {
+    if (xyz) for (int k = 0; k < 3; ++k) xyz[3 * i + k] = c->rec_xyz[3 * i + k];
+    if (xs) xs[i] = c->rec_xs[i];
+  }
+  return n;
+}
+
+int vd_box_from_center(const double center[3], const double size[3], double granularity, vd_box* box) {
+  if (!center || !size || !box || !(granularity > 0)) return VD_ERR_ARG;
+  for (int i = 0; i < 3; ++i) {
+    if (!(size[i] > 0)) return VD_ERR_ARG;
+    box->n[i] = (int)std::ceil(size[i] / granularity);
+    const double span = granularity * box->n[i];
+    box->begin[i] = center[i] - span / 2;
+    box->end[i] = box->begin[i] + span;
+  }
+  return VD_OK;
+}
+
+int vd_grid_populate(vd_ctx* c, const vd_box* box) {
+  if (!c) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    check_box(box);
+    need_device(c);
+    c->box = *box;
+    c->gd = make_desc(*box);
+    // receptor cell list over the box grown by the cutoff (typed atoms only)
+    const double cell = 4.0, grow = 8.0 + 1e-3;
+    GridCells gcl{};
+    double org[3];
+    for (int i = 0; i < 3; ++i) {
+      org[i] = box->begin[i] - grow;
+      gcl.origin[i] = (float)org[i];
+      gcl.n[i] = (int)std::floor((box->end[i] + grow - org[i]) / cell) + 1;
+    }
+    gcl.cell = (float)cell;
+    const long long ncell = (long long)gcl.n[0] * gcl.n[1] * gcl.n[2];
+    std::vector<int> count(ncell + 1, 0);
+    std::vector<long long> cidx(c->rec_xs.size(), -1);
+    for (size_t i = 0; i < c->rec_xs.size(); ++i) {
+      if (c->rec_xs[i] >= VD_XS_SIZE) continue;
+      int ci[3]; bool ok = true;
+      for (int k = 0; k < 3; ++k) {
+        const double v = (c->rec_xyz[3 * i + k] - org[k]) / cell;
+        if (!(v >= 0) || v >= gcl.n[k]) { ok = false; break; }
+        ci[k] = (int)v;
+      }
+      if (!ok) continue;
+      cidx[i] = ((long long)ci[2] * gcl.n[1] + ci[1]) * gcl.n[0] + ci[0];
+      count[cidx[i] + 1]++;
+    }
+    for (long long k = 0; k < ncell; ++k) count[k + 1] += count[k];
+    std::vector<int2> ranges(ncell);
+    for (long long k = 0; k < ncell; ++k) ranges[k] = make_int2(count[k], count[k + 1]);
+    std::vector<float4> sorted(std::max(count[ncell], 1));
+    std::vector<int> fill(count.begin(), count.end() - 1);
+    for (size_t i = 0; i < c->rec_xs.size(); ++i) {
+      if (cidx[i] < 0) continue;
+      float4 a;
+      a.x = (float)(c->rec_xyz[3 * i] - org[0]); a.y = (float)(c->rec_xyz[3 * i + 1] - org[1]); a.z = (float)(c->rec_xyz[3 * i + 2] - org[2]);
+      uint32_t t = (uint32_t)c->rec_xs[i]; std::memcpy(&a.w, &t, 4);
+      sorted[fill[cidx[i]]++] = a;
+    }
+    c->grid_atoms.ensure(sorted.size());
+    c->grid_cells.ensure(ranges.size());
+    HIP_CHECK(hipMemcpyAsync(c->grid_atoms.p, sorted.data(), sorted.size() * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(c->grid_cells.p, ranges.data(), ranges.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    c->maps.ensure((size_t)VD_XS_SIZE * c->gd.plane);
+    HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+    HIP_CHECK(vdl::launch_grid(c->grid_atoms.p, c->grid_cells.p, gcl, reinterpret_cast<const float4*>(c->lut_rows.p), c->maps.p,
+                               c->gd, c->stream));
+    HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->grid_ms = elapsed_ms(c);
+    c->maps_ready = true;
+  });
+}
+
+double vd_grid_last_ms(const vd_ctx* c) { return c ? c->grid_ms : 0.0; }
+
+int vd_grid_download(vd_ctx* c, float* maps) {
+  if (!c || !maps) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    need_device(c);
+    if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no maps: call vd_grid_populate or vd_grid_upload first");
+    HIP_CHECK(hipMemcpy(maps, c->maps.p, (size_t)VD_XS_SIZE * c->gd.plane * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int vd_grid_upload(vd_ctx* c, const vd_box* box, const float* maps) {
+  if (!c || !maps) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    check_box(box);
+    need_device(c);
+    c->box = *box;
+    c->gd = make_desc(*box);
+    c->maps.ensure((size_t)VD_XS_SIZE * c->gd.plane);
+    HIP_CHECK(hipMemcpy(c->maps.p, maps, (size_t)VD_XS_SIZE * c->gd.plane * 4, hipMemcpyHostToDevice));
+    c->maps_ready = true;
+  });
+}
+
+int vd_ligand_add_pdbqt(vd_ctx* c, const char* text, int* id) {
+  if (!c || !text) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    vd::Ligand L = vd::parse_ligand_pdbqt(text);
+    if (min_group(L) == 0) throw ArgError(VD_ERR_CAPACITY, "ligand has more than 57 torsions");
+    c->ligs.push_back(std::move(L));
+    if (id) *id = (int)c->ligs.size() - 1;
+  });
+}
+
+int vd_ligand_info_get(vd_ctx* c, int id, vd_ligand_info* info) {
+  if (!c || !info) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    const vd::Ligand& L = lig_at(c, id);
+    info->n_atoms = (int)L.atoms.size(); info->n_torsions = L.n_torsions(); info->n_pairs = (int)L.pairs.size();
+    info->n_heavy = L.n_heavy; info->torsdof = L.torsdof; info->tree_depth = L.max_depth;
+  });
+}
+
+int vd_ligand_export(vd_ctx* c, int id, int* xs, int* el, int* node, float* local, int* parent, int* begin, int* end,
+                     float* rel_origin, float* rel_axis, int* pairs) {
+  if (!c) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    const vd::Ligand& L = lig_at(c, id);
+    for (size_t i = 0; i < L.atoms.size(); ++i) {
+      if (xs) xs[i] = L.atoms[i].xs;
+      if (el) el[i] = L.atoms[i].el;
+      if (node) node[i] = L.atoms[i].node;
+      if (local) for (int k = 0; k < 3; ++k) local[3 * i + k] = (float)L.atoms[i].local[k];
+    }
+    for (size_t k = 0; k < L.nodes.size(); ++k) {
+      if (parent) parent[k] = L.nodes[k].parent;
+      if (begin) begin[k] = L.nodes[k].begin;
+      if (end) end[k] = L.nodes[k].end;
+      for (int j = 0; j < 3; ++j) {
+        if (rel_origin) rel_origin[3 * k + j] = (float)L.nodes[k].rel_origin[j];
+        if (rel_axis) rel_axis[3 * k + j] = (float)L.nodes[k].rel_axis[j];
+      }
+    }
+    if (pairs)
+      for (size_t p = 0; p < L.pairs.size(); ++p) {
+        pairs[3 * p] = (int)(L.pairs[p] >> 16); pairs[3 * p + 1] = (int)(L.pairs[p] & 255); pairs[3 * p + 2] = (int)((L.pairs[p] >> 8) & 255);
+      }
+  });
+}
+
+int vd_ligand_count(const vd_ctx* c) { return c ? (int)c->ligs.size() : 0; }
+
+int vd_eval(vd_ctx* c, int id, long long n, const float* conf, float* e, float* grad, float v_intra, float v_inter,
+            float slope, int cpw, int device_ptrs) {
+  if (!c) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    need_device(c);
+    if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no maps");
+    if (n < 0 || (n > 0 && (!conf || !e || !grad))) throw ArgError(VD_ERR_ARG, "bad eval arrays");
+    if (cpw != 0 && cpw != 1 && cpw != 2 && cpw != 4) throw ArgError(VD_ERR_ARG, "chains_per_wave must be 0, 1, 2 or 4");
+    const vd::Ligand& L = lig_at(c, id);
+    if (n == 0) return;
+    const int T = L.n_torsions();
+    int G = pick_group(c, L, cpw);
+    Packed pk = pack(c, {id});
+    c->blob.ensure(pk.blob.size() / 4 + 1);
+    c->headers.ensure(1);
+    HIP_CHECK(hipMemcpyAsync(c->blob.p, pk.blob.data(), pk.blob.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(c->headers.p, pk.headers.data(), sizeof(vd_lig_header), hipMemcpyHostToDevice, c->stream));
+    const float* d_conf = conf; float* d_e = e; float* d_g = grad;
+    if (!device_ptrs) {
+      c->io_a.ensure((size_t)n * (7 + T)); c->io_b.ensure(n); c->io_c.ensure((size_t)n * (6 + T));
+      HIP_CHECK(hipMemcpyAsync(c->io_a.p, conf, (size_t)n * (7 + T) * 4, hipMemcpyHostToDevice, c->stream));
+      d_conf = c->io_a.p; d_e = c->io_b.p; d_g = c->io_c.p;
+    }
+    EvalArgs a{};
+    a.blob = c->blob.p; a.header = c->headers.p; a.host_header = pk.headers[0];
+    a.K = make_evalk(c, v_intra, v_inter, slope);
+    a.conf = d_conf; a.n_conf = n; a.e = d_e; a.grad = d_g;
+    const long long per_block = 64 / G;
+    const int blocks = (int)std::min<long long>((n + per_block - 1) / per_block, (long long)c->n_cu * 16);
+    HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+    HIP_CHECK(vdl::launch_eval(G, a, blocks, c->stream));
+    HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+    if (!device_ptrs) {
+      HIP_CHECK(hipMemcpyAsync(e, d_e, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+      HIP_CHECK(hipMemcpyAsync(grad, d_g, (size_t)n * (6 + T) * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->eval_ms = elapsed_ms(c);
+  });
+}
+
+double vd_eval_last_ms(const vd_ctx* c) { return c ? c->eval_ms : 0.0; }
+
+int vd_dock_params_default(vd_dock_params* p) {
+  if (!p) return VD_ERR_ARG;
+  std::memset(p, 0, sizeof(*p));
+  p->chains = 8192; p->search_depth = 0; p->bfgs_steps = 0;
+  p->local_search = VD_LS_RILC_BFGS; p->policy = VD_MC_VINA; p->mutation_quirk = 0; p->chains_per_wave = 0;
+  p->seed = 42; p->temperature = 1.2f; p->amplitude = 2.0f;
+  p->v_intra = 10.0f; p->v_inter = 10.0f; p->slope = 1e6f;
+  return VD_OK;
+}
+
+int vd_dock_heuristics(int n_atoms, int n_torsions, int rilc, int* depth, int* bfgs) {
+  if (n_atoms < 0 || n_torsions < 0) return VD_ERR_ARG;
+  if (depth) *depth = heur_depth(n_atoms, n_torsions, rilc);
+  if (bfgs) *bfgs = heur_bfgs(n_atoms, rilc);
+  return VD_OK;
+}
+
+int vd_dock(vd_ctx* c, int id, const vd_dock_params* p, float* e, float* conf, float* heavy, vd_stats* st) {
+  if (!c) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    need_device(c);
+    check_params(p);
+    if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no maps");
+    if (st) std::memset(st, 0, sizeof(*st));
+    LigPlan pl = plan_for(c, id, p);
+    prepare_batch(c, {pl});
+    std::vector<int> dv{pl.depth}, bv{pl.bfgs};
+    Packed pk = pack(c, {id}, &dv, &bv);
+    const int cpw = 64 / pl.G;
+    std::vector<int2> items;
+    for (int c0 = 0; c0 < p->chains; c0 += cpw) items.push_back(make_int2(0, c0));
+    run_mc(c, pl.G, pk, {(uint32_t)id}, items, c->batch.off, make_consts(c, p, pl.depth, pl.bfgs), c->batch.e.p,
+           c->batch.conf.p, c->batch.heavy.p, false, st, true);
+    if (e) HIP_CHECK(hipMemcpy(e, c->batch.e.p, (size_t)p->chains * 4, hipMemcpyDeviceToHost));
+    if (conf) HIP_CHECK(hipMemcpy(conf, c->batch.conf.p, (size_t)p->chains * (7 + pl.T) * 4, hipMemcpyDeviceToHost));
+    if (heavy) HIP_CHECK(hipMemcpy(heavy, c->batch.heavy.p, (size_t)p->chains * pl.H * 12, hipMemcpyDeviceToHost));
+  });
+}
+
+static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, vd_stats* st, bool persistent) {
+  if (!c || n < 0 || (n > 0 && !ids)) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    need_device(c);
+    check_params(p);
+    if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no maps");
+    if (st) std::memset(st, 0, sizeof(*st));
+    std::vector<LigPlan> plans;
+    for (int k = 0; k < n; ++k) plans.push_back(plan_for(c, ids[k], p));
+    prepare_batch(c, plans);
+    if (n == 0) return;
+    if (!persistent) {
+      // the plain per-ligand loop: one launch per ligand
+      for (int k = 0; k < n; ++k) {
+        const LigPlan& pl = plans[k];
+        std::vector<int> dv{pl.depth}, bv{pl.bfgs};
+        Packed pk = pack(c, {pl.id}, &dv, &bv);
+        const int cpw = 64 / pl.G;
+        std::vector<int2> items;
+        for (int c0 = 0; c0 < p->chains; c0 += cpw) items.push_back(make_int2(0, c0));
+        std::vector<longlong4> off{c->batch.off[k]};
+        run_mc(c, pl.G, pk, {(uint32_t)pl.id}, items, off, make_consts(c, p, pl.depth, pl.bfgs), c->batch.e.p,
+               c->batch.conf.p, c->batch.heavy.p, false, st, true);
+      }
+      return;
+    }
+    // one persistent launch per group width; items ordered by decreasing estimated cost
+    std::map<int, std::vector<int>> classes;
+    for (int k = 0; k < n; ++k) classes[plans[k].G].push_back(k);
+    for (auto& cl : classes) {
+      const int G = cl.first;
+      std::vector<int>& ks = cl.second;
+      std::vector<int> lig_ids, dv, bv;
+      for (int k : ks) { lig_ids.push_back(plans[k].id); dv.push_back(plans[k].depth); bv.push_back(plans[k].bfgs); }
+      Packed pk = pack(c, lig_ids, &dv, &bv);
+      std::vector<uint32_t> streams;
+      std::vector<longlong4> off;
+      for (int k : ks) { streams.push_back((uint32_t)plans[k].id); off.push_back(c->batch.off[k]); }
+      // cost estimate per chain: steps x iterations x (atoms + pairs)
+      std::vector<std::pair<double, int>> order;
+      for (size_t s = 0; s < ks.size(); ++s)
+        order.push_back({-(double)dv[s] * bv[s] * (pk.headers[s].n_atoms + 0.5 * pk.headers[s].n_pairs + 16), (int)s});
+      std::stable_sort(order.begin(), order.end());
+      const int cpw = 64 / G;
+      std::vector<int2> items;
+      for (auto& o : order)
+        for (int c0 = 0; c0 < p->chains; c0 += cpw) items.push_back(make_int2(o.second, c0));
+      run_mc(c, G, pk, streams, items, off, make_consts(c, p, 0, 0), c->batch.e.p, c->batch.conf.p, c->batch.heavy.p,
+             true, st, true);
+    }
+  });
+}
+
+int vd_dock_batch(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, vd_stats* st) {
+  return dock_many(c, n, ids, p, st, true);
+}
+int vd_dock_batch_plain(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, vd_stats* st) {
+  return dock_many(c, n, ids, p, st, false);
+}
+
+int vd_batch_results(vd_ctx* c, int k, float* e, float* conf, float* heavy) {
+  if (!c) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    need_device(c);
+    BatchState& b = c->batch;
+    if (k < 0 || k >= (int)b.ids.size()) throw ArgError(VD_ERR_ARG, "batch index out of range");
+    const longlong4 o = b.off[k];
+    const size_t ch = b.chains[k];
+    if (e) HIP_CHECK(hipMemcpy(e, b.e.p + o.x, ch * 4, hipMemcpyDeviceToHost));
+    if (conf) HIP_CHECK(hipMemcpy(conf, b.conf.p + o.y, ch * (7 + b.T[k]) * 4, hipMemcpyDeviceToHost));
+    if (heavy) HIP_CHECK(hipMemcpy(heavy, b.heavy.p + o.z, ch * b.H[k] * 12, hipMemcpyDeviceToHost));
+  });
+}
+
+int vd_batch_best(vd_ctx* c, float* best_e) {
+  if (!c || !best_e) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    need_device(c);
+    BatchState& b = c->batch;
+    size_t tot = 0;
+    for (int ch : b.chains) tot += ch;
+    std::vector<float> e(tot);
+    if (tot) HIP_CHECK(hipMemcpy(e.data(), b.e.p, tot * 4, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < b.ids.size(); ++k) {
+      float m = INFINITY;
+      for (int i = 0; i < b.chains[k]; ++i) m = std::min(m, e[b.off[k].x + i]);
+      best_e[k] = m;
+    }
+  });
+}
+
+int vd_cluster(int n, int n_heavy, const float* e, const float* heavy, double min_rmsd, int max_size, int* kept) {
+  if (n < 0 || n_heavy < 0 || (n > 0 && (!e || !heavy || !kept)) || max_size < 1) return VD_ERR_ARG;
+  std::vector<int> k = vd::cluster_poses(n, n_heavy, e, heavy, min_rmsd, max_size);
+  for (size_t i = 0; i < k.size(); ++i) kept[i] = k[i];
+  return (int)k.size();
+}
+
+int vd_synth_receptor(int n_atoms, double cube, double cavity_radius, uint64_t seed, float* xyz, int* xs) {
+  if (n_atoms < 0 || !xyz || !xs || !(cube > 0)) return VD_ERR_ARG;
+  std::vector<float> x; std::vector<int> t;
+  vd::synth_receptor(n_atoms, cube, cavity_radius, seed, x, t);
+  std::memcpy(xyz, x.data(), x.size() * 4);
+  std::memcpy(xs, t.data(), t.size() * 4);
+  return (int)t.size();
+}
+
+int vd_synth_ligand(int n_heavy, int n_rot, uint64_t seed, char* out, int cap) {
+  if (n_heavy < 1 || n_rot < 0) return VD_ERR_ARG;
+  std::string s = vd::synth_ligand_pdbqt(n_heavy, n_rot, seed);
+  if (out && (int)s.size() < cap) { std::memcpy(out, s.c_str(), s.size() + 1); }
+  return (int)s.size() + 1;
+}
+
+}  // extern "C"

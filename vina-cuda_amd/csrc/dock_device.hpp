@@ -1,0 +1,469 @@
+// dock_device.hpp -- device code of the docking hot path for gfx950 (wave64).
+//
+// One chain is owned by a group of G lanes (G = 64, 32 or 16; 64/G chains per wavefront).
+// A workgroup is exactly one wavefront, so __syncthreads() is a wave-local LDS fence.
+// Every collective (group shuffles, evaluations, fences) is executed in wave-uniform control
+// flow; per-chain decisions only predicate register updates.  This replaces the reference's
+// 32-thread blocks with shared-memory reductions (V11/inc/cuda/quasi_newton.cuh:352-428,
+// 905-975, 1102-1180, 1510-1551).
+//
+// Lane layout of a chain (G lanes, lane l):
+//   kinematics / adjoint: lane k < N owns rigid node k (preorder, node 0 = ROOT)
+//   atoms:                atom i is handled by lane i % G
+//   pairs:                pair p by lane p % G
+//   change vector (6+T):  torsion k-1 in lane k (1 <= k <= T); position x,y,z and
+//                         orientation x,y,z in lanes G-6 .. G-1; other lanes hold 0
+//   conformation:         position and orientation quaternion replicated in all lanes of the
+//                         group, torsion k-1 in lane k.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "vd_common.h"
+
+namespace vdk {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kEpsFl = 2.220446049250313e-16f;   // epsilon_fl as the reference passes it to the device
+
+// ----------------------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11): the random stream shared with the host oracle.
+struct U4 { uint32_t x, y, z, w; };
+__device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+  }
+  return U4{c0, c1, c2, c3};
+}
+__device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+
+// ----------------------------------------------------------------------------------------
+// Group collectives (G divides 64; xor masks < G stay inside the group)
+template <int G> __device__ __forceinline__ float gsum(float v) {
+#pragma unroll
+  for (int m = G / 2; m > 0; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+template <int G> __device__ __forceinline__ float gget(float v, int src) { return __shfl(v, src, G); }
+template <int G> __device__ __forceinline__ int ggeti(int v, int src) { return __shfl(v, src, G); }
+
+// ----------------------------------------------------------------------------------------
+// Rotations.  Quaternions are (w, x, y, z); products are Hamilton products.
+struct Quat { float w, x, y, z; };
+__device__ __forceinline__ Quat qmul(const Quat& a, const Quat& b) {
+  return Quat{a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z,
+              a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+              a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+              a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w};
+}
+__device__ __forceinline__ Quat qnormalize(Quat q) {
+  const float s = q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z;
+  const float inv = rsqrtf(s);
+  return Quat{q.w * inv, q.x * inv, q.y * inv, q.z * inv};
+}
+// rotation matrix, rows r0 r1 r2 (V11/lib/quaternion.cpp:155-185)
+__device__ __forceinline__ void qmat(const Quat& q, float m[9]) {
+  const float aa = q.w * q.w, ab = q.w * q.x, ac = q.w * q.y, ad = q.w * q.z, bb = q.x * q.x, bc = q.x * q.y,
+              bd = q.x * q.z, cc = q.y * q.y, cd = q.y * q.z, dd = q.z * q.z;
+  m[0] = aa + bb - cc - dd; m[1] = 2 * (-ad + bc);    m[2] = 2 * (ac + bd);
+  m[3] = 2 * (ad + bc);     m[4] = aa - bb + cc - dd; m[5] = 2 * (-ab + cd);
+  m[6] = 2 * (-ac + bd);    m[7] = 2 * (ab + cd);     m[8] = aa - bb - cc + dd;
+}
+__device__ __forceinline__ float3 qrot(const Quat& q, float3 v) {
+  float m[9]; qmat(q, m);
+  return make_float3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+                     m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
+// V11/lib/common.h:316-340
+__device__ __forceinline__ float normalize_angle(float x) {
+  if (!(x >= -3 * kPi && x <= 3 * kPi)) {
+    if (x > 3 * kPi) x -= 2 * kPi * ceilf((x - kPi) / (2 * kPi));
+    else if (x < -3 * kPi) x += 2 * kPi * ceilf((-x - kPi) / (2 * kPi));
+  }
+  if (x > kPi) x -= 2 * kPi;
+  else if (x < -kPi) x += 2 * kPi;
+  return x;
+}
+// V11/lib/quaternion.cpp:65-75
+__device__ __forceinline__ Quat axis_angle(float3 a, float angle) {
+  angle = normalize_angle(angle);
+  float s, c; __sincosf(0.5f * angle, &s, &c);
+  return Quat{c, s * a.x, s * a.y, s * a.z};
+}
+// quaternion_increment (V11/lib/quaternion.cpp:87-95, 235-240)
+__device__ __forceinline__ Quat qincrement(Quat q, float rx, float ry, float rz) {
+  const float ang = sqrtf(rx * rx + ry * ry + rz * rz);
+  if (ang > kEpsFl) {
+    const float inv = 1.0f / ang;
+    q = qmul(axis_angle(make_float3(rx * inv, ry * inv, rz * inv), ang), q);
+  }
+  return qnormalize(q);
+}
+
+// ----------------------------------------------------------------------------------------
+struct LigView {               // one ligand, staged in LDS for the wave
+  const float4* atom;          // A: local xyz + meta (node | xs << 8 | heavy << 16)
+  const float4* node_o;        // N: rel_origin xyz + parent
+  const float4* node_a;        // N: rel_axis xyz + subtree_end
+  const uint32_t* pairs;       // P, in global memory (L2 resident)
+  int A, N, P, T, H, rounds;
+};
+struct Scratch {               // per chain, LDS
+  float* xyz;                  // 3A, coordinates relative to the root origin
+  float* frc;                  // 3A, dE/dx
+  float4* frame;               // 3N rows (R row, origin component)
+  float4* nacc;                // 2N: per-node force and moment sums
+};
+struct EvalK {
+  const float* __restrict__ maps; vd_grid_desc gd;
+  const float4* __restrict__ lutq;
+  float v_intra, v_inter, slope;
+};
+struct Conf {                  // register view of one conformation
+  float px, py, pz; Quat q; float tor;
+};
+
+__device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+
+// Trilinear map value and derivative at p (grid::evaluate_aux, V11/lib/grid.cpp:50-167).
+// prel = position relative to the map origin.  NaN/inf inputs stay in bounds.
+__device__ __forceinline__ float map_eval(const float* __restrict__ m, const vd_grid_desc& gd, float3 prel,
+                                          float slope, float v, float3& deriv) {
+  float s[3] = {prel.x * gd.factor[0], prel.y * gd.factor[1], prel.z * gd.factor[2]};
+  float miss[3]; int region[3], a[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (s[i] >= 0.0f && s[i] < gd.dim_fl_minus_1[i]) {
+      region[i] = 0; a[i] = (int)s[i]; s[i] -= (float)a[i]; miss[i] = 0.0f;
+      a[i] = min(a[i], gd.dim[i] - 2);
+    } else if (s[i] >= gd.dim_fl_minus_1[i]) {
+      miss[i] = s[i] - gd.dim_fl_minus_1[i]; region[i] = 1; a[i] = gd.dim[i] - 2; s[i] = 1.0f;
+    } else {
+      miss[i] = -s[i]; region[i] = -1; a[i] = 0; s[i] = 0.0f;
+    }
+  }
+  const float penalty = slope * (miss[0] * gd.factor_inv[0] + miss[1] * gd.factor_inv[1] + miss[2] * gd.factor_inv[2]);
+  const int d0 = gd.dim[0], d01 = gd.dim[0] * gd.dim[1];
+  const float* b = m + (a[0] + d0 * (a[1] + gd.dim[1] * a[2]));
+  const float f000 = b[0], f100 = b[1], f010 = b[d0], f110 = b[d0 + 1];
+  const float f001 = b[d01], f101 = b[d01 + 1], f011 = b[d01 + d0], f111 = b[d01 + d0 + 1];
+  const float x = s[0], y = s[1], z = s[2], mx = 1 - x, my = 1 - y, mz = 1 - z;
+  float f = f000 * mx * my * mz + f100 * x * my * mz + f010 * mx * y * mz + f110 * x * y * mz +
+            f001 * mx * my * z + f101 * x * my * z + f011 * mx * y * z + f111 * x * y * z;
+  float gx = (f100 - f000) * my * mz + (f110 - f010) * y * mz + (f101 - f001) * my * z + (f111 - f011) * y * z;
+  float gy = (f010 - f000) * mx * mz + (f110 - f100) * x * mz + (f011 - f001) * mx * z + (f111 - f101) * x * z;
+  float gz = (f001 - f000) * mx * my + (f101 - f100) * x * my + (f011 - f010) * mx * y + (f111 - f110) * x * y;
+  if (f > 0.0f) {  // curl (V11/lib/curl.h:31-38)
+    const float t = (v < kEpsFl) ? 0.0f : v / (v + f);
+    f *= t; const float t2 = t * t; gx *= t2; gy *= t2; gz *= t2;
+  }
+  deriv.x = gd.factor[0] * (region[0] == 0 ? gx : 0.0f) + slope * region[0];
+  deriv.y = gd.factor[1] * (region[1] == 0 ? gy : 0.0f) + slope * region[1];
+  deriv.z = gd.factor[2] * (region[2] == 0 ? gz : 0.0f) + slope * region[2];
+  return f + penalty;
+}
+
+// Forward kinematics (tree.h:205-209, 282-292).  Node transforms compose along the tree as
+// q_k = q_parent (x) axis_angle(rel_axis_k, theta_k), o_k = o_parent + R_parent rel_origin_k,
+// which equals the reference's lab-frame update because axis_angle(R a, t) R = R axis_angle(a, t).
+// The per-path products are formed by pointer jumping in ceil(log2(depth+1)) rounds.
+// Writes frame rows to LDS; returns the node's lab-frame axis in `ax` and origin in `org`.
+template <int G>
+__device__ __forceinline__ void kinematics(const LigView& L, const Scratch& S, const Conf& c, int l,
+                                           float3& ax, float3& org) {
+  Quat q{1, 0, 0, 0}; float3 t = make_float3(0, 0, 0); int anc = -1;
+  float3 rel_axis = make_float3(0, 0, 0);
+  if (l == 0) { q = c.q; }
+  else if (l < L.N) {
+    const float4 no = L.node_o[l], na = L.node_a[l];
+    rel_axis = make_float3(na.x, na.y, na.z);
+    q = axis_angle(rel_axis, c.tor);
+    t = make_float3(no.x, no.y, no.z);
+    anc = (int)fbits(no.w);
+  }
+  for (int r = 0; r < L.rounds; ++r) {
+    const int src = anc >= 0 ? anc : l;
+    const Quat qa{gget<G>(q.w, src), gget<G>(q.x, src), gget<G>(q.y, src), gget<G>(q.z, src)};
+    const float3 ta = make_float3(gget<G>(t.x, src), gget<G>(t.y, src), gget<G>(t.z, src));
+    const int aa = ggeti<G>(anc, src);
+    if (anc >= 0) {
+      const float3 rt = qrot(qa, t);
+      t = make_float3(ta.x + rt.x, ta.y + rt.y, ta.z + rt.z);
+      q = qmul(qa, q);
+      anc = aa;
+    }
+  }
+  q = qnormalize(q);
+  float m[9]; qmat(q, m);
+  org = t;
+  ax = make_float3(m[0] * rel_axis.x + m[1] * rel_axis.y + m[2] * rel_axis.z,
+                   m[3] * rel_axis.x + m[4] * rel_axis.y + m[5] * rel_axis.z,
+                   m[6] * rel_axis.x + m[7] * rel_axis.y + m[8] * rel_axis.z);
+  if (l < L.N) {
+    S.frame[3 * l + 0] = make_float4(m[0], m[1], m[2], t.x);
+    S.frame[3 * l + 1] = make_float4(m[3], m[4], m[5], t.y);
+    S.frame[3 * l + 2] = make_float4(m[6], m[7], m[8], t.z);
+    S.nacc[2 * l + 0] = make_float4(0, 0, 0, 0);
+    S.nacc[2 * l + 1] = make_float4(0, 0, 0, 0);
+  }
+}
+
+__device__ __forceinline__ float3 atom_position(const LigView& L, const Scratch& S, int i, uint32_t& meta) {
+  const float4 a = L.atom[i];
+  meta = fbits(a.w);
+  const int nd = meta & 255;
+  const float4 r0 = S.frame[3 * nd], r1 = S.frame[3 * nd + 1], r2 = S.frame[3 * nd + 2];
+  return make_float3(r0.x * a.x + r0.y * a.y + r0.z * a.z + r0.w, r1.x * a.x + r1.y * a.y + r1.z * a.z + r1.w,
+                     r2.x * a.x + r2.y * a.y + r2.z * a.z + r2.w);
+}
+
+// Coordinates only (model::set).  Leaves xyz (relative to the root origin) in LDS.
+template <int G>
+__device__ void set_coords(const LigView& L, const Scratch& S, const Conf& c, int l) {
+  float3 ax, org;
+  kinematics<G>(L, S, c, l, ax, org);
+  __syncthreads();
+  for (int i = l; i < L.A; i += G) {
+    uint32_t meta; const float3 y = atom_position(L, S, i, meta);
+    S.xyz[3 * i] = y.x; S.xyz[3 * i + 1] = y.y; S.xyz[3 * i + 2] = y.z;
+  }
+  __syncthreads();
+}
+
+// Radius of gyration of heavy atoms about the root origin (model::gyration_radius,
+// V11/lib/model.cpp:1161-1175); call after set_coords.
+template <int G>
+__device__ float gyration_radius(const LigView& L, const Scratch& S, int l) {
+  float acc = 0.0f, cnt = 0.0f;
+  for (int i = l; i < L.A; i += G) {
+    const uint32_t meta = fbits(L.atom[i].w);
+    if ((meta >> 16) & 1u) { acc += S.xyz[3 * i] * S.xyz[3 * i] + S.xyz[3 * i + 1] * S.xyz[3 * i + 1] + S.xyz[3 * i + 2] * S.xyz[3 * i + 2]; cnt += 1.0f; }
+  }
+  acc = gsum<G>(acc); cnt = gsum<G>(cnt);
+  return cnt > 0.0f ? sqrtf(acc / cnt) : 0.0f;
+}
+
+// Energy and gradient of one conformation per group (model::eval_deriv,
+// V11/lib/model.cpp:1324-1334; device form quasi_newton.cuh:1294-1339).
+// Returns e (group-uniform); g = this lane's gradient component.
+template <int G>
+__device__ float eval_deriv(const LigView& L, const Scratch& S, const EvalK& K, const Conf& c, int l, float& g) {
+  float3 ax, org;
+  kinematics<G>(L, S, c, l, ax, org);
+  __syncthreads();
+  // intermolecular term: trilinear maps (cache::eval_deriv, V11/lib/cache.cpp:73-88)
+  const float3 prel = make_float3(c.px - K.gd.init[0], c.py - K.gd.init[1], c.pz - K.gd.init[2]);
+  float e_inter = 0.0f;
+  for (int i = l; i < L.A; i += G) {
+    uint32_t meta; const float3 y = atom_position(L, S, i, meta);
+    S.xyz[3 * i] = y.x; S.xyz[3 * i + 1] = y.y; S.xyz[3 * i + 2] = y.z;
+    const int xs = (meta >> 8) & 255;
+    float3 d = make_float3(0, 0, 0);
+    if (xs < VD_XS_SIZE)
+      e_inter += map_eval(K.maps + (size_t)xs * K.gd.plane, K.gd, make_float3(prel.x + y.x, prel.y + y.y, prel.z + y.z),
+                          K.slope, K.v_inter, d);
+    S.frc[3 * i] = d.x; S.frc[3 * i + 1] = d.y; S.frc[3 * i + 2] = d.z;
+  }
+  __syncthreads();
+  // intramolecular pairs (eval_interacting_pairs_deriv, V11/lib/model.cpp:1241-1259)
+  float e_intra = 0.0f;
+  for (int p = l; p < L.P; p += G) {
+    const uint32_t w = L.pairs[p];
+    const int a = w & 255, b = (w >> 8) & 255, tp = w >> 16;
+    const float rx = S.xyz[3 * b] - S.xyz[3 * a], ry = S.xyz[3 * b + 1] - S.xyz[3 * a + 1], rz = S.xyz[3 * b + 2] - S.xyz[3 * a + 2];
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    if (r2 < VD_CUTOFF_SQR) {
+      const float rf = VD_LUT_FACTOR * r2;
+      const int i1 = (int)rf;
+      const float rem = rf - (float)i1;
+      const float4 q = K.lutq[tp * VD_LUT_BINS + i1];
+      float e = q.x + rem * (q.z - q.x);
+      float dor = q.y + rem * (q.w - q.y);
+      if (e > 0.0f) { const float t = (K.v_intra < kEpsFl) ? 0.0f : K.v_intra / (K.v_intra + e); e *= t; dor *= t * t; }
+      e_intra += e;
+      const float fx = dor * rx, fy = dor * ry, fz = dor * rz;
+      atomicAdd(&S.frc[3 * a], -fx); atomicAdd(&S.frc[3 * a + 1], -fy); atomicAdd(&S.frc[3 * a + 2], -fz);
+      atomicAdd(&S.frc[3 * b], fx); atomicAdd(&S.frc[3 * b + 1], fy); atomicAdd(&S.frc[3 * b + 2], fz);
+    }
+  }
+  __syncthreads();
+  // adjoint: per-node force and moment (about the root origin) sums
+  for (int i = l; i < L.A; i += G) {
+    const uint32_t meta = fbits(L.atom[i].w);
+    if (((meta >> 8) & 255) >= VD_XS_SIZE) continue;   // untyped atoms carry no force
+    const float fx = S.frc[3 * i], fy = S.frc[3 * i + 1], fz = S.frc[3 * i + 2];
+    const float x = S.xyz[3 * i], y = S.xyz[3 * i + 1], z = S.xyz[3 * i + 2];
+    float* acc = reinterpret_cast<float*>(&S.nacc[2 * (meta & 255)]);
+    atomicAdd(acc + 0, fx); atomicAdd(acc + 1, fy); atomicAdd(acc + 2, fz);
+    atomicAdd(acc + 4, y * fz - z * fy); atomicAdd(acc + 5, z * fx - x * fz); atomicAdd(acc + 6, x * fy - y * fx);
+  }
+  __syncthreads();
+  // subtree sums: the subtree of node k is the preorder range [k, subtree_end_k); the torque
+  // about o_k is M - o_k x F (tree.h:341-348, 373-380; torsion derivative = torque . axis)
+  float F[3] = {0, 0, 0}, M[3] = {0, 0, 0};
+  if (l < L.N) {
+    const int end = (int)fbits(L.node_a[l].w);
+    for (int j = l; j < end; ++j) {
+      const float4 f = S.nacc[2 * j], m = S.nacc[2 * j + 1];
+      F[0] += f.x; F[1] += f.y; F[2] += f.z; M[0] += m.x; M[1] += m.y; M[2] += m.z;
+    }
+  }
+  const float tx = M[0] - (org.y * F[2] - org.z * F[1]);
+  const float ty = M[1] - (org.z * F[0] - org.x * F[2]);
+  const float tz = M[2] - (org.x * F[1] - org.y * F[0]);
+  const float gt = tx * ax.x + ty * ax.y + tz * ax.z;
+  const float r0 = gget<G>(F[0], 0), r1 = gget<G>(F[1], 0), r2 = gget<G>(F[2], 0);
+  const float r3 = gget<G>(tx, 0), r4 = gget<G>(ty, 0), r5 = gget<G>(tz, 0);
+  const int c6 = l - (G - 6);
+  g = (l >= 1 && l <= L.T) ? gt
+      : c6 == 0 ? r0 : c6 == 1 ? r1 : c6 == 2 ? r2 : c6 == 3 ? r3 : c6 == 4 ? r4 : c6 == 5 ? r5 : 0.0f;
+  return gsum<G>(e_inter + e_intra);
+}
+
+// x <- x (+) factor*d (output_type_cl_increment, V11/inc/cuda/mutate_conf.cuh:8-29;
+// conf.h:60-66, 118-124).  d is the lane's change component.
+template <int G>
+__device__ __forceinline__ Conf increment(const Conf& x, float d, float factor, int l, int T) {
+  Conf y = x;
+  const float d0 = gget<G>(d, G - 6), d1 = gget<G>(d, G - 5), d2 = gget<G>(d, G - 4);
+  const float d3 = gget<G>(d, G - 3), d4 = gget<G>(d, G - 2), d5 = gget<G>(d, G - 1);
+  y.px += factor * d0; y.py += factor * d1; y.pz += factor * d2;
+  y.q = qincrement(y.q, factor * d3, factor * d4, factor * d5);
+  if (l >= 1 && l <= T) y.tor = normalize_angle(y.tor + normalize_angle(factor * d));
+  return y;
+}
+
+__device__ __forceinline__ Conf csel(bool p, const Conf& a, const Conf& b) { return p ? a : b; }
+
+// ----------------------------------------------------------------------------------------
+// RILC-BFGS local search with the Lewis-Overton line search
+// (V11/inc/cuda/quasi_newton.cuh:1710-1806, 2172-2389), run by every group of the wave in
+// lockstep.  Groups with enabled == false evaluate harmlessly and keep x unchanged.
+// Dot products are exact group sums over all 6+T components.
+template <int G>
+__device__ float rilc_bfgs(const LigView& L, const Scratch& S, const EvalK& K, Conf& x, bool enabled,
+                           float x_e, int max_steps, int l, int& evals) {
+  float g;
+  float fx = eval_deriv<G>(L, S, K, x, l, g);
+  evals += enabled ? 1 : 0;
+  const float fx_orig = fx; const Conf x_orig = x;
+  float d = -g;
+  bool active = enabled && (sqrtf(gsum<G>(g * g)) >= 1e-5f);
+  float step = 1.0f / sqrtf(gsum<G>(d * d));
+  int k = 1;
+  while (__any(active)) {
+    const Conf xp = x; const float gp = g; const float fxp = fx;
+    // line search
+    const float dginit = gsum<G>(gp * d);
+    bool ls = active && !(0.0f < dginit);
+    int result = -1;
+    const float finit = fx, dgtest = 1e-4f * dginit, dstest = 0.1f * dginit;
+    float mu = 0.0f, nu = 1.0e20f; bool brackt = false; int count = 0;
+    while (__any(ls)) {
+      const Conf xt = increment<G>(xp, d, step, l, L.T);
+      float gt;
+      const float ft = eval_deriv<G>(L, S, K, csel(ls, xt, x), l, gt);
+      const float gtd = gsum<G>(gt * d);
+      if (ls) {
+        x = xt; fx = ft; g = gt; ++count; ++evals;
+        bool done = false;
+        if (fx > finit + step * dgtest) { nu = step; brackt = true; }
+        else if (gtd < dstest) { mu = step; }
+        else { result = count; done = true; }
+        if (!done && count >= 10) { result = (fx > finit) ? -1 : count; done = true; }
+        if (!done) step = brackt ? 0.5f * (mu + nu) : 2.0f * step;
+        if (done) ls = false;
+      }
+    }
+    const float gnorm = sqrtf(gsum<G>(g * g));
+    bool update = false;
+    if (active) {
+      if (result < 0) { x = xp; g = gp; fx = fxp; active = false; }
+      else if (!(gnorm >= 1e-5f)) active = false;
+      else if ((max_steps != 0 && max_steps <= k) || k >= 1000) active = false;   // 1000: hard bound
+      else { ++k; update = true; }
+    }
+    // one-pair update of the direction (sign of the y term as in quasi_newton.cuh:2342-2350)
+    const float s = step * d, y = g - gp;
+    const float ys = gsum<G>(y * s), yy = gsum<G>(y * y), ss = gsum<G>(s * s), gpgp = gsum<G>(gp * gp);
+    const float nd = -g;
+    const float alpha = gsum<G>(s * nd) / ys;
+    const float nd2 = (nd + alpha * y) * (ys / yy);
+    const float beta = gsum<G>(y * nd2) / ys;
+    const float nd3 = nd2 + (alpha - beta) * s;
+    if (update) { d = (ys > ss * sqrtf(gpgp) * 1.0e-6f) ? nd3 : nd; step = 1.0f; }
+  }
+  if (enabled && !(fx <= fx_orig)) { fx = fx_orig; x = x_orig; }
+  return enabled ? fx : x_e;
+}
+
+// Dense BFGS with backtracking line search (V11/lib/bfgs.h:30-140; device form
+// quasi_newton.cuh:1859-1916, 2391-2499).  H (n x n, n = 6+T <= G) lives in LDS, row i in the
+// lane that owns change component i.
+template <int G>
+__device__ float dense_bfgs(const LigView& L, const Scratch& S, float* H, float* vbuf, const EvalK& K, Conf& x,
+                            bool enabled, float x_e, int max_steps, int l, int& evals) {
+  // H is G x G, lane l owns row l; vbuf holds G floats for broadcasting a vector
+  for (int j = 0; j < G; ++j) H[l * G + j] = (j == l) ? 1.0f : 0.0f;
+  auto matvec = [&](float v) {     // returns -(H v)_l
+    __syncthreads(); vbuf[l] = v; __syncthreads();
+    float acc = 0.0f;
+    for (int j = 0; j < G; ++j) acc += H[l * G + j] * vbuf[j];
+    __syncthreads();
+    return -acc;
+  };
+  float g;
+  float f0 = eval_deriv<G>(L, S, K, x, l, g);
+  evals += enabled ? 1 : 0;
+  const float f_orig = f0; const Conf x_orig = x;
+  bool active = enabled;
+  for (int step = 0; step < max_steps && __any(active); ++step) {
+    const float p = matvec(g);
+    const float pg = gsum<G>(p * g);
+    float alpha = 1.0f, f1 = f0, g_new = g; Conf x_new = x;
+    bool ls = active;
+    for (int trial = 0; trial < 10 && __any(ls); ++trial) {
+      const Conf xt = increment<G>(x, p, alpha, l, L.T);
+      float gt;
+      const float ft = eval_deriv<G>(L, S, K, csel(ls, xt, x), l, gt);
+      if (ls) {
+        x_new = xt; f1 = ft; g_new = gt; ++evals;
+        if (f1 - f0 < 1e-4f * alpha * pg) ls = false; else alpha *= 0.5f;
+      }
+    }
+    const float y = g_new - g;
+    const float gnorm = sqrtf(gsum<G>(g * g));
+    bool upd = false;
+    if (active) {
+      f0 = f1; x = x_new;
+      if (!(gnorm >= 1e-5f)) active = false;
+      else { g = g_new; upd = true; }
+    }
+    const float yy = gsum<G>(y * y), yp = gsum<G>(y * p);
+    if (step == 0 && upd && fabsf(yy) > kEpsFl) {
+      const float diag = alpha * yp / yy;
+      for (int j = 0; j < G; ++j) H[l * G + j] = (j == l) ? diag : H[l * G + j];
+    }
+    const float mhy = matvec(y);
+    const float yhy = -gsum<G>(y * mhy);
+    const float r = 1.0f / (alpha * yp);
+    const bool do_upd = upd && !(alpha * yp < kEpsFl);
+    // H_lj += alpha r (mhy_l p_j + mhy_j p_l) + alpha^2 (r^2 yhy + r) p_l p_j
+    float* pb = vbuf + G;
+    __syncthreads();
+    vbuf[l] = mhy; pb[l] = p;
+    __syncthreads();
+    if (do_upd)
+      for (int j = 0; j < G; ++j)
+        H[l * G + j] += alpha * r * (mhy * pb[j] + vbuf[j] * p) + alpha * alpha * (r * r * yhy + r) * p * pb[j];
+    __syncthreads();
+  }
+  if (enabled && !(f0 <= f_orig)) { f0 = f_orig; x = x_orig; }
+  return enabled ? f0 : x_e;
+}
+
+}  // namespace vdk

@@ -1,0 +1,85 @@
+// host_model.h -- host-side model building for libvdock.
+//
+// Builds what the HIP kernels consume: the FP32 look-up tables (from the FP64 scoring
+// function), typed receptor atoms, and packed ligand records (rigid-node tree in DFS
+// preorder, atoms with node-local coordinates, intramolecular pairs).  Follows the
+// reference's host code:
+//   V11/lib/precalculate.h:121-140, 255-339  (LUT)
+//   V11/lib/everything.cpp:97-216, 381-479   (terms)
+//   V11/lib/parse_pdbqt.cpp:101-118, 286-652 (PDBQT tree)
+//   V11/lib/model.cpp:562-896                (bonds, XS typing, pairs)
+//   V11/lib/main_procedure_cu.cpp:551-763    (flattening for the device)
+#pragma once
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include <stdexcept>
+#include "vd_common.h"
+
+namespace vd {
+
+struct parse_error : std::runtime_error {
+  explicit parse_error(const std::string& m) : std::runtime_error(m) {}
+};
+
+// XS typing helpers (V11/lib/atom_constants.h:209-282)
+extern const double kXsRadius[VD_XS_SIZE];
+bool xs_is_hydrophobic(int t);
+bool xs_is_acceptor(int t);
+bool xs_is_donor(int t);
+inline int type_pair_index(int a, int b) { if (a > b) { int t = a; a = b; b = t; } return a + b * (b + 1) / 2; }
+
+// Weighted sum of the five Vina terms for XS types t1, t2 at distance r.
+double score_terms(int t1, int t2, double r, const double w[5]);
+
+// FP64 tables of precalculate (smooth.first, smooth.second, fast) for all 153 pairs.
+struct Precalculated {
+  std::vector<double> smooth_e, smooth_dor, fast;   // [153][VD_LUT_BINS]
+  void build(const double w[5]);
+  // FP32 device layouts:
+  //   grid rows  [t1][bin][VD_GRID_ROW]: fast value for every t2 (zero padded)
+  //   pair quads [tp][bin] = (e_i, dor_i, e_{i+1}, dor_{i+1})
+  void grid_rows(std::vector<float>& out) const;
+  void pair_quads(std::vector<float>& out) const;
+};
+
+struct LigAtom {
+  int ad = 20, el = 11, xs = VD_XS_SIZE;
+  double xyz[3] = {0, 0, 0};     // absolute input coordinates
+  double local[3] = {0, 0, 0};   // relative to the owning node's origin
+  int node = 0;
+  unsigned serial = 0;
+  std::string line;              // the PDBQT line, for writing poses back out
+  std::vector<int> bonds;
+};
+struct LigNode {
+  int parent = -1, begin = 0, end = 0, subtree_end = 0, depth = 0;
+  double origin0[3] = {0, 0, 0}, rel_origin[3] = {0, 0, 0}, rel_axis[3] = {0, 0, 0};
+};
+struct Ligand {
+  std::vector<LigAtom> atoms;
+  std::vector<LigNode> nodes;      // preorder, node 0 = ROOT
+  std::vector<uint32_t> pairs;     // a | b << 8 | tp << 16
+  int torsdof = 0, n_heavy = 0, max_depth = 0;
+  std::vector<std::string> context; // REMARK/ROOT/BRANCH lines in file order (for output)
+  int n_torsions() const { return (int)nodes.size() - 1; }
+};
+
+Ligand parse_ligand_pdbqt(const std::string& text);
+// Typed receptor atoms (rigid receptor only): xyz[3n], xs[n] (17 = no XS type)
+void parse_receptor_pdbqt(const std::string& text, std::vector<float>& xyz, std::vector<int>& xs);
+
+// Appends a ligand's packed record to a device blob (16-byte words); returns the header.
+vd_lig_header pack_ligand(const Ligand& L, std::vector<float>& blob);
+
+// Synthetic inputs (recipes of the benchmark configurations).
+void synth_receptor(int n_atoms, double cube, double cavity_radius, uint64_t seed,
+                    std::vector<float>& xyz, std::vector<int>& xs);
+std::string synth_ligand_pdbqt(int n_heavy, int n_rot, uint64_t seed);
+
+// add_to_output_container over heavy-atom coordinates (V11/lib/coords.cpp:24-62).
+// Feeds results in the given order; returns indices kept, sorted by energy.
+std::vector<int> cluster_poses(int n, int heavy, const float* e, const float* heavy_xyz,
+                               double min_rmsd, int max_size);
+
+}  // namespace vd

@@ -1,0 +1,47 @@
+// kernels.h -- launch interface between the C ABI layer and the HIP kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "vd_common.h"
+#include "dock_device.hpp"
+
+// Receptor cell list for the grid kernel: atoms sorted by cell (x fastest), one (begin, end)
+// range per cell; cells of `cell` A cover the box grown by the cutoff, starting at `origin`.
+struct GridCells {
+  float origin[3];
+  float cell;
+  int n[3];
+};
+
+struct McArgs {
+  const float4* blob;
+  const vd_lig_header* headers;     // per ligand slot
+  const uint32_t* streams;          // random stream id per ligand slot
+  const int2* items;                // (ligand slot, first chain)
+  int n_items;
+  int* queue;                       // nullptr: item = blockIdx.x
+  vdk::EvalK K;
+  vd_mc_consts P;
+  float* out_e; float* out_conf; float* out_heavy;
+  const longlong4* out_off;         // per ligand slot: (e, conf, heavy) element offsets
+  unsigned long long* eval_count;
+  int Amax, Nmax;
+};
+
+struct EvalArgs {
+  const float4* blob;
+  const vd_lig_header* header;      // device copy
+  vd_lig_header host_header;
+  vdk::EvalK K;
+  const float* conf; long long n_conf;
+  float* e; float* grad;
+};
+
+namespace vdl {
+int mc_lds_bytes(int G, int Amax, int Nmax, int dense);
+int max_resident_mc_blocks(int G, int lds_bytes);
+hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& gc, const float4* lut_rows, float* maps,
+                       const vd_grid_desc& gd, hipStream_t s);
+hipError_t launch_eval(int G, const EvalArgs& a, int blocks, hipStream_t s);
+hipError_t launch_mc(int G, const McArgs& a, int blocks, hipStream_t s);
+hipError_t reset_counter(int* p, hipStream_t s);
+}  // namespace vdl

@@ -1,0 +1,340 @@
+// kernels.hip -- HIP kernels of libvdock for gfx950 and their launch wrappers.
+//
+//  vd_grid_kernel     17-map precompute (replaces kernel1/kernel_grid, V11/inc/cuda/kernel1.cu:93-249;
+//                     CPU semantics cache::populate, V11/lib/cache.cpp:148-208)
+//  vd_eval_kernel     energy + adjoint gradient of given conformations (m_eval_deriv_update,
+//                     V11/inc/cuda/quasi_newton.cuh:1294-1339; model::eval_deriv, model.cpp:1324-1334)
+//  vd_mc_kernel       Monte Carlo + local search chains (kernel2_update, V11/inc/cuda/kernel2.cu:219-564),
+//                     either one work item per workgroup (plain per-ligand launch) or a persistent
+//                     grid pulling (ligand, chain block) items from a device queue.
+#include <hip/hip_runtime.h>
+#include "dock_device.hpp"
+#include "kernels.h"
+
+using namespace vdk;
+
+namespace {
+
+// ------------------------------------------------------------------------------------------
+// Grid maps.  A workgroup of 256 lanes owns a 16x4x4 tile of points (x fastest); the receptor
+// atoms that can reach the tile come from a host-built cell list (cells of VD_GCELL A over the
+// box grown by the 8 A cutoff).  For every atom within the cutoff of a point, one 20-float row
+// of the map LUT gives the contribution to all 17 maps (rows indexed [t1][bin]).
+constexpr int TX = 16, TY = 4, TZ = 4;
+
+__global__ __launch_bounds__(256) void vd_grid_kernel(const float4* __restrict__ atoms, const int2* __restrict__ cells,
+                                                      GridCells gc, const float4* __restrict__ lut_rows,
+                                                      float* __restrict__ maps, vd_grid_desc gd, int tiles_x, int tiles_y) {
+  const int tile = blockIdx.x;
+  const int tx0 = (tile % tiles_x) * TX, ty0 = ((tile / tiles_x) % tiles_y) * TY, tz0 = (tile / (tiles_x * tiles_y)) * TZ;
+  const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
+  const int x = tx0 + lx, y = ty0 + ly, z = tz0 + lz;
+  const bool inside = x < gd.dim[0] && y < gd.dim[1] && z < gd.dim[2];
+  // probe coordinates as grid::index_to_argument (V11/lib/grid.h:49-53), relative to the cell origin
+  const float px = gd.init[0] + gd.factor_inv[0] * x - gc.origin[0];
+  const float py = gd.init[1] + gd.factor_inv[1] * y - gc.origin[1];
+  const float pz = gd.init[2] + gd.factor_inv[2] * z - gc.origin[2];
+  // tile bounds (relative), grown by the cutoff, -> cell range
+  const float lo_x = gd.init[0] + gd.factor_inv[0] * tx0 - gc.origin[0], hi_x = lo_x + gd.factor_inv[0] * (TX - 1);
+  const float lo_y = gd.init[1] + gd.factor_inv[1] * ty0 - gc.origin[1], hi_y = lo_y + gd.factor_inv[1] * (TY - 1);
+  const float lo_z = gd.init[2] + gd.factor_inv[2] * tz0 - gc.origin[2], hi_z = lo_z + gd.factor_inv[2] * (TZ - 1);
+  const float cut = 8.0f, inv = 1.0f / gc.cell;
+  const int cx0 = max(0, (int)floorf((lo_x - cut) * inv)), cx1 = min(gc.n[0] - 1, (int)floorf((hi_x + cut) * inv));
+  const int cy0 = max(0, (int)floorf((lo_y - cut) * inv)), cy1 = min(gc.n[1] - 1, (int)floorf((hi_y + cut) * inv));
+  const int cz0 = max(0, (int)floorf((lo_z - cut) * inv)), cz1 = min(gc.n[2] - 1, (int)floorf((hi_z + cut) * inv));
+  float acc[VD_XS_SIZE];
+#pragma unroll
+  for (int t = 0; t < VD_XS_SIZE; ++t) acc[t] = 0.0f;
+  for (int cz = cz0; cz <= cz1; ++cz)
+    for (int cy = cy0; cy <= cy1; ++cy) {
+      const int row = (cz * gc.n[1] + cy) * gc.n[0];
+      const int begin = cells[row + cx0].x, end = cells[row + cx1].y;   // cells of one x-row are contiguous
+      for (int i = begin; i < end; ++i) {
+        const float4 a = atoms[i];
+        // skip atoms farther than the cutoff from the whole tile box (wave-uniform test)
+        const float ddx = fmaxf(fmaxf(lo_x - a.x, a.x - hi_x), 0.0f), ddy = fmaxf(fmaxf(lo_y - a.y, a.y - hi_y), 0.0f),
+                    ddz = fmaxf(fmaxf(lo_z - a.z, a.z - hi_z), 0.0f);
+        if (ddx * ddx + ddy * ddy + ddz * ddz > VD_CUTOFF_SQR) continue;
+        const float dx = a.x - px, dy = a.y - py, dz = a.z - pz;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (inside && r2 <= VD_CUTOFF_SQR) {
+          const int t1 = (int)__float_as_uint(a.w);
+          const int bin = (int)(VD_LUT_FACTOR * r2);
+          const float4* rw = lut_rows + ((size_t)t1 * VD_LUT_BINS + bin) * (VD_GRID_ROW / 4);
+          const float4 v0 = rw[0], v1 = rw[1], v2 = rw[2], v3 = rw[3], v4 = rw[4];
+          acc[0] += v0.x; acc[1] += v0.y; acc[2] += v0.z; acc[3] += v0.w;
+          acc[4] += v1.x; acc[5] += v1.y; acc[6] += v1.z; acc[7] += v1.w;
+          acc[8] += v2.x; acc[9] += v2.y; acc[10] += v2.z; acc[11] += v2.w;
+          acc[12] += v3.x; acc[13] += v3.y; acc[14] += v3.z; acc[15] += v3.w;
+          acc[16] += v4.x;
+        }
+      }
+    }
+  if (inside) {
+    const size_t o = (size_t)x + (size_t)gd.dim[0] * ((size_t)y + (size_t)gd.dim[1] * z);
+#pragma unroll
+    for (int t = 0; t < VD_XS_SIZE; ++t) maps[(size_t)t * gd.plane + o] = acc[t];
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// LDS layout of one wave: ligand (atoms 4A, node_o 4N, node_a 4N floats), then per chain group
+// frame 12N, nacc 8N, xyz 3A (padded to 4), frc 3A (padded), and for dense BFGS H G*G + 2G.
+__host__ __device__ inline int pad4(int n) { return (n + 3) & ~3; }
+__host__ __device__ inline int lig_floats(int A, int N) { return 4 * A + 8 * N; }
+__host__ __device__ inline int group_floats(int A, int N, int G, int dense) {
+  return 12 * N + 8 * N + 2 * pad4(3 * A) + (dense ? G * G + 2 * G : 0);
+}
+
+template <int G>
+__device__ __forceinline__ void stage_ligand(const float4* __restrict__ blob, const vd_lig_header& h, float* lds, int lane) {
+  float4* dst = reinterpret_cast<float4*>(lds);
+  for (int i = lane; i < h.n_atoms; i += 64) dst[i] = blob[h.off_atoms + i];
+  for (int i = lane; i < 2 * h.n_nodes; i += 64) dst[h.n_atoms + i] = blob[h.off_nodes + i];
+}
+
+template <int G>
+__device__ __forceinline__ void make_views(const float4* __restrict__ blob, const vd_lig_header& h, float* lds, int Amax,
+                                           int Nmax, int dense, int group, LigView& L, Scratch& S, float*& H) {
+  float4* base = reinterpret_cast<float4*>(lds);
+  L.atom = base; L.node_o = base + h.n_atoms; L.node_a = base + h.n_atoms + h.n_nodes;
+  L.pairs = reinterpret_cast<const uint32_t*>(blob) + h.off_pairs;
+  L.A = h.n_atoms; L.N = h.n_nodes; L.P = h.n_pairs; L.T = h.n_torsions; L.H = h.n_heavy; L.rounds = h.jump_rounds;
+  float* g = lds + lig_floats(Amax, Nmax) + group * group_floats(Amax, Nmax, G, dense);
+  S.frame = reinterpret_cast<float4*>(g);
+  S.nacc = reinterpret_cast<float4*>(g + 12 * Nmax);
+  S.xyz = g + 20 * Nmax;
+  S.frc = S.xyz + pad4(3 * Amax);
+  H = S.frc + pad4(3 * Amax);
+}
+
+template <int G>
+__device__ float local_search(const LigView& L, const Scratch& S, float* H, const EvalK& K, int mode, Conf& x,
+                              bool enabled, float x_e, int max_steps, int l, int& evals) {
+  if (!__any(enabled)) return x_e;
+  if (mode == 1) return rilc_bfgs<G>(L, S, K, x, enabled, x_e, max_steps, l, evals);
+  return dense_bfgs<G>(L, S, H, H + G * G, K, x, enabled, x_e, max_steps, l, evals);
+}
+
+// Initial conformation of a chain (conf::randomize, V11/lib/conf.h:127-130, 214-217;
+// random_orientation, V11/lib/quaternion.cpp:206-219) from the shared random stream.
+template <int G>
+__device__ Conf initial_conf(const vd_mc_consts& P, uint32_t stream, uint32_t chain, int l, int T) {
+  Conf c;
+  const U4 w = philox(chain, VD_INIT_STEP, 0, 0, P.seed, stream);
+  c.px = P.box_lo[0] + (P.box_hi[0] - P.box_lo[0]) * u01(w.x);
+  c.py = P.box_lo[1] + (P.box_hi[1] - P.box_lo[1]) * u01(w.y);
+  c.pz = P.box_lo[2] + (P.box_hi[2] - P.box_lo[2]) * u01(w.z);
+  c.q = Quat{1, 0, 0, 0};
+  bool done = false;
+  for (uint32_t a = 0; a < 8 && !done; ++a) {
+    const U4 v = philox(chain, VD_INIT_STEP, 1, a, P.seed, stream);
+    const float r0 = sqrtf(-2.0f * logf(u01(v.x) + 1.0f / 16777216.0f)), r1 = sqrtf(-2.0f * logf(u01(v.z) + 1.0f / 16777216.0f));
+    float s0, c0, s1, c1;
+    sincosf(2.0f * kPi * u01(v.y), &s0, &c0); sincosf(2.0f * kPi * u01(v.w), &s1, &c1);
+    const float n0 = r0 * c0, n1 = r0 * s0, n2 = r1 * c1, n3 = r1 * s1;
+    const float nrm = sqrtf(n0 * n0 + n1 * n1 + n2 * n2 + n3 * n3);
+    if (nrm > 1e-6f) { c.q = Quat{n0 / nrm, n1 / nrm, n2 / nrm, n3 / nrm}; done = true; }
+  }
+  c.tor = 0.0f;
+  if (l >= 1 && l <= T) {
+    const int j = l - 1;
+    const U4 v = philox(chain, VD_INIT_STEP, 2 + (uint32_t)(j / 4), 0, P.seed, stream);
+    const uint32_t word = (j % 4 == 0) ? v.x : (j % 4 == 1) ? v.y : (j % 4 == 2) ? v.z : v.w;
+    c.tor = -kPi + 2.0f * kPi * u01(word);
+  }
+  return c;
+}
+
+// One MC chain per group with the reference GPU schedule (V11/inc/cuda/kernel2.cu:290-426):
+// mutate -> local search -> Metropolis(T) -> on accept, if better than the best so far: second
+// local search, keep the best.  Mutation as V11/lib/mutate.cpp:62-127 (see initial_conf for the
+// random stream); the orientation step uses the gyration radius of the current conformation.
+template <int G>
+__device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const EvalK& K, const vd_mc_consts& P,
+                         int depth, int bfgs_steps, uint32_t stream, uint32_t chain, bool valid, int l, float* out_e,
+                         float* out_conf, float* out_heavy, int& evals) {
+  const uint32_t k0 = P.seed, k1 = stream;
+  Conf tmp = initial_conf<G>(P, stream, chain, l, L.T);
+  float tmp_e = 0.0f;
+  set_coords<G>(L, S, tmp, l);
+  float rg = gyration_radius<G>(L, S, l);
+  float best_e = INFINITY;
+  Conf best = tmp;
+  const int n_ent = P.mutation_quirk ? (1 + L.T) : (2 + L.T);
+  for (int step = 0; step < depth; ++step) {
+    const U4 w = philox(chain, (uint32_t)step, 0, 0, k0, k1);
+    const int which = min((int)(u01(w.x) * (float)n_ent), n_ent - 1);
+    const float u_metro = u01(w.y), tor = -kPi + 2.0f * kPi * u01(w.z);
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    bool need = which <= 1;
+    for (uint32_t a = 0; a < 64 && __any(need); ++a) {   // P(64 rejections) ~ 1e-20
+      const U4 v = philox(chain, (uint32_t)step, 1, a, k0, k1);
+      const float vx = 2.0f * u01(v.x) - 1.0f, vy = 2.0f * u01(v.y) - 1.0f, vz = 2.0f * u01(v.z) - 1.0f;
+      if (need && vx * vx + vy * vy + vz * vz < 1.0f) { sx = vx; sy = vy; sz = vz; need = false; }
+    }
+    Conf cand = tmp;
+    if (which == 0) { cand.px += P.amplitude * sx; cand.py += P.amplitude * sy; cand.pz += P.amplitude * sz; }
+    else if (which == 1) { if (rg > kEpsFl) { const float f = P.amplitude / rg; cand.q = qincrement(cand.q, f * sx, f * sy, f * sz); } }
+    else if (l == which - 1) cand.tor = tor;
+    const float ce = local_search<G>(L, S, H, K, P.local_search, cand, valid, 0.0f, bfgs_steps, l, evals);
+    const bool accept = valid && (step == 0 || ce < tmp_e || u_metro < expf((tmp_e - ce) / P.temperature));
+    if (accept) { tmp = cand; tmp_e = ce; }
+    const bool second = accept && tmp_e < best_e;
+    const float e2 = local_search<G>(L, S, H, K, P.local_search, tmp, second, tmp_e, bfgs_steps, l, evals);
+    if (second) { tmp_e = e2; if (tmp_e < best_e) { best = tmp; best_e = tmp_e; } }
+    if (__any(accept)) {
+      set_coords<G>(L, S, tmp, l);
+      const float r = gyration_radius<G>(L, S, l);
+      if (accept) rg = r;
+    }
+  }
+  set_coords<G>(L, S, best, l);
+  if (valid) {
+    if (l == 0) {
+      out_e[chain] = best_e;
+      float* c = out_conf + (size_t)chain * (7 + L.T);
+      c[0] = best.px; c[1] = best.py; c[2] = best.pz; c[3] = best.q.w; c[4] = best.q.x; c[5] = best.q.y; c[6] = best.q.z;
+    }
+    if (l >= 1 && l <= L.T) out_conf[(size_t)chain * (7 + L.T) + 6 + l] = best.tor;
+    for (int i = l; i < L.A; i += G) {
+      const uint32_t meta = fbits(L.atom[i].w);
+      if ((meta >> 16) & 1u) {
+        const int slot = (meta >> 17) & 255;
+        float* o = out_heavy + ((size_t)chain * L.H + slot) * 3;
+        o[0] = best.px + S.xyz[3 * i]; o[1] = best.py + S.xyz[3 * i + 1]; o[2] = best.pz + S.xyz[3 * i + 2];
+      }
+    }
+  }
+}
+
+template <int G>
+__global__ __launch_bounds__(64) void vd_mc_kernel(McArgs a) {
+  extern __shared__ float4 lds4[];
+  float* lds = reinterpret_cast<float*>(lds4);
+  constexpr int CPW = 64 / G;
+  const int lane = threadIdx.x, group = lane / G, l = lane % G;
+  int cur = -1;
+  long long my_evals = 0;
+  for (int it = 0;; ++it) {
+    int item;
+    if (a.queue) {
+      int v = 0;
+      if (lane == 0) v = atomicAdd(a.queue, 1);
+      item = __shfl(v, 0);
+    } else {
+      item = (it == 0) ? (int)blockIdx.x : a.n_items;
+    }
+    if (item >= a.n_items) break;
+    const int2 wi = a.items[item];               // (ligand slot, first chain of the block)
+    const vd_lig_header h = a.headers[wi.x];
+    if (wi.x != cur) {
+      __syncthreads();
+      stage_ligand<G>(a.blob, h, lds, lane);
+      __syncthreads();
+      cur = wi.x;
+    }
+    LigView L; Scratch S; float* H;
+    make_views<G>(a.blob, h, lds, a.Amax, a.Nmax, a.P.local_search == 0, group, L, S, H);
+    const uint32_t chain = (uint32_t)(wi.y + group);
+    const bool valid = (int)chain < a.P.chains;
+    int evals = 0;
+    mc_chain<G>(L, S, H, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l, a.out_e + a.out_off[wi.x].x,
+                a.out_conf + a.out_off[wi.x].y, a.out_heavy + a.out_off[wi.x].z, evals);
+    if (l == 0) my_evals += evals;
+  }
+  if (a.eval_count) {
+    long long sum = 0;
+    for (int gi = 0; gi < CPW; ++gi) sum += __shfl(my_evals, gi * G);
+    if (lane == 0) atomicAdd(a.eval_count, (unsigned long long)sum);
+  }
+}
+
+template <int G>
+__global__ __launch_bounds__(64) void vd_eval_kernel(EvalArgs a) {
+  extern __shared__ float4 lds4[];
+  float* lds = reinterpret_cast<float*>(lds4);
+  constexpr int CPW = 64 / G;
+  const int lane = threadIdx.x, group = lane / G, l = lane % G;
+  const vd_lig_header h = *a.header;
+  stage_ligand<G>(a.blob, h, lds, lane);
+  __syncthreads();
+  LigView L; Scratch S; float* H;
+  make_views<G>(a.blob, h, lds, h.n_atoms, h.n_nodes, 0, group, L, S, H);
+  const int T = h.n_torsions, nc = 7 + T, ng = 6 + T;
+  for (long long base = (long long)blockIdx.x * CPW; base < a.n_conf; base += (long long)gridDim.x * CPW) {
+    const long long c = base + group;
+    const bool valid = c < a.n_conf;
+    const float* x = a.conf + (valid ? c : 0) * nc;
+    Conf cf;
+    cf.px = x[0]; cf.py = x[1]; cf.pz = x[2]; cf.q = Quat{x[3], x[4], x[5], x[6]};
+    cf.tor = (l >= 1 && l <= T) ? x[6 + l] : 0.0f;
+    float g;
+    const float e = eval_deriv<G>(L, S, a.K, cf, l, g);
+    if (valid) {
+      if (l == 0) a.e[c] = e;
+      float* go = a.grad + c * ng;
+      if (l >= 1 && l <= T) go[5 + l] = g;
+      if (l >= G - 6) go[l - (G - 6)] = g;
+    }
+  }
+}
+
+}  // namespace
+
+namespace vdl {
+
+int mc_lds_bytes(int G, int Amax, int Nmax, int dense) {
+  return 4 * (lig_floats(Amax, Nmax) + (64 / G) * group_floats(Amax, Nmax, G, dense));
+}
+
+hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& gc, const float4* lut_rows, float* maps,
+                       const vd_grid_desc& gd, hipStream_t s) {
+  const int tx = (gd.dim[0] + TX - 1) / TX, ty = (gd.dim[1] + TY - 1) / TY, tz = (gd.dim[2] + TZ - 1) / TZ;
+  hipLaunchKernelGGL(vd_grid_kernel, dim3(tx * ty * tz), dim3(256), 0, s, atoms, cells, gc, lut_rows, maps, gd, tx, ty);
+  return hipGetLastError();
+}
+
+hipError_t launch_eval(int G, const EvalArgs& a, int blocks, hipStream_t s) {
+  const int A = a.host_header.n_atoms, N = a.host_header.n_nodes;
+  const int lds = mc_lds_bytes(G, A, N, 0);
+  switch (G) {
+    case 64: hipLaunchKernelGGL(vd_eval_kernel<64>, dim3(blocks), dim3(64), lds, s, a); break;
+    case 32: hipLaunchKernelGGL(vd_eval_kernel<32>, dim3(blocks), dim3(64), lds, s, a); break;
+    case 16: hipLaunchKernelGGL(vd_eval_kernel<16>, dim3(blocks), dim3(64), lds, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_mc(int G, const McArgs& a, int blocks, hipStream_t s) {
+  const int lds = mc_lds_bytes(G, a.Amax, a.Nmax, a.P.local_search == 0);
+  switch (G) {
+    case 64: hipLaunchKernelGGL(vd_mc_kernel<64>, dim3(blocks), dim3(64), lds, s, a); break;
+    case 32: hipLaunchKernelGGL(vd_mc_kernel<32>, dim3(blocks), dim3(64), lds, s, a); break;
+    case 16: hipLaunchKernelGGL(vd_mc_kernel<16>, dim3(blocks), dim3(64), lds, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+__global__ void vd_fill_int(int* p, int v) { *p = v; }
+hipError_t reset_counter(int* p, hipStream_t s) {
+  hipLaunchKernelGGL(vd_fill_int, dim3(1), dim3(1), 0, s, p, 0);
+  return hipGetLastError();
+}
+
+int max_resident_mc_blocks(int G, int lds_bytes) {
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  hipError_t e = hipSuccess;
+  switch (G) {
+    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, vd_mc_kernel<64>, 64, lds_bytes); break;
+    case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, vd_mc_kernel<32>, 64, lds_bytes); break;
+    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, vd_mc_kernel<16>, 64, lds_bytes); break;
+    default: return 0;
+  }
+  return e == hipSuccess ? n : 0;
+}
+
+}  // namespace vdl

@@ -1,0 +1,330 @@
+"""vdock: Python front end of libvdock (include/vdock.h), the MI355X docking hot path.
+
+The compute runs in libvdock.so (HIP kernels for gfx950 behind a C ABI).  This module only
+marshals arrays through ctypes; it has no fallback: if the library is missing or no GPU is
+visible, the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvdock.so")
+
+VD_OK, VD_ERR_ARG, VD_ERR_CAPACITY, VD_ERR_HIP, VD_ERR_PARSE, VD_ERR_NO_DEVICE = 0, -1, -2, -3, -4, -5
+VD_LS_BFGS, VD_LS_RILC_BFGS = 0, 1
+VD_MC_VINA, VD_MC_QVINA2 = 0, 1
+N_XS = 17
+N_TYPE_PAIRS = 153
+LUT_BINS = 2051
+DEFAULT_WEIGHTS = (-0.035579, -0.005156, 0.840245, -0.035069, -0.587439, 0.05846)
+
+# Every symbol the header declares (checked by the CPU test suite).
+EXPORTS = (
+    "vd_version", "vd_create", "vd_destroy", "vd_last_error", "vd_set_weights", "vd_get_lut",
+    "vd_set_receptor", "vd_set_receptor_pdbqt", "vd_get_receptor", "vd_box_from_center",
+    "vd_grid_populate", "vd_grid_last_ms", "vd_grid_download", "vd_grid_upload",
+    "vd_ligand_add_pdbqt", "vd_ligand_info_get", "vd_ligand_export", "vd_ligand_count",
+    "vd_eval", "vd_eval_last_ms", "vd_dock_params_default", "vd_dock_heuristics", "vd_dock",
+    "vd_dock_batch", "vd_dock_batch_plain", "vd_batch_results", "vd_batch_best", "vd_cluster",
+    "vd_synth_receptor", "vd_synth_ligand",
+)
+
+
+class VdockError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"vdock error {code}: {msg}")
+        self.code = code
+
+
+class Box(C.Structure):
+    _fields_ = [("begin", C.c_double * 3), ("end", C.c_double * 3), ("n", C.c_int * 3)]
+
+    @property
+    def dims(self):
+        return tuple(int(self.n[i]) + 1 for i in range(3))
+
+    def as_tuple(self):
+        return list(self.begin), list(self.end), list(self.n)
+
+
+class LigandInfo(C.Structure):
+    _fields_ = [("n_atoms", C.c_int), ("n_torsions", C.c_int), ("n_pairs", C.c_int), ("n_heavy", C.c_int),
+                ("torsdof", C.c_int), ("tree_depth", C.c_int)]
+
+
+class DockParams(C.Structure):
+    _fields_ = [("chains", C.c_int), ("search_depth", C.c_int), ("bfgs_steps", C.c_int), ("local_search", C.c_int),
+                ("policy", C.c_int), ("mutation_quirk", C.c_int), ("chains_per_wave", C.c_int), ("seed", C.c_uint32),
+                ("temperature", C.c_float), ("amplitude", C.c_float), ("v_intra", C.c_float), ("v_inter", C.c_float),
+                ("slope", C.c_float)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("evals", C.c_ulonglong), ("chains_per_wave", C.c_int), ("blocks", C.c_int)]
+
+
+_lib = None
+
+
+def load():
+    """Loads libvdock.so (built by __graft_entry__.build()); raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libvdock.so not found at {LIB_PATH}; build it with __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    vp, dp, fp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_int)
+    sig = {
+        "vd_version": (C.c_char_p, []),
+        "vd_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+        "vd_destroy": (None, [vp]),
+        "vd_last_error": (C.c_char_p, [vp]),
+        "vd_set_weights": (C.c_int, [vp, dp]),
+        "vd_get_lut": (C.c_int, [vp, C.c_int, dp, dp, dp]),
+        "vd_set_receptor": (C.c_int, [vp, C.c_int, fp, ip]),
+        "vd_set_receptor_pdbqt": (C.c_int, [vp, C.c_char_p, ip]),
+        "vd_get_receptor": (C.c_int, [vp, C.c_int, fp, ip]),
+        "vd_box_from_center": (C.c_int, [dp, dp, C.c_double, C.POINTER(Box)]),
+        "vd_grid_populate": (C.c_int, [vp, C.POINTER(Box)]),
+        "vd_grid_last_ms": (C.c_double, [vp]),
+        "vd_grid_download": (C.c_int, [vp, fp]),
+        "vd_grid_upload": (C.c_int, [vp, C.POINTER(Box), fp]),
+        "vd_ligand_add_pdbqt": (C.c_int, [vp, C.c_char_p, ip]),
+        "vd_ligand_info_get": (C.c_int, [vp, C.c_int, C.POINTER(LigandInfo)]),
+        "vd_ligand_export": (C.c_int, [vp, C.c_int, ip, ip, ip, fp, ip, ip, ip, fp, fp, ip]),
+        "vd_ligand_count": (C.c_int, [vp]),
+        "vd_eval": (C.c_int, [vp, C.c_int, C.c_longlong, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int]),
+        "vd_eval_last_ms": (C.c_double, [vp]),
+        "vd_dock_params_default": (C.c_int, [C.POINTER(DockParams)]),
+        "vd_dock_heuristics": (C.c_int, [C.c_int, C.c_int, C.c_int, ip, ip]),
+        "vd_dock": (C.c_int, [vp, C.c_int, C.POINTER(DockParams), fp, fp, fp, C.POINTER(Stats)]),
+        "vd_dock_batch": (C.c_int, [vp, C.c_int, ip, C.POINTER(DockParams), C.POINTER(Stats)]),
+        "vd_dock_batch_plain": (C.c_int, [vp, C.c_int, ip, C.POINTER(DockParams), C.POINTER(Stats)]),
+        "vd_batch_results": (C.c_int, [vp, C.c_int, fp, fp, fp]),
+        "vd_batch_best": (C.c_int, [vp, fp]),
+        "vd_cluster": (C.c_int, [C.c_int, C.c_int, fp, fp, C.c_double, C.c_int, ip]),
+        "vd_synth_receptor": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_uint64, fp, ip]),
+        "vd_synth_ligand": (C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_char_p, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def default_params(**kw):
+    p = DockParams()
+    load().vd_dock_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def heuristics(n_atoms, n_torsions, rilc=True):
+    d, b = C.c_int(), C.c_int()
+    load().vd_dock_heuristics(n_atoms, n_torsions, 1 if rilc else 0, C.byref(d), C.byref(b))
+    return d.value, b.value
+
+
+def box_from_center(center, size, granularity=0.375):
+    b = Box()
+    c = np.array(center, dtype=np.float64)
+    s = np.array(size, dtype=np.float64)
+    rc = load().vd_box_from_center(_d(c), _d(s), granularity, C.byref(b))
+    if rc != VD_OK:
+        raise VdockError(rc, "bad box")
+    return b
+
+
+def synth_receptor(n_atoms, cube, cavity_radius, seed):
+    xyz = np.zeros((n_atoms, 3), dtype=np.float32)
+    xs = np.zeros(n_atoms, dtype=np.int32)
+    n = load().vd_synth_receptor(n_atoms, cube, cavity_radius, seed, _f(xyz), _i(xs))
+    if n < 0:
+        raise VdockError(n, "synth_receptor")
+    return xyz[:n].copy(), xs[:n].copy()
+
+
+def synth_ligand(n_heavy, n_rot, seed):
+    need = load().vd_synth_ligand(n_heavy, n_rot, seed, None, 0)
+    buf = C.create_string_buffer(need + 1)
+    load().vd_synth_ligand(n_heavy, n_rot, seed, buf, need + 1)
+    return buf.value.decode()
+
+
+def cluster(e, heavy, min_rmsd=1.0, max_size=20):
+    e = np.ascontiguousarray(e, dtype=np.float32)
+    h = np.ascontiguousarray(heavy, dtype=np.float32)
+    kept = np.zeros(max_size, dtype=np.int32)
+    k = load().vd_cluster(len(e), h.shape[1] if h.ndim == 3 else 0, _f(e), _f(h), min_rmsd, max_size, _i(kept))
+    if k < 0:
+        raise VdockError(k, "cluster")
+    return kept[:k].copy()
+
+
+class Context:
+    """One HIP device context (one stream).  Mirrors the C ABI call by call."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.vd_create(device, C.byref(h))
+        if rc != VD_OK:
+            raise VdockError(rc, "vd_create failed (no usable GPU?)")
+        self.h = h
+        self.box = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.vd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != VD_OK:
+            raise VdockError(rc, self.lib.vd_last_error(self.h).decode())
+
+    # -- scoring function
+    def set_weights(self, w):
+        a = np.array(w, dtype=np.float64)
+        self._check(self.lib.vd_set_weights(self.h, _d(a)))
+
+    def lut(self, tp):
+        se, sd, fa = (np.zeros(LUT_BINS) for _ in range(3))
+        self._check(self.lib.vd_get_lut(self.h, tp, _d(se), _d(sd), _d(fa)))
+        return se, sd, fa
+
+    # -- receptor and maps
+    def set_receptor(self, xyz, xs):
+        x = np.ascontiguousarray(xyz, dtype=np.float32)
+        t = np.ascontiguousarray(xs, dtype=np.int32)
+        self._check(self.lib.vd_set_receptor(self.h, len(t), _f(x), _i(t)))
+
+    def set_receptor_pdbqt(self, text):
+        n = C.c_int()
+        self._check(self.lib.vd_set_receptor_pdbqt(self.h, text.encode(), C.byref(n)))
+        return n.value
+
+    def receptor(self):
+        n = self.lib.vd_get_receptor(self.h, 0, None, None)
+        xyz = np.zeros((max(n, 1), 3), dtype=np.float32)
+        xs = np.zeros(max(n, 1), dtype=np.int32)
+        self.lib.vd_get_receptor(self.h, n, _f(xyz), _i(xs))
+        return xyz[:n], xs[:n]
+
+    def grid_populate(self, box):
+        self._check(self.lib.vd_grid_populate(self.h, C.byref(box)))
+        self.box = box
+        return self.lib.vd_grid_last_ms(self.h)
+
+    def grid_download(self):
+        nx, ny, nz = self.box.dims
+        out = np.zeros((N_XS, nz, ny, nx), dtype=np.float32)
+        self._check(self.lib.vd_grid_download(self.h, _f(out)))
+        return out
+
+    def grid_upload(self, box, maps):
+        m = np.ascontiguousarray(maps, dtype=np.float32)
+        self._check(self.lib.vd_grid_upload(self.h, C.byref(box), _f(m)))
+        self.box = box
+
+    # -- ligands
+    def add_ligand(self, text):
+        i = C.c_int()
+        self._check(self.lib.vd_ligand_add_pdbqt(self.h, text.encode(), C.byref(i)))
+        return i.value
+
+    def ligand_info(self, lid):
+        info = LigandInfo()
+        self._check(self.lib.vd_ligand_info_get(self.h, lid, C.byref(info)))
+        return info
+
+    def ligand_export(self, lid):
+        inf = self.ligand_info(lid)
+        A, N, P = inf.n_atoms, inf.n_torsions + 1, inf.n_pairs
+        xs, el, node = (np.zeros(A, dtype=np.int32) for _ in range(3))
+        local = np.zeros((A, 3), dtype=np.float32)
+        parent, begin, end = (np.zeros(N, dtype=np.int32) for _ in range(3))
+        ro, ra = np.zeros((N, 3), dtype=np.float32), np.zeros((N, 3), dtype=np.float32)
+        pairs = np.zeros((max(P, 1), 3), dtype=np.int32)
+        self._check(self.lib.vd_ligand_export(self.h, lid, _i(xs), _i(el), _i(node), _f(local), _i(parent), _i(begin),
+                                              _i(end), _f(ro), _f(ra), _i(pairs)))
+        return dict(xs=xs, el=el, node=node, local=local, parent=parent, begin=begin, end=end, rel_origin=ro,
+                    rel_axis=ra, pairs=pairs[:P])
+
+    # -- energy and gradient
+    def eval(self, lid, conf, v_intra=10.0, v_inter=10.0, slope=1e6, chains_per_wave=0):
+        conf = np.ascontiguousarray(conf, dtype=np.float32)
+        n, m = conf.shape
+        T = m - 7
+        e = np.zeros(n, dtype=np.float32)
+        g = np.zeros((n, 6 + T), dtype=np.float32)
+        self._check(self.lib.vd_eval(self.h, lid, n, conf.ctypes.data, e.ctypes.data, g.ctypes.data, v_intra, v_inter,
+                                     slope, chains_per_wave, 0))
+        return e, g
+
+    def eval_device(self, lid, conf_ptr, e_ptr, g_ptr, n, v_intra=10.0, v_inter=10.0, slope=1e6, chains_per_wave=0):
+        """Device-pointer variant: pointers are integers (e.g. torch tensor .data_ptr())."""
+        self._check(self.lib.vd_eval(self.h, lid, n, conf_ptr, e_ptr, g_ptr, v_intra, v_inter, slope,
+                                     chains_per_wave, 1))
+        return self.lib.vd_eval_last_ms(self.h)
+
+    def eval_last_ms(self):
+        return self.lib.vd_eval_last_ms(self.h)
+
+    # -- docking
+    def dock(self, lid, params):
+        inf = self.ligand_info(lid)
+        n = params.chains
+        e = np.zeros(n, dtype=np.float32)
+        conf = np.zeros((n, 7 + inf.n_torsions), dtype=np.float32)
+        heavy = np.zeros((n, inf.n_heavy, 3), dtype=np.float32)
+        st = Stats()
+        self._check(self.lib.vd_dock(self.h, lid, C.byref(params), _f(e), _f(conf), _f(heavy), C.byref(st)))
+        return e, conf, heavy, st
+
+    def dock_batch(self, ids, params, plain=False):
+        a = np.ascontiguousarray(ids, dtype=np.int32)
+        st = Stats()
+        f = self.lib.vd_dock_batch_plain if plain else self.lib.vd_dock_batch
+        self._check(f(self.h, len(a), _i(a), C.byref(params), C.byref(st)))
+        self._batch_ids = list(a)
+        return st
+
+    def batch_results(self, k, params):
+        lid = self._batch_ids[k]
+        inf = self.ligand_info(lid)
+        n = params.chains
+        e = np.zeros(n, dtype=np.float32)
+        conf = np.zeros((n, 7 + inf.n_torsions), dtype=np.float32)
+        heavy = np.zeros((n, inf.n_heavy, 3), dtype=np.float32)
+        self._check(self.lib.vd_batch_results(self.h, k, _f(e), _f(conf), _f(heavy)))
+        return e, conf, heavy
+
+    def batch_best(self):
+        out = np.zeros(len(self._batch_ids), dtype=np.float32)
+        self._check(self.lib.vd_batch_best(self.h, _f(out)))
+        return out
+
+
+def version():
+    return load().vd_version().decode()
