@@ -276,7 +276,7 @@ void run_mc(vd_ctx* c, int G, const Packed& pk, const std::vector<uint32_t>& str
   if (persistent) {
     a.queue = c->queue.p;
     HIP_CHECK(vdl::reset_counter(c->queue.p, c->stream));
-    const int per_cu = std::max(1, vdl::max_resident_mc_blocks(G, lds));
+    const int per_cu = std::max(1, vdl::max_resident_mc_blocks(G, P.local_search, lds));
     blocks = std::min<long long>((long long)per_cu * c->n_cu, (long long)items.size());
     blocks = std::max(blocks, 1);
   }

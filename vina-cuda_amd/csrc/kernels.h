@@ -38,7 +38,7 @@ struct EvalArgs {
 
 namespace vdl {
 int mc_lds_bytes(int G, int Amax, int Nmax, int dense);
-int max_resident_mc_blocks(int G, int lds_bytes);
+int max_resident_mc_blocks(int G, int mode, int lds_bytes);
 hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& gc, const float4* lut_rows, float* maps,
                        const vd_grid_desc& gd, hipStream_t s);
 hipError_t launch_eval(int G, const EvalArgs& a, int blocks, hipStream_t s);

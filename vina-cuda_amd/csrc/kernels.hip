@@ -108,12 +108,12 @@ __device__ __forceinline__ void make_views(const float4* __restrict__ blob, cons
   H = S.frc + pad4(3 * Amax);
 }
 
-template <int G>
-__device__ float local_search(const LigView& L, const Scratch& S, float* H, const EvalK& K, int mode, Conf& x,
+template <int G, int MODE>
+__device__ float local_search(const LigView& L, const Scratch& S, float* H, const EvalK& K, Conf& x,
                               bool enabled, float x_e, int max_steps, int l, int& evals) {
   if (!__any(enabled)) return x_e;
-  if (mode == 1) return rilc_bfgs<G>(L, S, K, x, enabled, x_e, max_steps, l, evals);
-  return dense_bfgs<G>(L, S, H, H + G * G, K, x, enabled, x_e, max_steps, l, evals);
+  if constexpr (MODE == 1) return rilc_bfgs<G>(L, S, K, x, enabled, x_e, max_steps, l, evals);
+  else return dense_bfgs<G>(L, S, H, H + G * G, K, x, enabled, x_e, max_steps, l, evals);
 }
 
 // Initial conformation of a chain (conf::randomize, V11/lib/conf.h:127-130, 214-217;
@@ -150,7 +150,7 @@ __device__ Conf initial_conf(const vd_mc_consts& P, uint32_t stream, uint32_t ch
 // mutate -> local search -> Metropolis(T) -> on accept, if better than the best so far: second
 // local search, keep the best.  Mutation as V11/lib/mutate.cpp:62-127 (see initial_conf for the
 // random stream); the orientation step uses the gyration radius of the current conformation.
-template <int G>
+template <int G, int MODE>
 __device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const EvalK& K, const vd_mc_consts& P,
                          int depth, int bfgs_steps, uint32_t stream, uint32_t chain, bool valid, int l, float* out_e,
                          float* out_conf, float* out_heavy, int& evals) {
@@ -177,11 +177,11 @@ __device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const Eva
     if (which == 0) { cand.px += P.amplitude * sx; cand.py += P.amplitude * sy; cand.pz += P.amplitude * sz; }
     else if (which == 1) { if (rg > kEpsFl) { const float f = P.amplitude / rg; cand.q = qincrement(cand.q, f * sx, f * sy, f * sz); } }
     else if (l == which - 1) cand.tor = tor;
-    const float ce = local_search<G>(L, S, H, K, P.local_search, cand, valid, 0.0f, bfgs_steps, l, evals);
+    const float ce = local_search<G, MODE>(L, S, H, K, cand, valid, 0.0f, bfgs_steps, l, evals);
     const bool accept = valid && (step == 0 || ce < tmp_e || u_metro < expf((tmp_e - ce) / P.temperature));
     if (accept) { tmp = cand; tmp_e = ce; }
     const bool second = accept && tmp_e < best_e;
-    const float e2 = local_search<G>(L, S, H, K, P.local_search, tmp, second, tmp_e, bfgs_steps, l, evals);
+    const float e2 = local_search<G, MODE>(L, S, H, K, tmp, second, tmp_e, bfgs_steps, l, evals);
     if (second) { tmp_e = e2; if (tmp_e < best_e) { best = tmp; best_e = tmp_e; } }
     if (__any(accept)) {
       set_coords<G>(L, S, tmp, l);
@@ -208,7 +208,7 @@ __device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const Eva
   }
 }
 
-template <int G>
+template <int G, int MODE>
 __global__ __launch_bounds__(64) void vd_mc_kernel(McArgs a) {
   extern __shared__ float4 lds4[];
   float* lds = reinterpret_cast<float*>(lds4);
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(64) void vd_mc_kernel(McArgs a) {
     const uint32_t chain = (uint32_t)(wi.y + group);
     const bool valid = (int)chain < a.P.chains;
     int evals = 0;
-    mc_chain<G>(L, S, H, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l, a.out_e + a.out_off[wi.x].x,
+    mc_chain<G, MODE>(L, S, H, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l, a.out_e + a.out_off[wi.x].x,
                 a.out_conf + a.out_off[wi.x].y, a.out_heavy + a.out_off[wi.x].z, evals);
     if (l == 0) my_evals += evals;
   }
@@ -307,15 +307,20 @@ hipError_t launch_eval(int G, const EvalArgs& a, int blocks, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_mc(int G, const McArgs& a, int blocks, hipStream_t s) {
-  const int lds = mc_lds_bytes(G, a.Amax, a.Nmax, a.P.local_search == 0);
+template <int MODE>
+static hipError_t launch_mc_mode(int G, const McArgs& a, int blocks, int lds, hipStream_t s) {
   switch (G) {
-    case 64: hipLaunchKernelGGL(vd_mc_kernel<64>, dim3(blocks), dim3(64), lds, s, a); break;
-    case 32: hipLaunchKernelGGL(vd_mc_kernel<32>, dim3(blocks), dim3(64), lds, s, a); break;
-    case 16: hipLaunchKernelGGL(vd_mc_kernel<16>, dim3(blocks), dim3(64), lds, s, a); break;
+    case 64: hipLaunchKernelGGL((vd_mc_kernel<64, MODE>), dim3(blocks), dim3(64), lds, s, a); break;
+    case 32: hipLaunchKernelGGL((vd_mc_kernel<32, MODE>), dim3(blocks), dim3(64), lds, s, a); break;
+    case 16: hipLaunchKernelGGL((vd_mc_kernel<16, MODE>), dim3(blocks), dim3(64), lds, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_mc(int G, const McArgs& a, int blocks, hipStream_t s) {
+  const int lds = mc_lds_bytes(G, a.Amax, a.Nmax, a.P.local_search == 0);
+  return a.P.local_search == 1 ? launch_mc_mode<1>(G, a, blocks, lds, s) : launch_mc_mode<0>(G, a, blocks, lds, s);
 }
 
 __global__ void vd_fill_int(int* p, int v) { *p = v; }
@@ -324,17 +329,21 @@ hipError_t reset_counter(int* p, hipStream_t s) {
   return hipGetLastError();
 }
 
-int max_resident_mc_blocks(int G, int lds_bytes) {
-  int dev = 0, n = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 0;
-  hipError_t e = hipSuccess;
+template <int MODE>
+static int occupancy(int G, int lds_bytes) {
+  int n = 0;
+  hipError_t e = hipErrorInvalidValue;
   switch (G) {
-    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, vd_mc_kernel<64>, 64, lds_bytes); break;
-    case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, vd_mc_kernel<32>, 64, lds_bytes); break;
-    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, vd_mc_kernel<16>, 64, lds_bytes); break;
-    default: return 0;
+    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, vd_mc_kernel<64, MODE>, 64, lds_bytes); break;
+    case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, vd_mc_kernel<32, MODE>, 64, lds_bytes); break;
+    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, vd_mc_kernel<16, MODE>, 64, lds_bytes); break;
+    default: break;
   }
   return e == hipSuccess ? n : 0;
+}
+
+int max_resident_mc_blocks(int G, int mode, int lds_bytes) {
+  return mode == 1 ? occupancy<1>(G, lds_bytes) : occupancy<0>(G, lds_bytes);
 }
 
 }  // namespace vdl
