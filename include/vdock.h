@@ -15,6 +15,7 @@
  */
 #ifndef VDOCK_H
 #define VDOCK_H
+#include <stddef.h>
 #include <stdint.h>
 #ifdef __cplusplus
 extern "C" {
@@ -115,6 +116,7 @@ typedef struct {
   int policy;            /* VD_MC_VINA or VD_MC_QVINA2 */
   int mutation_quirk;    /* 1: mutate one of 1+T entities like the reference GPU host tables */
   int chains_per_wave;   /* 1, 2, 4; 0 = autotuned per ligand size bucket */
+  int waves_per_simd;    /* register budget of the MC kernel: 2, 3 or 4 resident waves per SIMD; 0 = autotuned */
   uint32_t seed;
   float temperature;     /* 1.2 */
   float amplitude;       /* mutation amplitude 2 */
@@ -150,6 +152,15 @@ int vd_batch_best(vd_ctx* ctx, float* best_e);
 /* Output pose list: add_to_output_container over chains in order (min_rmsd, max_size;
  * V11/lib/coords.cpp:47-62).  Writes kept chain indices sorted by energy; returns the count. */
 int vd_cluster(int n, int n_heavy, const float* e, const float* heavy, double min_rmsd, int max_size, int* kept);
+
+/* Device memory on the context's GPU (for callers that keep inputs resident in HBM).
+ * vd_copy direction: 0 host->device, 1 device->host, 2 device->device; synchronous. */
+int vd_device_alloc(vd_ctx* ctx, size_t bytes, void** ptr);
+int vd_device_free(vd_ctx* ctx, void* ptr);
+int vd_copy(vd_ctx* ctx, void* dst, const void* src, size_t bytes, int direction);
+/* n random conformations of ligand id written to device memory d_conf[n][7+T]: translation
+ * uniform in the current box, uniform random orientation, torsions uniform in [-pi, pi). */
+int vd_random_confs(vd_ctx* ctx, int id, long long n, uint32_t seed, float* d_conf);
 
 /* Synthetic inputs of the benchmark recipes: Poisson-disk receptor with a central cavity,
  * random acyclic ligand with n_rot rotatable bonds (PDBQT text; returns its length, or the

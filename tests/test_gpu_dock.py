@@ -1,0 +1,88 @@
+"""Statistical docking parity (GPU MC vs the FP64 oracle's MC with the same schedule, heuristics and
+random stream).  Docking is stochastic and FP32 vs FP64 trajectories separate after a few steps,
+so parity is judged over 16 seeds:
+  - best score over the 16 runs within 0.1 kcal/mol of the oracle's best over its 16 runs;
+  - mean per-run best score within 0.1 kcal/mol;
+  - the two overall top poses within 2 A RMSD (heavy atoms);
+  - the fraction of runs whose top pose lands within 2 A of the overall top pose is no worse than
+    the oracle's own fraction minus 0.25 (equal search accuracy).
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def scene(ctx):
+    """The reference's 1u1b example: its receptor and its search box (1u1b_config.txt)."""
+    import vdock
+    from conftest import golden
+    ctx.set_receptor_pdbqt(golden("1u1b_protein.pdbqt"))
+    box = vdock.box_from_center([-30.661, -29.540, 17.010], [24.0, 26.25, 22.5], 0.375)
+    ctx.grid_populate(box)
+    maps = ctx.grid_download()
+    g = orc.Grid(orc.Lut(), *box.as_tuple())
+    g.set_maps(maps.astype(np.float64))                    # the oracle docks on the same maps
+    return dict(box=box, maps=maps, grid=g)
+
+
+def rmsd(a, b):
+    return float(np.sqrt(((a - b) ** 2).sum(axis=-1).mean()))
+
+
+@pytest.mark.parametrize("heavy,tors,lseed", [(12, 2, 31), (16, 4, 32)])
+def test_docking_statistical_parity(ctx, scene, heavy, tors, lseed):
+    import vdock
+    ctx.grid_upload(scene["box"], scene["maps"])
+    text = vdock.synth_ligand(heavy, tors, lseed)
+    lid = ctx.add_ligand(text)
+    lig = orc.Ligand(text)
+    chains, depth, bfgs = 512, 8, 16
+    b, e, _ = scene["box"].as_tuple()
+    gb, ob, gp, op_ = [], [], [], []
+    for seed in range(16):
+        p = vdock.default_params(chains=chains, search_depth=depth, bfgs_steps=bfgs, seed=seed)
+        eg, cg, hg, _ = ctx.dock(lid, p)
+        prm = orc.mc_params(chains, depth, bfgs, b, e, seed=seed, stream=lid, threads=8)
+        eo, co, ho, _ = orc.mc_run(scene["grid"], lig, prm)
+        gb.append(float(eg.min()))
+        ob.append(float(eo.min()))
+        gp.append(hg[int(np.argmin(eg))].astype(np.float64))
+        op_.append(ho[int(np.argmin(eo))])
+        say(f"seed {seed}: gpu {gb[-1]:.3f} oracle {ob[-1]:.3f}")
+    gb, ob = np.array(gb), np.array(ob)
+    top_g, top_o = gp[int(np.argmin(gb))], op_[int(np.argmin(ob))]
+    succ_g = np.mean([rmsd(x, top_o) <= 2.0 for x in gp])
+    succ_o = np.mean([rmsd(x, top_o) <= 2.0 for x in op_])
+    say("best gpu", gb.min(), "oracle", ob.min(), "mean", gb.mean(), ob.mean(), "top-pose rmsd", rmsd(top_g, top_o),
+        "runs within 2A of top: gpu", succ_g, "oracle", succ_o)
+    assert abs(gb.min() - ob.min()) <= 0.1
+    assert abs(gb.mean() - ob.mean()) <= 0.1
+    assert rmsd(top_g, top_o) <= 2.0
+    assert succ_g >= succ_o - 0.25
+
+
+def test_initial_conformations_match_oracle(ctx, scene):
+    """With zero MC steps the device keeps each chain's initial conformation: it must equal the
+    oracle's draw from the shared Philox stream (FP32 rounding aside)."""
+    import vdock
+    ctx.grid_upload(scene["box"], scene["maps"])
+    text = vdock.synth_ligand(14, 3, 77)
+    lid = ctx.add_ligand(text)
+    lig = orc.Ligand(text)
+    b, e, _ = scene["box"].as_tuple()
+    # one step whose local search cannot move: bfgs_steps=1 and compare the energies instead
+    p = vdock.default_params(chains=32, search_depth=1, bfgs_steps=1, seed=3)
+    eg, cg, _, _ = ctx.dock(lid, p)
+    op = orc.mc_params(32, 1, 1, b, e, seed=3, stream=lid)
+    eo, co, _, _ = orc.mc_run(scene["grid"], lig, op)
+    close = np.abs(eg - eo) <= np.maximum(1e-2, 1e-3 * np.abs(eo))
+    say("one-step energies within tolerance:", close.mean())
+    assert close.mean() >= 0.9

@@ -87,7 +87,8 @@ struct vd_ctx {
   BatchState batch;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int n_cu = 0;
-  std::map<int, int> tuned;       // size bucket -> chains per wave
+  std::map<int, std::pair<int, int>> tuned;  // size bucket -> (lanes per chain, waves per SIMD)
+  DevBuf<float> tune_e, tune_conf, tune_heavy;
 };
 
 namespace {
@@ -166,6 +167,8 @@ void check_params(const vd_dock_params* p) {
   if (p->policy != VD_MC_VINA && p->policy != VD_MC_QVINA2) throw ArgError(VD_ERR_ARG, "bad policy");
   if (p->chains_per_wave != 0 && p->chains_per_wave != 1 && p->chains_per_wave != 2 && p->chains_per_wave != 4)
     throw ArgError(VD_ERR_ARG, "chains_per_wave must be 0, 1, 2 or 4");
+  if (p->waves_per_simd != 0 && (p->waves_per_simd < 2 || p->waves_per_simd > 4))
+    throw ArgError(VD_ERR_ARG, "waves_per_simd must be 0, 2, 3 or 4");
 }
 
 int heur_depth(int A, int T, int rilc) {
@@ -184,23 +187,32 @@ int min_group(const vd::Ligand& L) {
   if (T + 7 <= 64) return 64;
   return 0;
 }
+// Autotuner buckets: atoms in [0,24), [24,48), [48,80), [80,..) x torsions in the ranges that
+// admit 4, 2 or 1 chains per wavefront.
 int size_bucket(const vd::Ligand& L) {
-  int a = (int)L.atoms.size(), t = L.n_torsions();
-  return (std::min(a / 16, 15) << 8) | std::min(t / 4, 15);
+  const int a = (int)L.atoms.size(), t = L.n_torsions();
+  const int ab = a < 24 ? 0 : a < 48 ? 1 : a < 80 ? 2 : 3;
+  const int tb = t + 7 <= 16 ? 0 : t + 7 <= 32 ? 1 : 2;
+  return ab * 4 + tb;
 }
 int pick_group(vd_ctx* c, const vd::Ligand& L, int cpw) {
   const int gmin = min_group(L);
   if (gmin == 0) throw ArgError(VD_ERR_CAPACITY, "ligand has too many torsions");
-  int G = 64;
-  if (cpw == 2) G = 32;
+  int G = gmin;
+  if (cpw == 1) G = 64;
+  else if (cpw == 2) G = 32;
   else if (cpw == 4) G = 16;
-  else if (cpw == 0) {
+  else {
     auto it = c->tuned.find(size_bucket(L));
-    if (it != c->tuned.end()) G = it->second;
-    else G = std::max(gmin, (int)L.atoms.size() + 4 <= 32 ? 32 : 64);
+    if (it != c->tuned.end()) G = it->second.first;
   }
   if (G < gmin) G = gmin;
   return G;
+}
+
+int pick_waves(vd_ctx* c, const vd::Ligand& L, int G) {
+  auto it = c->tuned.find(size_bucket(L));
+  return (it != c->tuned.end() && it->second.first == G) ? it->second.second : 2;
 }
 
 vd_mc_consts make_consts(vd_ctx* c, const vd_dock_params* p, int depth, int bfgs) {
@@ -223,7 +235,7 @@ vdk::EvalK make_evalk(vd_ctx* c, float v_intra, float v_inter, float slope) {
 struct Packed {
   std::vector<float> blob;
   std::vector<vd_lig_header> headers;
-  int Amax = 0, Nmax = 0;
+  int Amax = 0, Nmax = 0, Pmax = 0;
 };
 Packed pack(vd_ctx* c, const std::vector<int>& ids, const std::vector<int>* depth = nullptr,
             const std::vector<int>* bfgs = nullptr) {
@@ -235,6 +247,7 @@ Packed pack(vd_ctx* c, const std::vector<int>& ids, const std::vector<int>* dept
     if (bfgs) pk.headers.back().bfgs_steps = (*bfgs)[k];
     pk.Amax = std::max(pk.Amax, (int)L.atoms.size());
     pk.Nmax = std::max(pk.Nmax, (int)L.nodes.size());
+    pk.Pmax = std::max(pk.Pmax, (int)L.pairs.size());
   }
   return pk;
 }
@@ -246,7 +259,7 @@ double elapsed_ms(vd_ctx* c) {
 }
 
 // Runs one MC launch over the given ligand slots (all with the same G) and work items.
-void run_mc(vd_ctx* c, int G, const Packed& pk, const std::vector<uint32_t>& streams, const std::vector<int2>& items,
+void run_mc(vd_ctx* c, int G, int W, const Packed& pk, const std::vector<uint32_t>& streams, const std::vector<int2>& items,
             const std::vector<longlong4>& off, const vd_mc_consts& P, float* d_e, float* d_conf, float* d_heavy,
             bool persistent, vd_stats* st, bool timed) {
   c->blob.ensure(pk.blob.size() / 4 + 1);
@@ -269,19 +282,19 @@ void run_mc(vd_ctx* c, int G, const Packed& pk, const std::vector<uint32_t>& str
   a.P = P;
   a.out_e = d_e; a.out_conf = d_conf; a.out_heavy = d_heavy; a.out_off = c->offs.p;
   a.eval_count = c->evals.p;
-  a.Amax = pk.Amax; a.Nmax = pk.Nmax;
-  const int lds = vdl::mc_lds_bytes(G, pk.Amax, pk.Nmax, P.local_search == 0);
-  if (lds > 160 * 1024) throw ArgError(VD_ERR_CAPACITY, "ligand too large for the LDS layout");
+  a.Amax = pk.Amax; a.Nmax = pk.Nmax; a.Pmax = pk.Pmax;
+  const int lds = vdl::mc_lds_bytes(G, pk.Amax, pk.Nmax, pk.Pmax, P.local_search == 0);
+  if (lds > 64 * 1024) throw ArgError(VD_ERR_CAPACITY, "ligand too large for the LDS layout");
   int blocks = (int)items.size();
   if (persistent) {
     a.queue = c->queue.p;
     HIP_CHECK(vdl::reset_counter(c->queue.p, c->stream));
-    const int per_cu = std::max(1, vdl::max_resident_mc_blocks(G, P.local_search, lds));
+    const int per_cu = std::max(1, vdl::max_resident_mc_blocks(G, W, P.local_search, lds));
     blocks = std::min<long long>((long long)per_cu * c->n_cu, (long long)items.size());
     blocks = std::max(blocks, 1);
   }
   if (timed) HIP_CHECK(hipEventRecord(c->ev0, c->stream));
-  HIP_CHECK(vdl::launch_mc(G, a, blocks, c->stream));
+  HIP_CHECK(vdl::launch_mc(G, W, a, blocks, c->stream));
   if (timed) HIP_CHECK(hipEventRecord(c->ev1, c->stream));
   if (st) {
     unsigned long long ev = 0;
@@ -296,17 +309,55 @@ void run_mc(vd_ctx* c, int G, const Packed& pk, const std::vector<uint32_t>& str
   }
 }
 
-struct LigPlan { int id, G, depth, bfgs, T, H, chains; };
+struct LigPlan { int id, G, W, depth, bfgs, T, H, chains; };
 LigPlan plan_for(vd_ctx* c, int id, const vd_dock_params* p) {
   const vd::Ligand& L = lig_at(c, id);
   LigPlan pl;
   pl.id = id; pl.T = L.n_torsions(); pl.H = L.n_heavy; pl.chains = p->chains;
   pl.G = pick_group(c, L, p->chains_per_wave);
+  pl.W = p->waves_per_simd ? p->waves_per_simd : pick_waves(c, L, pl.G);
+  if (p->local_search != VD_LS_RILC_BFGS) pl.W = 2;
   const int rilc = p->local_search == VD_LS_RILC_BFGS;
   pl.depth = p->search_depth > 0 ? p->search_depth : heur_depth((int)L.atoms.size(), pl.T, rilc);
   pl.bfgs = p->bfgs_steps > 0 ? p->bfgs_steps : heur_bfgs((int)L.atoms.size(), rilc);
   pl.bfgs = std::max(pl.bfgs, 1);
   return pl;
+}
+
+// Autotuner: for every size bucket of the batch not tuned yet, dock a representative ligand with
+// each admissible group width at full occupancy (two MC steps) and keep the width with the most
+// evaluations per millisecond.
+void autotune(vd_ctx* c, const std::vector<int>& ids, const vd_dock_params* p) {
+  std::map<int, int> rep;
+  for (int id : ids) { const int b = size_bucket(lig_at(c, id)); if (!c->tuned.count(b) && !rep.count(b)) rep[b] = id; }
+  for (auto& kv : rep) {
+    const vd::Ligand& L = lig_at(c, kv.second);
+    const int gmin = min_group(L);
+    int best_g = gmin, best_w = 2; double best_rate = -1;
+    for (int G = gmin; G <= 64; G *= 2)
+      for (int W = 2; W <= (p->local_search == VD_LS_RILC_BFGS ? 4 : 2); ++W) {
+        vd_dock_params q = *p;
+        q.chains_per_wave = 64 / G;
+        const int rilc = p->local_search == VD_LS_RILC_BFGS;
+        const int bfgs = p->bfgs_steps > 0 ? p->bfgs_steps : heur_bfgs((int)L.atoms.size(), rilc);
+        const int cpw = 64 / G;
+        const int blocks = std::max(1, c->n_cu) * 4 * W;
+        q.chains = blocks * cpw;
+        std::vector<int> dv{2}, bv{std::max(bfgs, 1)};
+        Packed pk = pack(c, {kv.second}, &dv, &bv);
+        std::vector<int2> items;
+        for (int c0 = 0; c0 < q.chains; c0 += cpw) items.push_back(make_int2(0, c0));
+        std::vector<longlong4> off{make_longlong4(0, 0, 0, 0)};
+        c->tune_e.ensure(q.chains); c->tune_conf.ensure((size_t)q.chains * (7 + L.n_torsions()));
+        c->tune_heavy.ensure((size_t)q.chains * L.n_heavy * 3);
+        vd_stats st{};
+        run_mc(c, G, W, pk, {(uint32_t)kv.second}, items, off, make_consts(c, &q, 2, bv[0]), c->tune_e.p, c->tune_conf.p,
+               c->tune_heavy.p, false, &st, true);
+        const double rate = st.kernel_ms > 0 ? st.evals / st.kernel_ms : 0;
+        if (rate > best_rate) { best_rate = rate; best_g = G; best_w = W; }
+      }
+    c->tuned[kv.first] = std::make_pair(best_g, best_w);
+  }
 }
 
 // Prepares the batch result buffers for the ligands in `ids`.
@@ -368,6 +419,7 @@ void vd_destroy(vd_ctx* c) {
   c->blob.release(); c->headers.release(); c->streams.release(); c->items.release(); c->offs.release();
   c->queue.release(); c->evals.release(); c->io_a.release(); c->io_b.release(); c->io_c.release();
   c->batch.e.release(); c->batch.conf.release(); c->batch.heavy.release();
+  c->tune_e.release(); c->tune_conf.release(); c->tune_heavy.release();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -642,6 +694,7 @@ int vd_dock(vd_ctx* c, int id, const vd_dock_params* p, float* e, float* conf, f
     check_params(p);
     if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no maps");
     if (st) std::memset(st, 0, sizeof(*st));
+    if (p->chains_per_wave == 0 || p->waves_per_simd == 0) autotune(c, {id}, p);
     LigPlan pl = plan_for(c, id, p);
     prepare_batch(c, {pl});
     std::vector<int> dv{pl.depth}, bv{pl.bfgs};
@@ -649,7 +702,7 @@ int vd_dock(vd_ctx* c, int id, const vd_dock_params* p, float* e, float* conf, f
     const int cpw = 64 / pl.G;
     std::vector<int2> items;
     for (int c0 = 0; c0 < p->chains; c0 += cpw) items.push_back(make_int2(0, c0));
-    run_mc(c, pl.G, pk, {(uint32_t)id}, items, c->batch.off, make_consts(c, p, pl.depth, pl.bfgs), c->batch.e.p,
+    run_mc(c, pl.G, pl.W, pk, {(uint32_t)id}, items, c->batch.off, make_consts(c, p, pl.depth, pl.bfgs), c->batch.e.p,
            c->batch.conf.p, c->batch.heavy.p, false, st, true);
     if (e) HIP_CHECK(hipMemcpy(e, c->batch.e.p, (size_t)p->chains * 4, hipMemcpyDeviceToHost));
     if (conf) HIP_CHECK(hipMemcpy(conf, c->batch.conf.p, (size_t)p->chains * (7 + pl.T) * 4, hipMemcpyDeviceToHost));
@@ -664,6 +717,7 @@ static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, 
     check_params(p);
     if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no maps");
     if (st) std::memset(st, 0, sizeof(*st));
+    if (p->chains_per_wave == 0 || p->waves_per_simd == 0) autotune(c, std::vector<int>(ids, ids + n), p);
     std::vector<LigPlan> plans;
     for (int k = 0; k < n; ++k) plans.push_back(plan_for(c, ids[k], p));
     prepare_batch(c, plans);
@@ -678,16 +732,16 @@ static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, 
         std::vector<int2> items;
         for (int c0 = 0; c0 < p->chains; c0 += cpw) items.push_back(make_int2(0, c0));
         std::vector<longlong4> off{c->batch.off[k]};
-        run_mc(c, pl.G, pk, {(uint32_t)pl.id}, items, off, make_consts(c, p, pl.depth, pl.bfgs), c->batch.e.p,
+        run_mc(c, pl.G, pl.W, pk, {(uint32_t)pl.id}, items, off, make_consts(c, p, pl.depth, pl.bfgs), c->batch.e.p,
                c->batch.conf.p, c->batch.heavy.p, false, st, true);
       }
       return;
     }
     // one persistent launch per group width; items ordered by decreasing estimated cost
-    std::map<int, std::vector<int>> classes;
-    for (int k = 0; k < n; ++k) classes[plans[k].G].push_back(k);
+    std::map<std::pair<int, int>, std::vector<int>> classes;
+    for (int k = 0; k < n; ++k) classes[std::make_pair(plans[k].G, plans[k].W)].push_back(k);
     for (auto& cl : classes) {
-      const int G = cl.first;
+      const int G = cl.first.first, W = cl.first.second;
       std::vector<int>& ks = cl.second;
       std::vector<int> lig_ids, dv, bv;
       for (int k : ks) { lig_ids.push_back(plans[k].id); dv.push_back(plans[k].depth); bv.push_back(plans[k].bfgs); }
@@ -704,7 +758,7 @@ static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, 
       std::vector<int2> items;
       for (auto& o : order)
         for (int c0 = 0; c0 < p->chains; c0 += cpw) items.push_back(make_int2(o.second, c0));
-      run_mc(c, G, pk, streams, items, off, make_consts(c, p, 0, 0), c->batch.e.p, c->batch.conf.p, c->batch.heavy.p,
+      run_mc(c, G, W, pk, streams, items, off, make_consts(c, p, 0, 0), c->batch.e.p, c->batch.conf.p, c->batch.heavy.p,
              true, st, true);
     }
   });
@@ -753,6 +807,39 @@ int vd_cluster(int n, int n_heavy, const float* e, const float* heavy, double mi
   std::vector<int> k = vd::cluster_poses(n, n_heavy, e, heavy, min_rmsd, max_size);
   for (size_t i = 0; i < k.size(); ++i) kept[i] = k[i];
   return (int)k.size();
+}
+
+int vd_device_alloc(vd_ctx* c, size_t bytes, void** ptr) {
+  if (!c || !ptr) return VD_ERR_ARG;
+  return guard(c, [&]() { need_device(c); *ptr = nullptr; HIP_CHECK(hipMalloc(ptr, std::max<size_t>(bytes, 1))); });
+}
+
+int vd_device_free(vd_ctx* c, void* ptr) {
+  if (!c) return VD_ERR_ARG;
+  return guard(c, [&]() { need_device(c); if (ptr) HIP_CHECK(hipFree(ptr)); });
+}
+
+int vd_copy(vd_ctx* c, void* dst, const void* src, size_t bytes, int direction) {
+  if (!c || (bytes && (!dst || !src))) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    need_device(c);
+    const hipMemcpyKind k = direction == 0 ? hipMemcpyHostToDevice : direction == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, k, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int vd_random_confs(vd_ctx* c, int id, long long n, uint32_t seed, float* d_conf) {
+  if (!c || n < 0 || (n > 0 && !d_conf)) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    need_device(c);
+    if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no box: populate or upload maps first");
+    const vd::Ligand& L = lig_at(c, id);
+    float lo[3], hi[3];
+    for (int i = 0; i < 3; ++i) { lo[i] = (float)c->box.begin[i]; hi[i] = (float)c->box.end[i]; }
+    if (n) HIP_CHECK(vdl::random_confs(d_conf, n, L.n_torsions(), seed, lo, hi, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
 }
 
 int vd_synth_receptor(int n_atoms, double cube, double cavity_radius, uint64_t seed, float* xyz, int* xs) {

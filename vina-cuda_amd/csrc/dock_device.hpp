@@ -42,14 +42,29 @@ __device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint
 __device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
 
 // ----------------------------------------------------------------------------------------
-// Group collectives (G divides 64; xor masks < G stay inside the group)
+// Group collectives (G divides 64).  Sums: a row-of-16 all-reduce with DPP (quad_perm xor 1,
+// xor 2, row_half_mirror, row_mirror), then xor-16/xor-32 lane shuffles across rows.  Every step
+// adds the same two operands in both partner lanes, so all lanes of a group hold bit-identical
+// sums (group decisions taken from them agree in every lane).
+template <int CTRL> __device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
 template <int G> __device__ __forceinline__ float gsum(float v) {
-#pragma unroll
-  for (int m = G / 2; m > 0; m >>= 1) v += __shfl_xor(v, m);
+  v += dpp<0xB1>(v);    // quad_perm [1,0,3,2]
+  v += dpp<0x4E>(v);    // quad_perm [2,3,0,1]
+  v += dpp<0x141>(v);   // row_half_mirror
+  v += dpp<0x140>(v);   // row_mirror
+  if constexpr (G >= 32) v += __shfl_xor(v, 16);
+  if constexpr (G >= 64) v += __shfl_xor(v, 32);
   return v;
 }
 template <int G> __device__ __forceinline__ float gget(float v, int src) { return __shfl(v, src, G); }
 template <int G> __device__ __forceinline__ int ggeti(int v, int src) { return __shfl(v, src, G); }
+// broadcast from a fixed lane of the group (a wave-uniform readlane when the group is the wave)
+template <int G, int SRC> __device__ __forceinline__ float gbcast(float v) {
+  if constexpr (G == 64) return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), SRC));
+  else return __shfl(v, SRC, G);
+}
 
 // ----------------------------------------------------------------------------------------
 // Rotations.  Quaternions are (w, x, y, z); products are Hamilton products.
@@ -107,9 +122,9 @@ __device__ __forceinline__ Quat qincrement(Quat q, float rx, float ry, float rz)
 // ----------------------------------------------------------------------------------------
 struct LigView {               // one ligand, staged in LDS for the wave
   const float4* atom;          // A: local xyz + meta (node | xs << 8 | heavy << 16)
-  const float4* node_o;        // N: rel_origin xyz + parent
+  const float4* node_o;        // N: rel_origin xyz + bits (parent+1 | atom begin << 8 | atom end << 16)
   const float4* node_a;        // N: rel_axis xyz + subtree_end
-  const uint32_t* pairs;       // P, in global memory (L2 resident)
+  const uint32_t* pairs;       // P, staged in LDS
   int A, N, P, T, H, rounds;
 };
 struct Scratch {               // per chain, LDS
@@ -117,6 +132,7 @@ struct Scratch {               // per chain, LDS
   float* frc;                  // 3A, dE/dx
   float4* frame;               // 3N rows (R row, origin component)
   float4* nacc;                // 2N: per-node force and moment sums
+  float* stash;                // per wave: conformations parked out of registers, [slot][field][lane]
 };
 struct EvalK {
   const float* __restrict__ maps; vd_grid_desc gd;
@@ -183,7 +199,7 @@ __device__ __forceinline__ void kinematics(const LigView& L, const Scratch& S, c
     rel_axis = make_float3(na.x, na.y, na.z);
     q = axis_angle(rel_axis, c.tor);
     t = make_float3(no.x, no.y, no.z);
-    anc = (int)fbits(no.w);
+    anc = (int)(fbits(no.w) & 255u) - 1;
   }
   for (int r = 0; r < L.rounds; ++r) {
     const int src = anc >= 0 ? anc : l;
@@ -207,8 +223,6 @@ __device__ __forceinline__ void kinematics(const LigView& L, const Scratch& S, c
     S.frame[3 * l + 0] = make_float4(m[0], m[1], m[2], t.x);
     S.frame[3 * l + 1] = make_float4(m[3], m[4], m[5], t.y);
     S.frame[3 * l + 2] = make_float4(m[6], m[7], m[8], t.z);
-    S.nacc[2 * l + 0] = make_float4(0, 0, 0, 0);
-    S.nacc[2 * l + 1] = make_float4(0, 0, 0, 0);
   }
 }
 
@@ -258,48 +272,85 @@ __device__ float eval_deriv(const LigView& L, const Scratch& S, const EvalK& K, 
   // intermolecular term: trilinear maps (cache::eval_deriv, V11/lib/cache.cpp:73-88)
   const float3 prel = make_float3(c.px - K.gd.init[0], c.py - K.gd.init[1], c.pz - K.gd.init[2]);
   float e_inter = 0.0f;
-  for (int i = l; i < L.A; i += G) {
-    uint32_t meta; const float3 y = atom_position(L, S, i, meta);
-    S.xyz[3 * i] = y.x; S.xyz[3 * i + 1] = y.y; S.xyz[3 * i + 2] = y.z;
-    const int xs = (meta >> 8) & 255;
-    float3 d = make_float3(0, 0, 0);
-    if (xs < VD_XS_SIZE)
-      e_inter += map_eval(K.maps + (size_t)xs * K.gd.plane, K.gd, make_float3(prel.x + y.x, prel.y + y.y, prel.z + y.z),
-                          K.slope, K.v_inter, d);
-    S.frc[3 * i] = d.x; S.frc[3 * i + 1] = d.y; S.frc[3 * i + 2] = d.z;
-  }
-  __syncthreads();
-  // intramolecular pairs (eval_interacting_pairs_deriv, V11/lib/model.cpp:1241-1259)
-  float e_intra = 0.0f;
-  for (int p = l; p < L.P; p += G) {
-    const uint32_t w = L.pairs[p];
-    const int a = w & 255, b = (w >> 8) & 255, tp = w >> 16;
-    const float rx = S.xyz[3 * b] - S.xyz[3 * a], ry = S.xyz[3 * b + 1] - S.xyz[3 * a + 1], rz = S.xyz[3 * b + 2] - S.xyz[3 * a + 2];
-    const float r2 = rx * rx + ry * ry + rz * rz;
-    if (r2 < VD_CUTOFF_SQR) {
-      const float rf = VD_LUT_FACTOR * r2;
-      const int i1 = (int)rf;
-      const float rem = rf - (float)i1;
-      const float4 q = K.lutq[tp * VD_LUT_BINS + i1];
-      float e = q.x + rem * (q.z - q.x);
-      float dor = q.y + rem * (q.w - q.y);
-      if (e > 0.0f) { const float t = (K.v_intra < kEpsFl) ? 0.0f : K.v_intra / (K.v_intra + e); e *= t; dor *= t * t; }
-      e_intra += e;
-      const float fx = dor * rx, fy = dor * ry, fz = dor * rz;
-      atomicAdd(&S.frc[3 * a], -fx); atomicAdd(&S.frc[3 * a + 1], -fy); atomicAdd(&S.frc[3 * a + 2], -fz);
-      atomicAdd(&S.frc[3 * b], fx); atomicAdd(&S.frc[3 * b + 1], fy); atomicAdd(&S.frc[3 * b + 2], fz);
+  // two atoms per lane in flight: both sets of map reads are issued before either is consumed
+  for (int i0 = l; i0 < L.A; i0 += 2 * G) {
+    float3 d[2], y[2];
+    float ev[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int i = i0 + u * G;
+      d[u] = make_float3(0, 0, 0); ev[u] = 0.0f; y[u] = make_float3(0, 0, 0);
+      if (i < L.A) {
+        uint32_t meta; y[u] = atom_position(L, S, i, meta);
+        const int xs = (meta >> 8) & 255;
+        if (xs < VD_XS_SIZE)
+          ev[u] = map_eval(K.maps + (size_t)xs * K.gd.plane, K.gd, make_float3(prel.x + y[u].x, prel.y + y[u].y, prel.z + y[u].z),
+                           K.slope, K.v_inter, d[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int i = i0 + u * G;
+      if (i < L.A) {
+        S.xyz[3 * i] = y[u].x; S.xyz[3 * i + 1] = y[u].y; S.xyz[3 * i + 2] = y[u].z;
+        S.frc[3 * i] = d[u].x; S.frc[3 * i + 1] = d[u].y; S.frc[3 * i + 2] = d[u].z;
+        e_inter += ev[u];
+      }
     }
   }
   __syncthreads();
-  // adjoint: per-node force and moment (about the root origin) sums
-  for (int i = l; i < L.A; i += G) {
-    const uint32_t meta = fbits(L.atom[i].w);
-    if (((meta >> 8) & 255) >= VD_XS_SIZE) continue;   // untyped atoms carry no force
-    const float fx = S.frc[3 * i], fy = S.frc[3 * i + 1], fz = S.frc[3 * i + 2];
-    const float x = S.xyz[3 * i], y = S.xyz[3 * i + 1], z = S.xyz[3 * i + 2];
-    float* acc = reinterpret_cast<float*>(&S.nacc[2 * (meta & 255)]);
-    atomicAdd(acc + 0, fx); atomicAdd(acc + 1, fy); atomicAdd(acc + 2, fz);
-    atomicAdd(acc + 4, y * fz - z * fy); atomicAdd(acc + 5, z * fx - x * fz); atomicAdd(acc + 6, x * fy - y * fx);
+  // intramolecular pairs (eval_interacting_pairs_deriv, V11/lib/model.cpp:1241-1259), four per
+  // lane in flight: all four table reads are issued before the first is used
+  float e_intra = 0.0f;
+  constexpr int U = 4;
+  for (int p0 = l; p0 < L.P; p0 += U * G) {
+    float rx[U], ry[U], rz[U], rem[U];
+    int a[U], b[U];
+    bool ok[U];
+    float4 q[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int p = p0 + u * G;
+      ok[u] = p < L.P;
+      const uint32_t w = ok[u] ? L.pairs[p] : 0u;
+      a[u] = w & 255; b[u] = (w >> 8) & 255;
+      const int tp = w >> 16;
+      rx[u] = S.xyz[3 * b[u]] - S.xyz[3 * a[u]]; ry[u] = S.xyz[3 * b[u] + 1] - S.xyz[3 * a[u] + 1];
+      rz[u] = S.xyz[3 * b[u] + 2] - S.xyz[3 * a[u] + 2];
+      const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
+      ok[u] = ok[u] && r2 < VD_CUTOFF_SQR;
+      const float rf = VD_LUT_FACTOR * r2;
+      const int i1 = ok[u] ? (int)rf : 0;
+      rem[u] = rf - (float)i1;
+      q[u] = ok[u] ? K.lutq[tp * VD_LUT_BINS + i1] : make_float4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!ok[u]) continue;
+      float e = q[u].x + rem[u] * (q[u].z - q[u].x);
+      float dor = q[u].y + rem[u] * (q[u].w - q[u].y);
+      if (e > 0.0f) { const float t = (K.v_intra < kEpsFl) ? 0.0f : K.v_intra / (K.v_intra + e); e *= t; dor *= t * t; }
+      e_intra += e;
+      const float fx = dor * rx[u], fy = dor * ry[u], fz = dor * rz[u];
+      atomicAdd(&S.frc[3 * a[u]], -fx); atomicAdd(&S.frc[3 * a[u] + 1], -fy); atomicAdd(&S.frc[3 * a[u] + 2], -fz);
+      atomicAdd(&S.frc[3 * b[u]], fx); atomicAdd(&S.frc[3 * b[u] + 1], fy); atomicAdd(&S.frc[3 * b[u] + 2], fz);
+    }
+  }
+  __syncthreads();
+  // adjoint: per-node force and moment (about the root origin) sums; a node's atoms are the
+  // contiguous range [begin, end), summed in order by the node's lane (untyped atoms carry 0)
+  if (l < L.N) {
+    const uint32_t nb = fbits(L.node_o[l].w);
+    const int begin = (nb >> 8) & 255, end = (nb >> 16) & 255;
+    float f0 = 0, f1 = 0, f2 = 0, m0 = 0, m1 = 0, m2 = 0;
+    for (int i = begin; i < end; ++i) {
+      const float fx = S.frc[3 * i], fy = S.frc[3 * i + 1], fz = S.frc[3 * i + 2];
+      const float x = S.xyz[3 * i], y = S.xyz[3 * i + 1], z = S.xyz[3 * i + 2];
+      f0 += fx; f1 += fy; f2 += fz;
+      m0 += y * fz - z * fy; m1 += z * fx - x * fz; m2 += x * fy - y * fx;
+    }
+    S.nacc[2 * l] = make_float4(f0, f1, f2, 0.0f);
+    S.nacc[2 * l + 1] = make_float4(m0, m1, m2, 0.0f);
   }
   __syncthreads();
   // subtree sums: the subtree of node k is the preorder range [k, subtree_end_k); the torque
@@ -316,8 +367,8 @@ __device__ float eval_deriv(const LigView& L, const Scratch& S, const EvalK& K, 
   const float ty = M[1] - (org.z * F[0] - org.x * F[2]);
   const float tz = M[2] - (org.x * F[1] - org.y * F[0]);
   const float gt = tx * ax.x + ty * ax.y + tz * ax.z;
-  const float r0 = gget<G>(F[0], 0), r1 = gget<G>(F[1], 0), r2 = gget<G>(F[2], 0);
-  const float r3 = gget<G>(tx, 0), r4 = gget<G>(ty, 0), r5 = gget<G>(tz, 0);
+  const float r0 = gbcast<G, 0>(F[0]), r1 = gbcast<G, 0>(F[1]), r2 = gbcast<G, 0>(F[2]);
+  const float r3 = gbcast<G, 0>(tx), r4 = gbcast<G, 0>(ty), r5 = gbcast<G, 0>(tz);
   const int c6 = l - (G - 6);
   g = (l >= 1 && l <= L.T) ? gt
       : c6 == 0 ? r0 : c6 == 1 ? r1 : c6 == 2 ? r2 : c6 == 3 ? r3 : c6 == 4 ? r4 : c6 == 5 ? r5 : 0.0f;
@@ -329,8 +380,8 @@ __device__ float eval_deriv(const LigView& L, const Scratch& S, const EvalK& K, 
 template <int G>
 __device__ __forceinline__ Conf increment(const Conf& x, float d, float factor, int l, int T) {
   Conf y = x;
-  const float d0 = gget<G>(d, G - 6), d1 = gget<G>(d, G - 5), d2 = gget<G>(d, G - 4);
-  const float d3 = gget<G>(d, G - 3), d4 = gget<G>(d, G - 2), d5 = gget<G>(d, G - 1);
+  const float d0 = gbcast<G, G - 6>(d), d1 = gbcast<G, G - 5>(d), d2 = gbcast<G, G - 4>(d);
+  const float d3 = gbcast<G, G - 3>(d), d4 = gbcast<G, G - 2>(d), d5 = gbcast<G, G - 1>(d);
   y.px += factor * d0; y.py += factor * d1; y.pz += factor * d2;
   y.q = qincrement(y.q, factor * d3, factor * d4, factor * d5);
   if (l >= 1 && l <= T) y.tor = normalize_angle(y.tor + normalize_angle(factor * d));
@@ -338,6 +389,21 @@ __device__ __forceinline__ Conf increment(const Conf& x, float d, float factor, 
 }
 
 __device__ __forceinline__ Conf csel(bool p, const Conf& a, const Conf& b) { return p ? a : b; }
+
+// Conformations that are only read now and then live in LDS instead of registers (the MC kernel
+// is register-bound); slot s of lane `lane` is fields [s][0..7][lane].
+constexpr int kStashSlots = 4;
+enum { kSlotOrig = 0, kSlotPrev = 1, kSlotTmp = 2, kSlotBest = 3 };
+__device__ __forceinline__ void stash_put(float* st, int slot, const Conf& c) {
+  float* p = st + slot * 512 + (threadIdx.x & 63);
+  p[0] = c.px; p[64] = c.py; p[128] = c.pz; p[192] = c.q.w; p[256] = c.q.x; p[320] = c.q.y; p[384] = c.q.z; p[448] = c.tor;
+}
+__device__ __forceinline__ Conf stash_get(const float* st, int slot) {
+  const float* p = st + slot * 512 + (threadIdx.x & 63);
+  Conf c;
+  c.px = p[0]; c.py = p[64]; c.pz = p[128]; c.q = Quat{p[192], p[256], p[320], p[384]}; c.tor = p[448];
+  return c;
+}
 
 // ----------------------------------------------------------------------------------------
 // RILC-BFGS local search with the Lewis-Overton line search
@@ -350,13 +416,15 @@ __device__ float rilc_bfgs(const LigView& L, const Scratch& S, const EvalK& K, C
   float g;
   float fx = eval_deriv<G>(L, S, K, x, l, g);
   evals += enabled ? 1 : 0;
-  const float fx_orig = fx; const Conf x_orig = x;
+  const float fx_orig = fx;
+  stash_put(S.stash, kSlotOrig, x);
   float d = -g;
   bool active = enabled && (sqrtf(gsum<G>(g * g)) >= 1e-5f);
   float step = 1.0f / sqrtf(gsum<G>(d * d));
   int k = 1;
   while (__any(active)) {
-    const Conf xp = x; const float gp = g; const float fxp = fx;
+    stash_put(S.stash, kSlotPrev, x);
+    const float gp = g; const float fxp = fx;
     // line search
     const float dginit = gsum<G>(gp * d);
     bool ls = active && !(0.0f < dginit);
@@ -364,7 +432,7 @@ __device__ float rilc_bfgs(const LigView& L, const Scratch& S, const EvalK& K, C
     const float finit = fx, dgtest = 1e-4f * dginit, dstest = 0.1f * dginit;
     float mu = 0.0f, nu = 1.0e20f; bool brackt = false; int count = 0;
     while (__any(ls)) {
-      const Conf xt = increment<G>(xp, d, step, l, L.T);
+      const Conf xt = increment<G>(stash_get(S.stash, kSlotPrev), d, step, l, L.T);
       float gt;
       const float ft = eval_deriv<G>(L, S, K, csel(ls, xt, x), l, gt);
       const float gtd = gsum<G>(gt * d);
@@ -382,7 +450,7 @@ __device__ float rilc_bfgs(const LigView& L, const Scratch& S, const EvalK& K, C
     const float gnorm = sqrtf(gsum<G>(g * g));
     bool update = false;
     if (active) {
-      if (result < 0) { x = xp; g = gp; fx = fxp; active = false; }
+      if (result < 0) { x = stash_get(S.stash, kSlotPrev); g = gp; fx = fxp; active = false; }
       else if (!(gnorm >= 1e-5f)) active = false;
       else if ((max_steps != 0 && max_steps <= k) || k >= 1000) active = false;   // 1000: hard bound
       else { ++k; update = true; }
@@ -397,7 +465,7 @@ __device__ float rilc_bfgs(const LigView& L, const Scratch& S, const EvalK& K, C
     const float nd3 = nd2 + (alpha - beta) * s;
     if (update) { d = (ys > ss * sqrtf(gpgp) * 1.0e-6f) ? nd3 : nd; step = 1.0f; }
   }
-  if (enabled && !(fx <= fx_orig)) { fx = fx_orig; x = x_orig; }
+  if (enabled && !(fx <= fx_orig)) { fx = fx_orig; x = stash_get(S.stash, kSlotOrig); }
   return enabled ? fx : x_e;
 }
 

@@ -443,14 +443,21 @@ vd_lig_header pack_ligand(const Ligand& L, std::vector<float>& blob) {
   h.off_nodes = (int)(blob.size() / 4);
   for (const LigNode& nd : L.nodes) {
     blob.push_back((float)nd.rel_origin[0]); blob.push_back((float)nd.rel_origin[1]); blob.push_back((float)nd.rel_origin[2]);
-    blob.push_back(as_f((uint32_t)nd.parent));
+    blob.push_back(as_f((uint32_t)(nd.parent + 1) | ((uint32_t)nd.begin << 8) | ((uint32_t)nd.end << 16)));
   }
   for (const LigNode& nd : L.nodes) {
     blob.push_back((float)nd.rel_axis[0]); blob.push_back((float)nd.rel_axis[1]); blob.push_back((float)nd.rel_axis[2]);
     blob.push_back(as_f((uint32_t)nd.subtree_end));
   }
+  // device pair order: by diagonal (b - a, then a), so that the lanes of one pass over the list
+  // scatter forces to distinct atoms; only the order of the energy sum changes
+  std::vector<uint32_t> pairs(L.pairs);
+  std::stable_sort(pairs.begin(), pairs.end(), [](uint32_t x, uint32_t y) {
+    const int dx = (int)((x >> 8) & 255) - (int)(x & 255), dy = (int)((y >> 8) & 255) - (int)(y & 255);
+    return dx != dy ? dx < dy : (x & 255) < (y & 255);
+  });
   h.off_pairs = (int)blob.size();
-  for (uint32_t p : L.pairs) blob.push_back(as_f(p));
+  for (uint32_t p : pairs) blob.push_back(as_f(p));
   pad16();
   return h;
 }

@@ -24,7 +24,7 @@ struct McArgs {
   float* out_e; float* out_conf; float* out_heavy;
   const longlong4* out_off;         // per ligand slot: (e, conf, heavy) element offsets
   unsigned long long* eval_count;
-  int Amax, Nmax;
+  int Amax, Nmax, Pmax;
 };
 
 struct EvalArgs {
@@ -37,11 +37,12 @@ struct EvalArgs {
 };
 
 namespace vdl {
-int mc_lds_bytes(int G, int Amax, int Nmax, int dense);
-int max_resident_mc_blocks(int G, int mode, int lds_bytes);
+int mc_lds_bytes(int G, int Amax, int Nmax, int Pmax, int dense);
+int max_resident_mc_blocks(int G, int W, int mode, int lds_bytes);
 hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& gc, const float4* lut_rows, float* maps,
                        const vd_grid_desc& gd, hipStream_t s);
 hipError_t launch_eval(int G, const EvalArgs& a, int blocks, hipStream_t s);
-hipError_t launch_mc(int G, const McArgs& a, int blocks, hipStream_t s);
+hipError_t launch_mc(int G, int W, const McArgs& a, int blocks, hipStream_t s);
 hipError_t reset_counter(int* p, hipStream_t s);
+hipError_t random_confs(float* conf, long long n, int T, uint32_t seed, const float lo[3], const float hi[3], hipStream_t s);
 }  // namespace vdl

@@ -78,10 +78,10 @@ __global__ __launch_bounds__(256) void vd_grid_kernel(const float4* __restrict__
 }
 
 // ------------------------------------------------------------------------------------------
-// LDS layout of one wave: ligand (atoms 4A, node_o 4N, node_a 4N floats), then per chain group
+// LDS layout of one wave: ligand (atoms 4A, node_o 4N, node_a 4N floats, pair words P), then per chain group
 // frame 12N, nacc 8N, xyz 3A (padded to 4), frc 3A (padded), and for dense BFGS H G*G + 2G.
 __host__ __device__ inline int pad4(int n) { return (n + 3) & ~3; }
-__host__ __device__ inline int lig_floats(int A, int N) { return 4 * A + 8 * N; }
+__host__ __device__ inline int lig_floats(int A, int N, int P) { return 4 * A + 8 * N + pad4(P) + kStashSlots * 512; }
 __host__ __device__ inline int group_floats(int A, int N, int G, int dense) {
   return 12 * N + 8 * N + 2 * pad4(3 * A) + (dense ? G * G + 2 * G : 0);
 }
@@ -91,16 +91,20 @@ __device__ __forceinline__ void stage_ligand(const float4* __restrict__ blob, co
   float4* dst = reinterpret_cast<float4*>(lds);
   for (int i = lane; i < h.n_atoms; i += 64) dst[i] = blob[h.off_atoms + i];
   for (int i = lane; i < 2 * h.n_nodes; i += 64) dst[h.n_atoms + i] = blob[h.off_nodes + i];
+  uint32_t* pw = reinterpret_cast<uint32_t*>(dst + h.n_atoms + 2 * h.n_nodes);
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(blob) + h.off_pairs;
+  for (int i = lane; i < h.n_pairs; i += 64) pw[i] = src[i];
 }
 
 template <int G>
 __device__ __forceinline__ void make_views(const float4* __restrict__ blob, const vd_lig_header& h, float* lds, int Amax,
-                                           int Nmax, int dense, int group, LigView& L, Scratch& S, float*& H) {
+                                           int Nmax, int Pmax, int dense, int group, LigView& L, Scratch& S, float*& H) {
   float4* base = reinterpret_cast<float4*>(lds);
   L.atom = base; L.node_o = base + h.n_atoms; L.node_a = base + h.n_atoms + h.n_nodes;
-  L.pairs = reinterpret_cast<const uint32_t*>(blob) + h.off_pairs;
+  L.pairs = reinterpret_cast<const uint32_t*>(base + h.n_atoms + 2 * h.n_nodes);
   L.A = h.n_atoms; L.N = h.n_nodes; L.P = h.n_pairs; L.T = h.n_torsions; L.H = h.n_heavy; L.rounds = h.jump_rounds;
-  float* g = lds + lig_floats(Amax, Nmax) + group * group_floats(Amax, Nmax, G, dense);
+  S.stash = lds + lig_floats(Amax, Nmax, Pmax) - kStashSlots * 512;
+  float* g = lds + lig_floats(Amax, Nmax, Pmax) + group * group_floats(Amax, Nmax, G, dense);
   S.frame = reinterpret_cast<float4*>(g);
   S.nacc = reinterpret_cast<float4*>(g + 12 * Nmax);
   S.xyz = g + 20 * Nmax;
@@ -160,7 +164,7 @@ __device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const Eva
   set_coords<G>(L, S, tmp, l);
   float rg = gyration_radius<G>(L, S, l);
   float best_e = INFINITY;
-  Conf best = tmp;
+  stash_put(S.stash, kSlotBest, tmp);
   const int n_ent = P.mutation_quirk ? (1 + L.T) : (2 + L.T);
   for (int step = 0; step < depth; ++step) {
     const U4 w = philox(chain, (uint32_t)step, 0, 0, k0, k1);
@@ -174,21 +178,23 @@ __device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const Eva
       if (need && vx * vx + vy * vy + vz * vz < 1.0f) { sx = vx; sy = vy; sz = vz; need = false; }
     }
     Conf cand = tmp;
+    stash_put(S.stash, kSlotTmp, tmp);
     if (which == 0) { cand.px += P.amplitude * sx; cand.py += P.amplitude * sy; cand.pz += P.amplitude * sz; }
     else if (which == 1) { if (rg > kEpsFl) { const float f = P.amplitude / rg; cand.q = qincrement(cand.q, f * sx, f * sy, f * sz); } }
     else if (l == which - 1) cand.tor = tor;
     const float ce = local_search<G, MODE>(L, S, H, K, cand, valid, 0.0f, bfgs_steps, l, evals);
     const bool accept = valid && (step == 0 || ce < tmp_e || u_metro < expf((tmp_e - ce) / P.temperature));
-    if (accept) { tmp = cand; tmp_e = ce; }
+    if (accept) { tmp = cand; tmp_e = ce; } else tmp = stash_get(S.stash, kSlotTmp);
     const bool second = accept && tmp_e < best_e;
     const float e2 = local_search<G, MODE>(L, S, H, K, tmp, second, tmp_e, bfgs_steps, l, evals);
-    if (second) { tmp_e = e2; if (tmp_e < best_e) { best = tmp; best_e = tmp_e; } }
+    if (second) { tmp_e = e2; if (tmp_e < best_e) { stash_put(S.stash, kSlotBest, tmp); best_e = tmp_e; } }
     if (__any(accept)) {
       set_coords<G>(L, S, tmp, l);
       const float r = gyration_radius<G>(L, S, l);
       if (accept) rg = r;
     }
   }
+  const Conf best = stash_get(S.stash, kSlotBest);
   set_coords<G>(L, S, best, l);
   if (valid) {
     if (l == 0) {
@@ -208,8 +214,10 @@ __device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const Eva
   }
 }
 
-template <int G, int MODE>
-__global__ __launch_bounds__(64) void vd_mc_kernel(McArgs a) {
+// W = minimum waves per SIMD the register allocation must allow (2 = unconstrained); a tuning knob:
+// more resident waves hide more latency, at the price of register spills.
+template <int G, int MODE, int W>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void vd_mc_kernel(McArgs a) {
   extern __shared__ float4 lds4[];
   float* lds = reinterpret_cast<float*>(lds4);
   constexpr int CPW = 64 / G;
@@ -235,7 +243,7 @@ __global__ __launch_bounds__(64) void vd_mc_kernel(McArgs a) {
       cur = wi.x;
     }
     LigView L; Scratch S; float* H;
-    make_views<G>(a.blob, h, lds, a.Amax, a.Nmax, a.P.local_search == 0, group, L, S, H);
+    make_views<G>(a.blob, h, lds, a.Amax, a.Nmax, a.Pmax, MODE == 0, group, L, S, H);
     const uint32_t chain = (uint32_t)(wi.y + group);
     const bool valid = (int)chain < a.P.chains;
     int evals = 0;
@@ -260,7 +268,7 @@ __global__ __launch_bounds__(64) void vd_eval_kernel(EvalArgs a) {
   stage_ligand<G>(a.blob, h, lds, lane);
   __syncthreads();
   LigView L; Scratch S; float* H;
-  make_views<G>(a.blob, h, lds, h.n_atoms, h.n_nodes, 0, group, L, S, H);
+  make_views<G>(a.blob, h, lds, h.n_atoms, h.n_nodes, h.n_pairs, 0, group, L, S, H);
   const int T = h.n_torsions, nc = 7 + T, ng = 6 + T;
   for (long long base = (long long)blockIdx.x * CPW; base < a.n_conf; base += (long long)gridDim.x * CPW) {
     const long long c = base + group;
@@ -284,8 +292,8 @@ __global__ __launch_bounds__(64) void vd_eval_kernel(EvalArgs a) {
 
 namespace vdl {
 
-int mc_lds_bytes(int G, int Amax, int Nmax, int dense) {
-  return 4 * (lig_floats(Amax, Nmax) + (64 / G) * group_floats(Amax, Nmax, G, dense));
+int mc_lds_bytes(int G, int Amax, int Nmax, int Pmax, int dense) {
+  return 4 * (lig_floats(Amax, Nmax, Pmax) + (64 / G) * group_floats(Amax, Nmax, G, dense));
 }
 
 hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& gc, const float4* lut_rows, float* maps,
@@ -296,8 +304,8 @@ hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& 
 }
 
 hipError_t launch_eval(int G, const EvalArgs& a, int blocks, hipStream_t s) {
-  const int A = a.host_header.n_atoms, N = a.host_header.n_nodes;
-  const int lds = mc_lds_bytes(G, A, N, 0);
+  const int A = a.host_header.n_atoms, N = a.host_header.n_nodes, P = a.host_header.n_pairs;
+  const int lds = mc_lds_bytes(G, A, N, P, 0);
   switch (G) {
     case 64: hipLaunchKernelGGL(vd_eval_kernel<64>, dim3(blocks), dim3(64), lds, s, a); break;
     case 32: hipLaunchKernelGGL(vd_eval_kernel<32>, dim3(blocks), dim3(64), lds, s, a); break;
@@ -307,43 +315,79 @@ hipError_t launch_eval(int G, const EvalArgs& a, int blocks, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int MODE>
+template <int MODE, int W>
 static hipError_t launch_mc_mode(int G, const McArgs& a, int blocks, int lds, hipStream_t s) {
   switch (G) {
-    case 64: hipLaunchKernelGGL((vd_mc_kernel<64, MODE>), dim3(blocks), dim3(64), lds, s, a); break;
-    case 32: hipLaunchKernelGGL((vd_mc_kernel<32, MODE>), dim3(blocks), dim3(64), lds, s, a); break;
-    case 16: hipLaunchKernelGGL((vd_mc_kernel<16, MODE>), dim3(blocks), dim3(64), lds, s, a); break;
+    case 64: hipLaunchKernelGGL((vd_mc_kernel<64, MODE, W>), dim3(blocks), dim3(64), lds, s, a); break;
+    case 32: hipLaunchKernelGGL((vd_mc_kernel<32, MODE, W>), dim3(blocks), dim3(64), lds, s, a); break;
+    case 16: hipLaunchKernelGGL((vd_mc_kernel<16, MODE, W>), dim3(blocks), dim3(64), lds, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_mc(int G, const McArgs& a, int blocks, hipStream_t s) {
-  const int lds = mc_lds_bytes(G, a.Amax, a.Nmax, a.P.local_search == 0);
-  return a.P.local_search == 1 ? launch_mc_mode<1>(G, a, blocks, lds, s) : launch_mc_mode<0>(G, a, blocks, lds, s);
+hipError_t launch_mc(int G, int W, const McArgs& a, int blocks, hipStream_t s) {
+  const int lds = mc_lds_bytes(G, a.Amax, a.Nmax, a.Pmax, a.P.local_search == 0);
+  if (a.P.local_search == 0) return launch_mc_mode<0, 2>(G, a, blocks, lds, s);
+  switch (W) {
+    case 3: return launch_mc_mode<1, 3>(G, a, blocks, lds, s);
+    case 4: return launch_mc_mode<1, 4>(G, a, blocks, lds, s);
+    default: return launch_mc_mode<1, 2>(G, a, blocks, lds, s);
+  }
 }
 
 __global__ void vd_fill_int(int* p, int v) { *p = v; }
+
+// Random conformations of the adjoint benchmark: translation uniform in the box, orientation a
+// normalised 4-Gaussian (uniform on rotations), torsions uniform in [-pi, pi).
+__global__ void vd_random_confs_kernel(float* conf, long long n, int T, uint32_t seed, float3 lo, float3 hi) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  float* x = conf + c * (7 + T);
+  const uint32_t cl = (uint32_t)c, ch = (uint32_t)(c >> 32);
+  U4 w = philox(cl, ch, 0xFFFFFFFEu, 0, seed, 0x5EEDu);
+  x[0] = lo.x + (hi.x - lo.x) * u01(w.x); x[1] = lo.y + (hi.y - lo.y) * u01(w.y); x[2] = lo.z + (hi.z - lo.z) * u01(w.z);
+  w = philox(cl, ch, 0xFFFFFFFEu, 1, seed, 0x5EEDu);
+  const float r0 = sqrtf(-2.0f * logf(u01(w.x) + 1.0f / 16777216.0f)), r1 = sqrtf(-2.0f * logf(u01(w.z) + 1.0f / 16777216.0f));
+  float s0, c0, s1, c1;
+  sincosf(2.0f * kPi * u01(w.y), &s0, &c0); sincosf(2.0f * kPi * u01(w.w), &s1, &c1);
+  float q0 = r0 * c0, q1 = r0 * s0, q2 = r1 * c1, q3 = r1 * s1;
+  const float inv = rsqrtf(fmaxf(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3, 1e-30f));
+  x[3] = q0 * inv; x[4] = q1 * inv; x[5] = q2 * inv; x[6] = q3 * inv;
+  for (int j = 0; j < T; j += 4) {
+    w = philox(cl, ch, 0xFFFFFFFEu, 2 + (uint32_t)(j / 4), seed, 0x5EEDu);
+    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+    for (int k = 0; k < 4 && j + k < T; ++k) x[7 + j + k] = -kPi + 2.0f * kPi * u01(ws[k]);
+  }
+}
+hipError_t random_confs(float* conf, long long n, int T, uint32_t seed, const float lo[3], const float hi[3], hipStream_t s) {
+  const int threads = 256;
+  const long long blocks = (n + threads - 1) / threads;
+  hipLaunchKernelGGL(vd_random_confs_kernel, dim3((unsigned)blocks), dim3(threads), 0, s, conf, n, T, seed,
+                     make_float3(lo[0], lo[1], lo[2]), make_float3(hi[0], hi[1], hi[2]));
+  return hipGetLastError();
+}
 hipError_t reset_counter(int* p, hipStream_t s) {
   hipLaunchKernelGGL(vd_fill_int, dim3(1), dim3(1), 0, s, p, 0);
   return hipGetLastError();
 }
 
-template <int MODE>
+template <int MODE, int W>
 static int occupancy(int G, int lds_bytes) {
   int n = 0;
   hipError_t e = hipErrorInvalidValue;
   switch (G) {
-    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, vd_mc_kernel<64, MODE>, 64, lds_bytes); break;
-    case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, vd_mc_kernel<32, MODE>, 64, lds_bytes); break;
-    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, vd_mc_kernel<16, MODE>, 64, lds_bytes); break;
+    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, vd_mc_kernel<64, MODE, W>, 64, lds_bytes); break;
+    case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, vd_mc_kernel<32, MODE, W>, 64, lds_bytes); break;
+    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, vd_mc_kernel<16, MODE, W>, 64, lds_bytes); break;
     default: break;
   }
   return e == hipSuccess ? n : 0;
 }
 
-int max_resident_mc_blocks(int G, int mode, int lds_bytes) {
-  return mode == 1 ? occupancy<1>(G, lds_bytes) : occupancy<0>(G, lds_bytes);
+int max_resident_mc_blocks(int G, int W, int mode, int lds_bytes) {
+  if (mode == 0) return occupancy<0, 2>(G, lds_bytes);
+  return W == 3 ? occupancy<1, 3>(G, lds_bytes) : W == 4 ? occupancy<1, 4>(G, lds_bytes) : occupancy<1, 2>(G, lds_bytes);
 }
 
 }  // namespace vdl

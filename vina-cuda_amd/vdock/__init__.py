@@ -28,7 +28,7 @@ EXPORTS = (
     "vd_ligand_add_pdbqt", "vd_ligand_info_get", "vd_ligand_export", "vd_ligand_count",
     "vd_eval", "vd_eval_last_ms", "vd_dock_params_default", "vd_dock_heuristics", "vd_dock",
     "vd_dock_batch", "vd_dock_batch_plain", "vd_batch_results", "vd_batch_best", "vd_cluster",
-    "vd_synth_receptor", "vd_synth_ligand",
+    "vd_synth_receptor", "vd_synth_ligand", "vd_device_alloc", "vd_device_free", "vd_copy", "vd_random_confs",
 )
 
 
@@ -56,7 +56,8 @@ class LigandInfo(C.Structure):
 
 class DockParams(C.Structure):
     _fields_ = [("chains", C.c_int), ("search_depth", C.c_int), ("bfgs_steps", C.c_int), ("local_search", C.c_int),
-                ("policy", C.c_int), ("mutation_quirk", C.c_int), ("chains_per_wave", C.c_int), ("seed", C.c_uint32),
+                ("policy", C.c_int), ("mutation_quirk", C.c_int), ("chains_per_wave", C.c_int), ("waves_per_simd", C.c_int),
+                ("seed", C.c_uint32),
                 ("temperature", C.c_float), ("amplitude", C.c_float), ("v_intra", C.c_float), ("v_inter", C.c_float),
                 ("slope", C.c_float)]
 
@@ -106,6 +107,10 @@ def load():
         "vd_batch_results": (C.c_int, [vp, C.c_int, fp, fp, fp]),
         "vd_batch_best": (C.c_int, [vp, fp]),
         "vd_cluster": (C.c_int, [C.c_int, C.c_int, fp, fp, C.c_double, C.c_int, ip]),
+        "vd_device_alloc": (C.c_int, [vp, C.c_size_t, C.POINTER(C.c_void_p)]),
+        "vd_device_free": (C.c_int, [vp, vp]),
+        "vd_copy": (C.c_int, [vp, vp, vp, C.c_size_t, C.c_int]),
+        "vd_random_confs": (C.c_int, [vp, C.c_int, C.c_longlong, C.c_uint32, vp]),
         "vd_synth_receptor": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_uint64, fp, ip]),
         "vd_synth_ligand": (C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_char_p, C.c_int]),
     }
@@ -287,6 +292,23 @@ class Context:
         self._check(self.lib.vd_eval(self.h, lid, n, conf_ptr, e_ptr, g_ptr, v_intra, v_inter, slope,
                                      chains_per_wave, 1))
         return self.lib.vd_eval_last_ms(self.h)
+
+    # -- device memory (inputs resident in HBM)
+    def alloc(self, nbytes):
+        ptr = C.c_void_p()
+        self._check(self.lib.vd_device_alloc(self.h, nbytes, C.byref(ptr)))
+        return ptr.value
+
+    def free(self, ptr):
+        self._check(self.lib.vd_device_free(self.h, ptr))
+
+    def to_host(self, ptr, shape, dtype=np.float32):
+        out = np.empty(shape, dtype=dtype)
+        self._check(self.lib.vd_copy(self.h, out.ctypes.data, ptr, out.nbytes, 1))
+        return out
+
+    def random_confs(self, lid, n, seed, ptr):
+        self._check(self.lib.vd_random_confs(self.h, lid, n, seed, ptr))
 
     def eval_last_ms(self):
         return self.lib.vd_eval_last_ms(self.h)
