@@ -89,6 +89,12 @@ double orc_local_search(void* grid, void* lig, double* conf, int max_steps, int 
  * diagonal curvature coef[0..n), 1 = Rosenbrock); x updated in place, returns f */
 double orc_opt_test(int kind, int n, double* x, const double* coef, int max_steps, int mode, int* n_evals);
 
+/* ---- QuickVina2 visited history (QV2/inc/cuda/visited.cuh:7-226, intended semantics) ----
+ * Feeds m points of a ligand with T torsions into one fresh history in order: x[m][7+T]
+ * conformations, f[m] energies, g[m][6+T] derivatives.  interesting[k] = the gate's answer for
+ * point k; mode 1 adds a point when it is interesting (the gate), mode 2 adds every point. */
+int orc_visited_feed(int T, int m, const double* x, const double* f, const double* g, int mode, int* interesting);
+
 /* ---- Monte Carlo (kernel2.cu:219-426 schedule; mutate.cpp:62-127; monte_carlo.cpp:35-39) ---- */
 typedef struct {
   int    chains, search_depth, bfgs_steps, mode;   /* mode: 1 RILC-BFGS, 0 dense BFGS */
@@ -97,6 +103,7 @@ typedef struct {
   double box_lo[3], box_hi[3];
   double v_intra, v_inter, slope, temperature, amplitude;
   int    threads;
+  int    policy;                                   /* 0 Vina, 1 QuickVina2 visited-history gate */
 } orc_mc_params;
 /* per chain: e[c], conf[c*(7+T)], heavy coords[c*3*H]; returns total evals through *evals */
 int orc_mc_run(void* grid, void* lig, const orc_mc_params* p, double* e, double* conf,

@@ -63,6 +63,7 @@ def _load():
         "orc_mc_run": (C.c_int, [vp, vp, vp, dp, dp, dp, C.POINTER(C.c_longlong)]),
         "orc_initial_conf": (C.c_int, [vp, vp, C.c_int, dp]),
         "orc_cluster": (C.c_int, [C.c_int, C.c_int, dp, dp, C.c_double, C.c_int, ip]),
+        "orc_visited_feed": (C.c_int, [C.c_int, C.c_int, dp, dp, dp, C.c_int, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -303,11 +304,11 @@ class MCParams(C.Structure):
         ("mutation_quirk", C.c_int), ("seed", C.c_uint32), ("stream", C.c_uint32),
         ("box_lo", C.c_double * 3), ("box_hi", C.c_double * 3),
         ("v_intra", C.c_double), ("v_inter", C.c_double), ("slope", C.c_double),
-        ("temperature", C.c_double), ("amplitude", C.c_double), ("threads", C.c_int),
+        ("temperature", C.c_double), ("amplitude", C.c_double), ("threads", C.c_int), ("policy", C.c_int),
     ]
 
 
-def mc_params(chains, depth, bfgs_steps, box_lo, box_hi, seed=42, stream=0, mode=1, quirk=0, threads=0):
+def mc_params(chains, depth, bfgs_steps, box_lo, box_hi, seed=42, stream=0, mode=1, quirk=0, threads=0, policy=0):
     p = MCParams()
     p.chains, p.search_depth, p.bfgs_steps, p.mode, p.mutation_quirk = chains, depth, bfgs_steps, mode, quirk
     p.seed, p.stream = seed, stream
@@ -316,6 +317,7 @@ def mc_params(chains, depth, bfgs_steps, box_lo, box_hi, seed=42, stream=0, mode
         p.box_hi[i] = box_hi[i]
     p.v_intra, p.v_inter, p.slope, p.temperature, p.amplitude = 10.0, 10.0, 1e6, 1.2, 2.0
     p.threads = threads
+    p.policy = policy
     return p
 
 
@@ -343,3 +345,15 @@ def cluster(e, heavy, min_rmsd=1.0, max_size=20):
     kept = np.zeros(max_size, dtype=np.int32)
     k = lib().orc_cluster(len(e), h.shape[1], _d(e), _d(h), min_rmsd, max_size, _i(kept))
     return kept[:k].copy()
+
+
+def visited_feed(T, x, f, g, mode=1):
+    """Gate answers for points fed in order into one QuickVina2 history (mode 1: add the
+    interesting ones, mode 2: add all)."""
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 7 + T)
+    f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+    g = np.ascontiguousarray(g, dtype=np.float64).reshape(-1, 6 + T)
+    out = np.zeros(len(f), dtype=np.int32)
+    if lib().orc_visited_feed(T, len(f), _d(x), _d(f), _d(g), mode, _i(out)) != 0:
+        raise RuntimeError(_err())
+    return out.astype(bool)

@@ -767,6 +767,73 @@ fl dense_bfgs(F& f, Inc& inc, P& x, std::vector<fl>& g, int max_steps) {
 struct ConfInc { void operator()(Conf& x, const std::vector<fl>& d, fl a) const { increment(x, d, a); } };
 struct VecInc { void operator()(std::vector<fl>& x, const std::vector<fl>& d, fl a) const { for (size_t i = 0; i < x.size(); ++i) x[i] += a * d[i]; } };
 
+fl local_search(EvalCtx& ctx, Conf& x, int max_steps, int mode);
+
+// QuickVina2 visited history and its gate (QV2/inc/cuda/visited.cuh:7-226, used at
+// QV2/inc/cuda/quasi_newton.cuh:1545-1550), with the intended semantics: check() answers
+// "interesting" unless some dimension shows the new point on the same monotone slope as the
+// neighbour (the reference returns false at its end as well, so its gate never lets a local
+// search through once the history is full), torsion dimensions use the torsion derivative (the
+// reference reads the position derivative there), and entries are stored by value.
+struct VEntry { std::vector<fl> x; fl f; uint32_t zero = 0, pos = 0; };
+struct Visited {
+  int n = 0, count = 0, p = 0; bool full = false;
+  std::vector<VEntry> list;
+  static std::vector<fl> flat(const Conf& c) {
+    std::vector<fl> v{c.pos[0], c.pos[1], c.pos[2], c.q.a, c.q.b, c.q.c, c.q.d};
+    v.insert(v.end(), c.tors.begin(), c.tors.end());
+    return v;
+  }
+  // coordinate compared for change dimension i: position, the first three quaternion
+  // components (as the reference compares them), torsions
+  static fl coord(const std::vector<fl>& x, int i) { return i < 3 ? x[i] : i < 6 ? x[i] : x[i + 1]; }
+  void add(const Conf& c, fl f, const std::vector<fl>& g) {
+    VEntry e; e.x = flat(c); e.f = f;
+    for (size_t i = 0; i < g.size(); ++i) { if (g[i] == 0) e.zero |= 1u << i; else if (g[i] > 0) e.pos |= 1u << i; }
+    if (count == 0) { n = (int)g.size(); list.assign(10 * n, VEntry()); }
+    if (!full) { list[count++] = e; if (count >= 10 * n) { full = true; p = 0; } }
+    else { list[p] = e; p = (p + 1) % (10 * n); }
+  }
+  bool check(const std::vector<fl>& x, fl f, const std::vector<fl>& g, const VEntry& nb) const {
+    const bool y_bigger = (f - nb.f) > 0;
+    for (int i = 0; i < (int)g.size(); ++i) {
+      if ((nb.zero >> i & 1u) || g[i] == 0) continue;
+      const bool now_pos = g[i] > 0;
+      if (now_pos != (bool)(nb.pos >> i & 1u)) continue;
+      const bool x_bigger = (coord(x, i) - coord(nb.x, i)) > 0;
+      if (now_pos ? (x_bigger != y_bigger) : (x_bigger == y_bigger)) continue;
+      return false;
+    }
+    return true;
+  }
+  bool interesting(const Conf& c, fl f, const std::vector<fl>& g) const {
+    if (!full) return true;
+    const std::vector<fl> x = flat(c);
+    const int len = (int)list.size();
+    std::vector<fl> dist(len);
+    for (int k = 0; k < len; ++k) { fl d = 0; for (size_t j = 0; j < x.size(); ++j) d += sqr(list[k].x[j] - x[j]); dist[k] = d; }
+    std::vector<char> picked(len, 0);
+    for (int it = 0; it < n; ++it) {
+      int best = -1;
+      for (int k = 0; k < len; ++k) if (!picked[k] && (best < 0 || dist[k] < dist[best])) best = k;
+      picked[best] = 1;
+      if (check(x, f, g, list[best])) return true;
+    }
+    return false;
+  }
+};
+
+fl local_search(EvalCtx& ctx, Conf& x, int max_steps, int mode, Visited* visited) {
+  if (visited) {
+    std::vector<fl> g;
+    const fl f = ctx(x, g);
+    if (!visited->interesting(x, f, g)) { x.e = f; return f; }
+    visited->add(x, f, g);
+    --ctx.evals;           // the search below starts by evaluating the same point: count it once
+  }
+  return local_search(ctx, x, max_steps, mode);
+}
+
 fl local_search(EvalCtx& ctx, Conf& x, int max_steps, int mode) {
   std::vector<fl> g;
   ConfInc inc;
@@ -833,6 +900,8 @@ void mc_chain(EvalCtx& ctx, const orc_mc_params& p, int chain, Conf& best, std::
   fl best_e = std::numeric_limits<fl>::infinity();
   const int T = L.T();
   const int n_ent = p.mutation_quirk ? (1 + T) : (2 + T);
+  Visited vis;
+  Visited* vp = p.policy == 1 ? &vis : nullptr;
   for (int step = 0; step < p.search_depth; ++step) {
     Conf cand = tmp;
     uint32_t c0[4] = {(uint32_t)chain, (uint32_t)step, 0, 0}; philox(c0, key, w);
@@ -850,12 +919,12 @@ void mc_chain(EvalCtx& ctx, const orc_mc_params& p, int chain, Conf& best, std::
     } else {
       cand.tors[which - 2] = tor;
     }
-    local_search(ctx, cand, p.bfgs_steps, p.mode);
+    local_search(ctx, cand, p.bfgs_steps, p.mode, vp);
     bool accept = (step == 0) || (cand.e < tmp.e) || (u_metro < std::exp((tmp.e - cand.e) / p.temperature));  // monte_carlo.cpp:35-39
     if (accept) {
       tmp = cand;
       if (tmp.e < best_e) {
-        local_search(ctx, tmp, p.bfgs_steps, p.mode);
+        local_search(ctx, tmp, p.bfgs_steps, p.mode, vp);
         if (tmp.e < best_e) {
           best = tmp; best_e = tmp.e;
           set_conf(L, best, coords, F);
@@ -1011,6 +1080,25 @@ double orc_local_search(void* gp, void* lp, double* conf, int max_steps, int mod
   conf_to(c, conf);
   if (n_evals) *n_evals = (int)ctx.evals;
   return e;
+}
+
+int orc_visited_feed(int T, int m, const double* x, const double* f, const double* g, int mode, int* interesting) {
+  try {
+    if (T < 0 || m < 0 || (mode != 1 && mode != 2)) throw std::runtime_error("bad arguments");
+    Visited v;
+    const int n = 6 + T;
+    for (int k = 0; k < m; ++k) {
+      Conf c;
+      const double* xk = x + (size_t)k * (7 + T);
+      c.pos = V3(xk[0], xk[1], xk[2]); c.q = Q{xk[3], xk[4], xk[5], xk[6]};
+      c.tors.assign(xk + 7, xk + 7 + T);
+      std::vector<fl> gk(g + (size_t)k * n, g + (size_t)(k + 1) * n);
+      const bool it = v.interesting(c, f[k], gk);
+      interesting[k] = it ? 1 : 0;
+      if (mode == 2 || it) v.add(c, f[k], gk);
+    }
+    return 0;
+  } catch (std::exception& e) { return fail(e.what()); }
 }
 
 double orc_opt_test(int kind, int n, double* x, const double* coef, int max_steps, int mode, int* n_evals) {

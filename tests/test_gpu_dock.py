@@ -69,6 +69,51 @@ def test_docking_statistical_parity(ctx, scene, heavy, tors, lseed):
     assert succ_g >= succ_o - 0.25
 
 
+def _dock_seeds(ctx, scene, lid, lig, chains, depth, bfgs, policy, seeds=16):
+    import vdock
+    b, e, _ = scene["box"].as_tuple()
+    out = dict(gb=[], ob=[], gp=[], op=[], gev=0, oev=0)
+    for seed in range(seeds):
+        p = vdock.default_params(chains=chains, search_depth=depth, bfgs_steps=bfgs, seed=seed, policy=policy)
+        eg, cg, hg, st = ctx.dock(lid, p)
+        prm = orc.mc_params(chains, depth, bfgs, b, e, seed=seed, stream=lid, threads=8, policy=policy)
+        eo, co, ho, ev = orc.mc_run(scene["grid"], lig, prm)
+        out["gb"].append(float(eg.min())); out["ob"].append(float(eo.min()))
+        out["gp"].append(hg[int(np.argmin(eg))].astype(np.float64)); out["op"].append(ho[int(np.argmin(eo))])
+        out["gev"] += st.evals; out["oev"] += ev
+    return out
+
+
+@pytest.mark.parametrize("heavy,tors,lseed", [(10, 0, 51), (12, 1, 52)])
+def test_quickvina2_policy_parity(ctx, scene, heavy, tors, lseed):
+    """QuickVina2 policy (visited-history gate on both local searches) against the oracle's, with
+    a search long enough for the 10 (6+T)-entry histories to fill so that the gate skips local
+    searches.  Same 16-seed criteria as the Vina policy, plus: the gate saves evaluations on the
+    device in the same proportion as in the oracle (within 0.05 of the ratio to the Vina policy)."""
+    import vdock
+    ctx.grid_upload(scene["box"], scene["maps"])
+    text = vdock.synth_ligand(heavy, tors, lseed)
+    lid = ctx.add_ligand(text)
+    lig = orc.Ligand(text)
+    chains, depth, bfgs = 256, 128, 12
+    qv = _dock_seeds(ctx, scene, lid, lig, chains, depth, bfgs, policy=1)
+    vn = _dock_seeds(ctx, scene, lid, lig, chains, depth, bfgs, policy=0, seeds=4)   # evaluations per run without the gate
+    r_g = (qv["gev"] / 16) / (vn["gev"] / 4)
+    r_o = (qv["oev"] / 16) / (vn["oev"] / 4)
+    gb, ob = np.array(qv["gb"]), np.array(qv["ob"])
+    top_g, top_o = qv["gp"][int(np.argmin(gb))], qv["op"][int(np.argmin(ob))]
+    succ_g = np.mean([rmsd(x, top_o) <= 2.0 for x in qv["gp"]])
+    succ_o = np.mean([rmsd(x, top_o) <= 2.0 for x in qv["op"]])
+    say("qv2 best gpu", gb.min(), "oracle", ob.min(), "mean", gb.mean(), ob.mean(), "top-pose rmsd",
+        rmsd(top_g, top_o), "succ", succ_g, succ_o, "eval ratio to vina: gpu", r_g, "oracle", r_o)
+    assert abs(gb.min() - ob.min()) <= 0.1
+    assert abs(gb.mean() - ob.mean()) <= 0.1
+    assert rmsd(top_g, top_o) <= 2.0
+    assert succ_g >= succ_o - 0.25
+    assert r_o < 0.97                      # the gate is active in this setting
+    assert abs(r_g - r_o) <= 0.05
+
+
 def test_initial_conformations_match_oracle(ctx, scene):
     """With zero MC steps the device keeps each chain's initial conformation: it must equal the
     oracle's draw from the shared Philox stream (FP32 rounding aside)."""

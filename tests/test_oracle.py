@@ -397,3 +397,85 @@ def test_cluster_rules():
     assert list(kept) == [1, 2]
     kept = orc.cluster(e, heavy, 1.0, 20)
     assert list(kept) == [1, 2, 4]
+
+
+# ---- QuickVina2 visited history (QV2/inc/cuda/visited.cuh:7-226, intended semantics) ----
+# Points of a convex bowl f = sum_i (c_i - m_i)^2 over the change dimensions' coordinates
+# (position x,y,z, orientation components a,b,c, torsions) with g_i = 2 (c_i - m_i).  On one
+# side of the minimum in every dimension, each pair of points lies on one monotone slope, so
+# every point explains every other; across the minimum no dimension agrees in sign.
+VT = 2
+VN = 6 + VT
+VCAP = 10 * VN
+
+
+def _bowl(rng, k, side, lo=0.2, hi=1.5, around=None, spread=0.05):
+    m = np.linspace(-0.3, 0.3, VN)
+    coords = (around + rng.uniform(-spread, spread, (k, VN))) if around is not None else \
+        m + side * rng.uniform(lo, hi, (k, VN))
+    x = np.zeros((k, 7 + VT))
+    x[:, 0:6] = coords[:, 0:6]
+    x[:, 6] = rng.uniform(-1, 1, k)                 # orientation d: in the distance only
+    x[:, 7:] = coords[:, 6:]
+    f = ((coords - m) ** 2).sum(axis=1)
+    g = 2.0 * (coords - m)
+    return x, f, g
+
+
+def _cat(*parts):
+    return tuple(np.concatenate([p[i] for p in parts]) for i in range(3))
+
+
+def test_visited_not_full_is_always_interesting():
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=(VCAP, 7 + VT)); f = rng.normal(size=VCAP); g = rng.normal(size=(VCAP, VN))
+    assert orc.visited_feed(VT, x, f, g, mode=2).all()
+    # the gate (mode 1) adds every interesting point, so the history fills after exactly VCAP
+    assert orc.visited_feed(VT, x, f, g, mode=1).all()
+
+
+def test_visited_convex_side_rule():
+    rng = np.random.default_rng(6)
+    right = _bowl(rng, VCAP, +1)
+    more_right = _bowl(rng, 12, +1)
+    left = _bowl(rng, 1, -1)
+    a = orc.visited_feed(VT, *_cat(right, more_right), mode=2)
+    assert a[:VCAP].all() and not a[VCAP:].any()
+    b = orc.visited_feed(VT, *_cat(right, left), mode=2)
+    assert b[-1]
+
+
+def test_visited_ring_replaces_oldest():
+    rng = np.random.default_rng(7)
+    right, left = _bowl(rng, VCAP, +1), _bowl(rng, VCAP, -1)
+    # after VCAP more adds every right-side entry has been overwritten by a left-side one
+    assert not orc.visited_feed(VT, *_cat(right, left, _bowl(rng, 1, -1)), mode=2)[-1]
+    assert orc.visited_feed(VT, *_cat(right, left, _bowl(rng, 1, +1)), mode=2)[-1]
+    # one add short of that, the last right-side entry can still be among the neighbours
+    part = tuple(a[: VCAP - 1] for a in left)
+    q = _bowl(rng, 1, +1)
+    assert orc.visited_feed(VT, *_cat(right, part, q), mode=2)[-1]
+
+
+def test_visited_uses_exactly_n_nearest():
+    rng = np.random.default_rng(8)
+    centre = np.linspace(-0.3, 0.3, VN) + 1.0
+    q = _bowl(rng, 1, +1, around=centre, spread=0.01)
+    for near_count, expect in ((VN, False), (VN - 1, True)):
+        near = _bowl(rng, near_count, +1, around=centre)        # explain the query
+        far = _bowl(rng, VCAP - near_count, -1, lo=1.0, hi=2.0)  # never explain it, all farther
+        hist = _cat(far, near)
+        assert orc.visited_feed(VT, *_cat(hist, q), mode=2)[-1] == expect
+
+
+def test_visited_order_of_history_does_not_matter():
+    rng = np.random.default_rng(9)
+    answers = []
+    for trial in range(40):
+        x = rng.normal(size=(VCAP + 1, 7 + VT)); f = rng.normal(size=VCAP + 1); g = rng.normal(size=(VCAP + 1, VN))
+        a = orc.visited_feed(VT, x, f, g, mode=2)[-1]
+        perm = np.concatenate([rng.permutation(VCAP), [VCAP]])
+        b = orc.visited_feed(VT, x[perm], f[perm], g[perm], mode=2)[-1]
+        assert a == b
+        answers.append(a)
+    assert 0 < sum(answers) < len(answers)          # both answers occur

@@ -89,6 +89,7 @@ struct vd_ctx {
   int n_cu = 0;
   std::map<int, std::pair<int, int>> tuned;  // size bucket -> (lanes per chain, waves per SIMD)
   DevBuf<float> tune_e, tune_conf, tune_heavy;
+  DevBuf<float> hist;                        // QuickVina2 visited histories
 };
 
 namespace {
@@ -165,6 +166,8 @@ void check_params(const vd_dock_params* p) {
   if (p->search_depth < 0 || p->bfgs_steps < 0) throw ArgError(VD_ERR_ARG, "negative search_depth or bfgs_steps");
   if (p->local_search != VD_LS_BFGS && p->local_search != VD_LS_RILC_BFGS) throw ArgError(VD_ERR_ARG, "bad local_search");
   if (p->policy != VD_MC_VINA && p->policy != VD_MC_QVINA2) throw ArgError(VD_ERR_ARG, "bad policy");
+  if (p->policy == VD_MC_QVINA2 && p->local_search != VD_LS_RILC_BFGS)
+    throw ArgError(VD_ERR_ARG, "the QuickVina2 policy runs with RILC-BFGS only");
   if (p->chains_per_wave != 0 && p->chains_per_wave != 1 && p->chains_per_wave != 2 && p->chains_per_wave != 4)
     throw ArgError(VD_ERR_ARG, "chains_per_wave must be 0, 1, 2 or 4");
   if (p->waves_per_simd != 0 && (p->waves_per_simd < 2 || p->waves_per_simd > 4))
@@ -235,7 +238,7 @@ vdk::EvalK make_evalk(vd_ctx* c, float v_intra, float v_inter, float slope) {
 struct Packed {
   std::vector<float> blob;
   std::vector<vd_lig_header> headers;
-  int Amax = 0, Nmax = 0, Pmax = 0;
+  int Amax = 0, Nmax = 0, Pmax = 0, Tmax = 0;
 };
 Packed pack(vd_ctx* c, const std::vector<int>& ids, const std::vector<int>* depth = nullptr,
             const std::vector<int>* bfgs = nullptr) {
@@ -248,6 +251,7 @@ Packed pack(vd_ctx* c, const std::vector<int>& ids, const std::vector<int>* dept
     pk.Amax = std::max(pk.Amax, (int)L.atoms.size());
     pk.Nmax = std::max(pk.Nmax, (int)L.nodes.size());
     pk.Pmax = std::max(pk.Pmax, (int)L.pairs.size());
+    pk.Tmax = std::max(pk.Tmax, L.n_torsions());
   }
   return pk;
 }
@@ -289,9 +293,17 @@ void run_mc(vd_ctx* c, int G, int W, const Packed& pk, const std::vector<uint32_
   if (persistent) {
     a.queue = c->queue.p;
     HIP_CHECK(vdl::reset_counter(c->queue.p, c->stream));
-    const int per_cu = std::max(1, vdl::max_resident_mc_blocks(G, W, P.local_search, lds));
+    const int mode = P.local_search == 0 ? 0 : (P.policy == VD_MC_QVINA2 ? 2 : 1);
+    const int per_cu = std::max(1, vdl::max_resident_mc_blocks(G, W, mode, lds));
     blocks = std::min<long long>((long long)per_cu * c->n_cu, (long long)items.size());
     blocks = std::max(blocks, 1);
+  }
+  if (P.policy == VD_MC_QVINA2) {
+    // one visited-history region per chain slot of every launched block (blocks keep their
+    // slot for all the items they take from the queue)
+    a.hist_stride = vdk::History::floats_per_chain(pk.Tmax);
+    c->hist.ensure((size_t)blocks * (64 / G) * a.hist_stride);
+    a.hist = c->hist.p;
   }
   if (timed) HIP_CHECK(hipEventRecord(c->ev0, c->stream));
   HIP_CHECK(vdl::launch_mc(G, W, a, blocks, c->stream));
@@ -419,7 +431,7 @@ void vd_destroy(vd_ctx* c) {
   c->blob.release(); c->headers.release(); c->streams.release(); c->items.release(); c->offs.release();
   c->queue.release(); c->evals.release(); c->io_a.release(); c->io_b.release(); c->io_c.release();
   c->batch.e.release(); c->batch.conf.release(); c->batch.heavy.release();
-  c->tune_e.release(); c->tune_conf.release(); c->tune_heavy.release();
+  c->tune_e.release(); c->tune_conf.release(); c->tune_heavy.release(); c->hist.release();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -690,8 +702,8 @@ int vd_dock_heuristics(int n_atoms, int n_torsions, int rilc, int* depth, int* b
 int vd_dock(vd_ctx* c, int id, const vd_dock_params* p, float* e, float* conf, float* heavy, vd_stats* st) {
   if (!c) return VD_ERR_ARG;
   return guard(c, [&]() {
-    need_device(c);
     check_params(p);
+    need_device(c);
     if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no maps");
     if (st) std::memset(st, 0, sizeof(*st));
     if (p->chains_per_wave == 0 || p->waves_per_simd == 0) autotune(c, {id}, p);
@@ -713,8 +725,8 @@ int vd_dock(vd_ctx* c, int id, const vd_dock_params* p, float* e, float* conf, f
 static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, vd_stats* st, bool persistent) {
   if (!c || n < 0 || (n > 0 && !ids)) return VD_ERR_ARG;
   return guard(c, [&]() {
-    need_device(c);
     check_params(p);
+    need_device(c);
     if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no maps");
     if (st) std::memset(st, 0, sizeof(*st));
     if (p->chains_per_wave == 0 || p->waves_per_simd == 0) autotune(c, std::vector<int>(ids, ids + n), p);

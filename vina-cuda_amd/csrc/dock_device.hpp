@@ -406,20 +406,140 @@ __device__ __forceinline__ Conf stash_get(const float* st, int slot) {
 }
 
 // ----------------------------------------------------------------------------------------
+// QuickVina2 visited history of one chain (QV2/inc/cuda/visited.cuh:7-226) and the gate that
+// skips a local search whose start point is explained by its neighbours
+// (QV2/inc/cuda/quasi_newton.cuh:1545-1550).  Semantics as intended by the reference: a
+// neighbour explains the point when some change dimension shows both on one monotone slope
+// (same derivative sign, coordinate and energy differences ordered as that sign says); the point
+// is interesting unless each of its n nearest neighbours (n = 6+T, squared distance over the
+// 7+T conformation numbers) explains it; torsion dimensions use the torsion derivative.
+// Storage is a ring of cap = 10 n entries in global memory, one region per resident chain:
+//   X[c][cap]  conformation number c: c = 0 orientation z, 1..T torsions, T+1..T+6 position
+//              x,y,z and orientation w,x,y (the lane-layout order: lane 0, lanes 1..T,
+//              lanes G-6..G-1)
+//   F[cap]     energy;   Z[cap], Pm[cap]  lane masks of zero / positive derivatives.
+// Entries are compared on the coordinate of their dimension lane: lanes G-6..G-4 position,
+// G-3..G-1 orientation w,x,y (as the reference compares the first components of its
+// flattened vector), lane k torsion k-1.
+constexpr int kHistMaxRounds = 10;                // cap = 10 (6+T) <= 10 G entries -> 10 per lane
+struct History {
+  float* X; float* F; unsigned long long* Z; unsigned long long* Pm;
+  int cap, count, p;
+  __host__ __device__ static long long floats_per_chain(int T) { const long long cap = 10LL * (6 + T); return cap * (12 + T); }
+  __device__ void init(float* base, int T) {
+    cap = 10 * (6 + T); count = 0; p = 0;
+    X = base; F = base + (long long)cap * (7 + T);
+    Z = reinterpret_cast<unsigned long long*>(F + cap); Pm = Z + cap;
+  }
+};
+template <int G> __device__ __forceinline__ int hist_comp(int l, int T) {
+  return l <= T ? l : (l >= G - 6 ? T + 1 + (l - (G - 6)) : -1);
+}
+template <int G> __device__ __forceinline__ int hist_lane(int c, int T) { return c <= T ? c : (G - 6) + (c - T - 1); }
+template <int G> __device__ __forceinline__ float conf_coord(const Conf& x, int l, int T) {
+  const int c6 = l - (G - 6);
+  return l == 0 ? x.q.z : l <= T ? x.tor
+       : c6 == 0 ? x.px : c6 == 1 ? x.py : c6 == 2 ? x.pz : c6 == 3 ? x.q.w : c6 == 4 ? x.q.x : c6 == 5 ? x.q.y : 0.0f;
+}
+template <int G> __device__ __forceinline__ unsigned long long group_ballot(bool v) {
+  const unsigned long long b = __ballot(v);
+  if constexpr (G == 64) return b;
+  else return (b >> ((threadIdx.x & 63) / G * G)) & ((1ull << G) - 1ull);
+}
+template <int G> __device__ __forceinline__ bool is_dim_lane(int l, int T) { return (l >= 1 && l <= T) || l >= G - 6; }
+
+// Adds (x, f, g) when on; every lane of the wave calls it.
+template <int G>
+__device__ void hist_add(History& h, const Conf& x, float f, float g, int l, int T, bool on) {
+  const bool dim = is_dim_lane<G>(l, T);
+  const unsigned long long z = group_ballot<G>(dim && g == 0.0f), ps = group_ballot<G>(dim && g > 0.0f);
+  const int slot = h.count < h.cap ? h.count : h.p;
+  if (on) {
+    const int c = hist_comp<G>(l, T);
+    if (c >= 0) h.X[(long long)c * h.cap + slot] = conf_coord<G>(x, l, T);
+    if (l == 0) { h.F[slot] = f; h.Z[slot] = z; h.Pm[slot] = ps; }
+    if (h.count < h.cap) ++h.count; else h.p = (h.p + 1 == h.cap) ? 0 : h.p + 1;
+  }
+  __syncthreads();                                 // entries visible to the whole wave
+}
+
+// true when (x, f, g) is worth a local search; every lane of the wave calls it.
+template <int G>
+__device__ bool hist_interesting(const History& h, const Conf& x, float f, float g, int l, int T, bool on) {
+  const bool need = on && h.count == h.cap;
+  if (!__any(need)) return true;
+  const float coord = conf_coord<G>(x, l, T);
+  float d[kHistMaxRounds];
+#pragma unroll
+  for (int r = 0; r < kHistMaxRounds; ++r) d[r] = 0.0f;
+  for (int c = 0; c < 7 + T; ++c) {
+    const float xc = gget<G>(coord, hist_lane<G>(c, T));
+    const float* row = h.X + (long long)c * h.cap;
+#pragma unroll
+    for (int r = 0; r < kHistMaxRounds; ++r) {
+      const int e = l + G * r;
+      if (need && e < h.cap) { const float t = row[e] - xc; d[r] += t * t; }
+    }
+  }
+  const bool dim = is_dim_lane<G>(l, T);
+  const int cl = hist_comp<G>(l, T);
+  uint32_t picked = 0;
+  bool found = false;
+  for (int it = 0; it < 6 + T && __any(need && !found); ++it) {
+    // next nearest entry: smallest distance, lowest index among equals
+    float bd = INFINITY; int be = h.cap;
+#pragma unroll
+    for (int r = 0; r < kHistMaxRounds; ++r) {
+      const int e = l + G * r;
+      if (e < h.cap && !((picked >> r) & 1u) && d[r] < bd) { bd = d[r]; be = e; }
+    }
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+      const float od = __shfl_xor(bd, off, G); const int oe = __shfl_xor(be, off, G);
+      if (od < bd || (od == bd && oe < be)) { bd = od; be = oe; }
+    }
+    const bool pick = need && !found && be < h.cap;
+    if (pick && be % G == l) picked |= 1u << (be / G);
+    bool explains = false;
+    if (pick && dim && !(g == 0.0f)) {
+      const unsigned long long nz = h.Z[be], np = h.Pm[be];
+      if (!((nz >> l) & 1ull)) {
+        const bool now_pos = g > 0.0f;
+        if (now_pos == (bool)((np >> l) & 1ull)) {
+          const bool x_bigger = (coord - h.X[(long long)cl * h.cap + be]) > 0.0f;
+          const bool y_bigger = (f - h.F[be]) > 0.0f;
+          explains = now_pos ? (x_bigger == y_bigger) : (x_bigger != y_bigger);
+        }
+      }
+    }
+    const bool any_explains = group_ballot<G>(explains) != 0ull;
+    if (pick && !any_explains) found = true;
+  }
+  return !need || found;
+}
+
+// ----------------------------------------------------------------------------------------
 // RILC-BFGS local search with the Lewis-Overton line search
 // (V11/inc/cuda/quasi_newton.cuh:1710-1806, 2172-2389), run by every group of the wave in
 // lockstep.  Groups with enabled == false evaluate harmlessly and keep x unchanged.
 // Dot products are exact group sums over all 6+T components.
-template <int G>
+// QV: QuickVina2 gate on the start point with the chain's history (see History).
+template <int G, bool QV>
 __device__ float rilc_bfgs(const LigView& L, const Scratch& S, const EvalK& K, Conf& x, bool enabled,
-                           float x_e, int max_steps, int l, int& evals) {
+                           float x_e, int max_steps, int l, int& evals, History* hist) {
   float g;
   float fx = eval_deriv<G>(L, S, K, x, l, g);
   evals += enabled ? 1 : 0;
   const float fx_orig = fx;
   stash_put(S.stash, kSlotOrig, x);
+  bool search = enabled;
+  if constexpr (QV) {
+    const bool interesting = hist_interesting<G>(*hist, x, fx, g, l, L.T, enabled);
+    hist_add<G>(*hist, x, fx, g, l, L.T, enabled && interesting);
+    search = enabled && interesting;
+  }
   float d = -g;
-  bool active = enabled && (sqrtf(gsum<G>(g * g)) >= 1e-5f);
+  bool active = search && (sqrtf(gsum<G>(g * g)) >= 1e-5f);
   float step = 1.0f / sqrtf(gsum<G>(d * d));
   int k = 1;
   while (__any(active)) {

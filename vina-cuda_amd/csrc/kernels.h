@@ -25,6 +25,8 @@ struct McArgs {
   const longlong4* out_off;         // per ligand slot: (e, conf, heavy) element offsets
   unsigned long long* eval_count;
   int Amax, Nmax, Pmax;
+  float* hist;                      // QuickVina2 visited histories (nullptr: Vina policy),
+  long long hist_stride;            // floats per chain region; region = block * chains/wave + group
 };
 
 struct EvalArgs {

@@ -114,9 +114,9 @@ __device__ __forceinline__ void make_views(const float4* __restrict__ blob, cons
 
 template <int G, int MODE>
 __device__ float local_search(const LigView& L, const Scratch& S, float* H, const EvalK& K, Conf& x,
-                              bool enabled, float x_e, int max_steps, int l, int& evals) {
+                              bool enabled, float x_e, int max_steps, int l, int& evals, History* hist) {
   if (!__any(enabled)) return x_e;
-  if constexpr (MODE == 1) return rilc_bfgs<G>(L, S, K, x, enabled, x_e, max_steps, l, evals);
+  if constexpr (MODE >= 1) return rilc_bfgs<G, MODE == 2>(L, S, K, x, enabled, x_e, max_steps, l, evals, hist);
   else return dense_bfgs<G>(L, S, H, H + G * G, K, x, enabled, x_e, max_steps, l, evals);
 }
 
@@ -154,10 +154,12 @@ __device__ Conf initial_conf(const vd_mc_consts& P, uint32_t stream, uint32_t ch
 // mutate -> local search -> Metropolis(T) -> on accept, if better than the best so far: second
 // local search, keep the best.  Mutation as V11/lib/mutate.cpp:62-127 (see initial_conf for the
 // random stream); the orientation step uses the gyration radius of the current conformation.
+// hist != nullptr: QuickVina2 policy (QV2/inc/cuda/kernel2.cu, same schedule with the visited
+// history of the chain gating both local searches).
 template <int G, int MODE>
 __device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const EvalK& K, const vd_mc_consts& P,
                          int depth, int bfgs_steps, uint32_t stream, uint32_t chain, bool valid, int l, float* out_e,
-                         float* out_conf, float* out_heavy, int& evals) {
+                         float* out_conf, float* out_heavy, int& evals, History* hist) {
   const uint32_t k0 = P.seed, k1 = stream;
   Conf tmp = initial_conf<G>(P, stream, chain, l, L.T);
   float tmp_e = 0.0f;
@@ -182,11 +184,11 @@ __device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const Eva
     if (which == 0) { cand.px += P.amplitude * sx; cand.py += P.amplitude * sy; cand.pz += P.amplitude * sz; }
     else if (which == 1) { if (rg > kEpsFl) { const float f = P.amplitude / rg; cand.q = qincrement(cand.q, f * sx, f * sy, f * sz); } }
     else if (l == which - 1) cand.tor = tor;
-    const float ce = local_search<G, MODE>(L, S, H, K, cand, valid, 0.0f, bfgs_steps, l, evals);
+    const float ce = local_search<G, MODE>(L, S, H, K, cand, valid, 0.0f, bfgs_steps, l, evals, hist);
     const bool accept = valid && (step == 0 || ce < tmp_e || u_metro < expf((tmp_e - ce) / P.temperature));
     if (accept) { tmp = cand; tmp_e = ce; } else tmp = stash_get(S.stash, kSlotTmp);
     const bool second = accept && tmp_e < best_e;
-    const float e2 = local_search<G, MODE>(L, S, H, K, tmp, second, tmp_e, bfgs_steps, l, evals);
+    const float e2 = local_search<G, MODE>(L, S, H, K, tmp, second, tmp_e, bfgs_steps, l, evals, hist);
     if (second) { tmp_e = e2; if (tmp_e < best_e) { stash_put(S.stash, kSlotBest, tmp); best_e = tmp_e; } }
     if (__any(accept)) {
       set_coords<G>(L, S, tmp, l);
@@ -247,8 +249,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
     const uint32_t chain = (uint32_t)(wi.y + group);
     const bool valid = (int)chain < a.P.chains;
     int evals = 0;
+    History hist;
+    if constexpr (MODE == 2) hist.init(a.hist + ((long long)blockIdx.x * CPW + group) * a.hist_stride, h.n_torsions);
     mc_chain<G, MODE>(L, S, H, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l, a.out_e + a.out_off[wi.x].x,
-                a.out_conf + a.out_off[wi.x].y, a.out_heavy + a.out_off[wi.x].z, evals);
+                a.out_conf + a.out_off[wi.x].y, a.out_heavy + a.out_off[wi.x].z, evals, &hist);
     if (l == 0) my_evals += evals;
   }
   if (a.eval_count) {
@@ -329,6 +333,14 @@ static hipError_t launch_mc_mode(int G, const McArgs& a, int blocks, int lds, hi
 hipError_t launch_mc(int G, int W, const McArgs& a, int blocks, hipStream_t s) {
   const int lds = mc_lds_bytes(G, a.Amax, a.Nmax, a.Pmax, a.P.local_search == 0);
   if (a.P.local_search == 0) return launch_mc_mode<0, 2>(G, a, blocks, lds, s);
+  if (a.P.policy == 1) {
+    if (!a.hist) return hipErrorInvalidValue;
+    switch (W) {
+      case 3: return launch_mc_mode<2, 3>(G, a, blocks, lds, s);
+      case 4: return launch_mc_mode<2, 4>(G, a, blocks, lds, s);
+      default: return launch_mc_mode<2, 2>(G, a, blocks, lds, s);
+    }
+  }
   switch (W) {
     case 3: return launch_mc_mode<1, 3>(G, a, blocks, lds, s);
     case 4: return launch_mc_mode<1, 4>(G, a, blocks, lds, s);
@@ -385,8 +397,10 @@ static int occupancy(int G, int lds_bytes) {
   return e == hipSuccess ? n : 0;
 }
 
+// mode: 0 dense BFGS, 1 RILC-BFGS, 2 RILC-BFGS with the QuickVina2 history
 int max_resident_mc_blocks(int G, int W, int mode, int lds_bytes) {
   if (mode == 0) return occupancy<0, 2>(G, lds_bytes);
+  if (mode == 2) return W == 3 ? occupancy<2, 3>(G, lds_bytes) : W == 4 ? occupancy<2, 4>(G, lds_bytes) : occupancy<2, 2>(G, lds_bytes);
   return W == 3 ? occupancy<1, 3>(G, lds_bytes) : W == 4 ? occupancy<1, 4>(G, lds_bytes) : occupancy<1, 2>(G, lds_bytes);
 }
 
