@@ -52,16 +52,18 @@ def screening_ligands(vdock, n=1024):
     return out
 
 
-def mc_bytes(ctx, ids, st):
-    """Algorithmic bytes of the MC kernel: per evaluation 32 B per typed atom (8 map corners),
-    16 B per intramolecular pair (one interpolation quad), and the conformation read and
-    gradient written (4 (7+T) + 4 (6+T)).  Evaluations are split over ligands by their share
-    of the launch's evaluation count, weighted by nothing (the device counts the total)."""
-    per_eval = []
-    for i in ids:
-        inf = ctx.ligand_info(i)
-        per_eval.append(32 * inf.n_heavy + 16 * inf.n_pairs + 4 * (13 + 2 * inf.n_torsions))
-    return float(np.mean(per_eval)) * st.evals
+def dominant_kernel(launches):
+    """The MC kernel instantiation with the most device time over the timed steps, with its
+    launch count, average launch duration and algorithmic bytes per launch (per evaluation:
+    32 B per heavy atom for the eight map corners + 16 B per intramolecular pair for one
+    interpolation quad; the device counts evaluations per ligand)."""
+    by = {}
+    for rec in launches:
+        d = by.setdefault(rec.kernel, dict(n=0, ms=0.0, bytes=0.0, evals=0))
+        d["n"] += 1; d["ms"] += rec.kernel_ms; d["bytes"] += rec.eval_bytes; d["evals"] += rec.evals
+    name = max(by, key=lambda k: by[k]["ms"])
+    d = by[name]
+    return name, d, {k: round(v["ms"], 2) for k, v in by.items()}
 
 
 def run_cpu_baseline(vdock, texts, box, xyz, xs, chains, budget_s=15.0):
@@ -121,7 +123,7 @@ def main():
     for w in range(args.warmup):
         log(f"warmup {w}")
         ctx.dock_batch(ids, params)
-    times, kernel_ms, evals = [], 0.0, 0
+    times, kernel_ms, evals, launches = [], 0.0, 0, []
     st = None
     for s in range(args.steps):
         log(f"step {s}")
@@ -130,6 +132,7 @@ def main():
         times.append(time.perf_counter() - t0)
         kernel_ms += st.kernel_ms
         evals += st.evals
+        launches += ctx.last_launches()
     wall = sum(times)
     value = args.ligands * args.steps / wall
     best = ctx.batch_best()
@@ -156,11 +159,15 @@ def main():
         "evals_per_s": round(evals / (kernel_ms / 1000.0), 1),
         "mean_best_score": round(float(np.mean(best)), 4),
     }
-    mbytes = mc_bytes(ctx, ids, st) if st else 0.0
-    achieved = mbytes / (st.kernel_ms / 1000.0) / 1e9 if st and st.kernel_ms > 0 else 0.0
-    out["roofline"] = {"kernel": "vd_mc_kernel (persistent)", "bound": "hbm", "achieved": round(achieved, 2),
+    name, d, ms_by_kernel = dominant_kernel(launches)
+    avg_ms = d["ms"] / d["n"]
+    per_launch = d["bytes"] / d["n"]
+    achieved = per_launch / (avg_ms / 1000.0) / 1e9
+    out["roofline"] = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2),
                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                       "traffic": None}
+                       "traffic": None, "launches": d["n"], "avg_launch_ms": round(avg_ms, 3),
+                       "algorithmic_bytes_per_launch": round(per_launch), "evals_per_launch": d["evals"] // d["n"],
+                       "kernel_ms_by_instantiation": ms_by_kernel}
 
     if not args.no_extras:
         # per-ligand plain launch loop (one kernel_monte-style launch per ligand) on the same

@@ -134,6 +134,23 @@ typedef struct {
   int blocks;                  /* workgroups launched */
 } vd_stats;
 
+/* One MC kernel launch of the last vd_dock / vd_dock_batch / vd_dock_batch_plain call (the
+ * autotuner's trial launches excluded), in launch order.  The kernel is
+ * vd_mc_kernel<lanes_per_chain, mode, waves_per_simd>; mode 0 dense BFGS, 1 RILC-BFGS,
+ * 2 RILC-BFGS with the QuickVina2 history.  eval_bytes = algorithmic bytes of its evaluations:
+ * per evaluation 32 B per heavy atom (eight map corners) and 16 B per intramolecular pair (one
+ * interpolation quad of the pair table).  No counterpart in the reference (which prints only
+ * per-ligand kernel times, V11/lib/main_procedure_cu.cpp:975-994); it serves the roofline measurement. */
+typedef struct {
+  int lanes_per_chain, mode, waves_per_simd;
+  int blocks, items, ligands, persistent;
+  double kernel_ms;
+  unsigned long long evals;
+  double eval_bytes;
+} vd_launch;
+/* Copies up to max records; returns the number of launches (>= 0) or an error. */
+int vd_last_launches(vd_ctx* ctx, int max, vd_launch* out);
+
 /* Plain per-ligand launch (replaces kernel_monte, V11/inc/cuda/kernel2.cuh:65-79).
  * Per chain results: e[chains], conf[chains][7+T], heavy[chains][n_heavy][3] (host pointers,
  * any may be NULL).  The box is the one of the resident maps. */

@@ -90,6 +90,7 @@ struct vd_ctx {
   std::map<int, std::pair<int, int>> tuned;  // size bucket -> (lanes per chain, waves per SIMD)
   DevBuf<float> tune_e, tune_conf, tune_heavy;
   DevBuf<float> hist;                        // QuickVina2 visited histories
+  std::vector<vd_launch> launches;           // MC launches of the last docking call
 };
 
 namespace {
@@ -262,6 +263,10 @@ double elapsed_ms(vd_ctx* c) {
   return ms;
 }
 
+// Algorithmic bytes of one evaluation in the MC kernel: the eight map corners of every typed
+// atom (8 x 4 B) and one interpolation quad of the pair table per intramolecular pair (16 B).
+double eval_bytes(const vd_lig_header& h) { return 32.0 * h.n_heavy + 16.0 * h.n_pairs; }
+
 // Runs one MC launch over the given ligand slots (all with the same G) and work items.
 void run_mc(vd_ctx* c, int G, int W, const Packed& pk, const std::vector<uint32_t>& streams, const std::vector<int2>& items,
             const std::vector<longlong4>& off, const vd_mc_consts& P, float* d_e, float* d_conf, float* d_heavy,
@@ -272,20 +277,20 @@ void run_mc(vd_ctx* c, int G, int W, const Packed& pk, const std::vector<uint32_
   c->items.ensure(items.size());
   c->offs.ensure(off.size());
   c->queue.ensure(1);
-  c->evals.ensure(1);
+  c->evals.ensure(pk.headers.size());
   HIP_CHECK(hipMemcpyAsync(c->blob.p, pk.blob.data(), pk.blob.size() * 4, hipMemcpyHostToDevice, c->stream));
   HIP_CHECK(hipMemcpyAsync(c->headers.p, pk.headers.data(), pk.headers.size() * sizeof(vd_lig_header), hipMemcpyHostToDevice, c->stream));
   HIP_CHECK(hipMemcpyAsync(c->streams.p, streams.data(), streams.size() * 4, hipMemcpyHostToDevice, c->stream));
   HIP_CHECK(hipMemcpyAsync(c->items.p, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
   HIP_CHECK(hipMemcpyAsync(c->offs.p, off.data(), off.size() * sizeof(longlong4), hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemsetAsync(c->evals.p, 0, sizeof(unsigned long long), c->stream));
+  HIP_CHECK(hipMemsetAsync(c->evals.p, 0, pk.headers.size() * sizeof(unsigned long long), c->stream));
   McArgs a{};
   a.blob = c->blob.p; a.headers = c->headers.p; a.streams = c->streams.p; a.items = c->items.p;
   a.n_items = (int)items.size();
   a.K = make_evalk(c, P.v_intra, P.v_inter, P.slope);
   a.P = P;
   a.out_e = d_e; a.out_conf = d_conf; a.out_heavy = d_heavy; a.out_off = c->offs.p;
-  a.eval_count = c->evals.p;
+  a.slot_evals = c->evals.p;
   a.Amax = pk.Amax; a.Nmax = pk.Nmax; a.Pmax = pk.Pmax;
   const int lds = vdl::mc_lds_bytes(G, pk.Amax, pk.Nmax, pk.Pmax, P.local_search == 0);
   if (lds > 64 * 1024) throw ArgError(VD_ERR_CAPACITY, "ligand too large for the LDS layout");
@@ -308,16 +313,24 @@ void run_mc(vd_ctx* c, int G, int W, const Packed& pk, const std::vector<uint32_
   if (timed) HIP_CHECK(hipEventRecord(c->ev0, c->stream));
   HIP_CHECK(vdl::launch_mc(G, W, a, blocks, c->stream));
   if (timed) HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+  std::vector<unsigned long long> ev(pk.headers.size());
+  HIP_CHECK(hipMemcpyAsync(ev.data(), c->evals.p, ev.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  vd_launch rec{};
+  rec.lanes_per_chain = G; rec.waves_per_simd = W; rec.blocks = blocks; rec.items = (int)items.size();
+  rec.ligands = (int)pk.headers.size(); rec.persistent = persistent ? 1 : 0;
+  rec.mode = P.local_search == 0 ? 0 : (P.policy == VD_MC_QVINA2 ? 2 : 1);
+  rec.kernel_ms = timed ? elapsed_ms(c) : 0.0;
+  for (size_t k = 0; k < ev.size(); ++k) {
+    rec.evals += ev[k];
+    rec.eval_bytes += (double)ev[k] * eval_bytes(pk.headers[k]);
+  }
+  if (timed) c->launches.push_back(rec);
   if (st) {
-    unsigned long long ev = 0;
-    HIP_CHECK(hipMemcpyAsync(&ev, c->evals.p, sizeof(ev), hipMemcpyDeviceToHost, c->stream));
-    HIP_CHECK(hipStreamSynchronize(c->stream));
-    st->evals += ev;
+    st->evals += rec.evals;
     st->blocks += blocks;
     st->chains_per_wave = 64 / G;
-    if (timed) st->kernel_ms += elapsed_ms(c);
-  } else {
-    HIP_CHECK(hipStreamSynchronize(c->stream));
+    st->kernel_ms += rec.kernel_ms;
   }
 }
 
@@ -707,6 +720,7 @@ int vd_dock(vd_ctx* c, int id, const vd_dock_params* p, float* e, float* conf, f
     if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no maps");
     if (st) std::memset(st, 0, sizeof(*st));
     if (p->chains_per_wave == 0 || p->waves_per_simd == 0) autotune(c, {id}, p);
+    c->launches.clear();
     LigPlan pl = plan_for(c, id, p);
     prepare_batch(c, {pl});
     std::vector<int> dv{pl.depth}, bv{pl.bfgs};
@@ -730,6 +744,7 @@ static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, 
     if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no maps");
     if (st) std::memset(st, 0, sizeof(*st));
     if (p->chains_per_wave == 0 || p->waves_per_simd == 0) autotune(c, std::vector<int>(ids, ids + n), p);
+    c->launches.clear();
     std::vector<LigPlan> plans;
     for (int k = 0; k < n; ++k) plans.push_back(plan_for(c, ids[k], p));
     prepare_batch(c, plans);
@@ -781,6 +796,13 @@ int vd_dock_batch(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, vd_
 }
 int vd_dock_batch_plain(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, vd_stats* st) {
   return dock_many(c, n, ids, p, st, false);
+}
+
+int vd_last_launches(vd_ctx* c, int max, vd_launch* out) {
+  if (!c || max < 0 || (max > 0 && !out)) return VD_ERR_ARG;
+  const int n = (int)c->launches.size();
+  for (int k = 0; k < n && k < max; ++k) out[k] = c->launches[k];
+  return n;
 }
 
 int vd_batch_results(vd_ctx* c, int k, float* e, float* conf, float* heavy) {

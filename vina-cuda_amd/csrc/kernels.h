@@ -23,7 +23,7 @@ struct McArgs {
   vd_mc_consts P;
   float* out_e; float* out_conf; float* out_heavy;
   const longlong4* out_off;         // per ligand slot: (e, conf, heavy) element offsets
-  unsigned long long* eval_count;
+  unsigned long long* slot_evals;    // per ligand slot: evaluations performed (zeroed by the caller)
   int Amax, Nmax, Pmax;
   float* hist;                      // QuickVina2 visited histories (nullptr: Vina policy),
   long long hist_stride;            // floats per chain region; region = block * chains/wave + group

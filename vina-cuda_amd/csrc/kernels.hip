@@ -225,7 +225,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
   constexpr int CPW = 64 / G;
   const int lane = threadIdx.x, group = lane / G, l = lane % G;
   int cur = -1;
-  long long my_evals = 0;
   for (int it = 0;; ++it) {
     int item;
     if (a.queue) {
@@ -253,12 +252,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
     if constexpr (MODE == 2) hist.init(a.hist + ((long long)blockIdx.x * CPW + group) * a.hist_stride, h.n_torsions);
     mc_chain<G, MODE>(L, S, H, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l, a.out_e + a.out_off[wi.x].x,
                 a.out_conf + a.out_off[wi.x].y, a.out_heavy + a.out_off[wi.x].z, evals, &hist);
-    if (l == 0) my_evals += evals;
-  }
-  if (a.eval_count) {
-    long long sum = 0;
-    for (int gi = 0; gi < CPW; ++gi) sum += __shfl(my_evals, gi * G);
-    if (lane == 0) atomicAdd(a.eval_count, (unsigned long long)sum);
+    if (a.slot_evals) {
+      int sum = 0;
+      for (int gi = 0; gi < CPW; ++gi) sum += __shfl(evals, gi * G);
+      if (lane == 0) atomicAdd(a.slot_evals + wi.x, (unsigned long long)sum);
+    }
   }
 }
 

@@ -29,6 +29,7 @@ EXPORTS = (
     "vd_eval", "vd_eval_last_ms", "vd_dock_params_default", "vd_dock_heuristics", "vd_dock",
     "vd_dock_batch", "vd_dock_batch_plain", "vd_batch_results", "vd_batch_best", "vd_cluster",
     "vd_synth_receptor", "vd_synth_ligand", "vd_device_alloc", "vd_device_free", "vd_copy", "vd_random_confs",
+    "vd_last_launches",
 )
 
 
@@ -64,6 +65,17 @@ class DockParams(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("evals", C.c_ulonglong), ("chains_per_wave", C.c_int), ("blocks", C.c_int)]
+
+
+class Launch(C.Structure):
+    _fields_ = [("lanes_per_chain", C.c_int), ("mode", C.c_int), ("waves_per_simd", C.c_int), ("blocks", C.c_int),
+                ("items", C.c_int), ("ligands", C.c_int), ("persistent", C.c_int), ("kernel_ms", C.c_double),
+                ("evals", C.c_ulonglong), ("eval_bytes", C.c_double)]
+
+    @property
+    def kernel(self):
+        """Name of the launched kernel as a profiler lists it (template arguments)."""
+        return "vd_mc_kernel<%d, %d, %d>" % (self.lanes_per_chain, self.mode, self.waves_per_simd)
 
 
 _lib = None
@@ -106,6 +118,7 @@ def load():
         "vd_dock_batch_plain": (C.c_int, [vp, C.c_int, ip, C.POINTER(DockParams), C.POINTER(Stats)]),
         "vd_batch_results": (C.c_int, [vp, C.c_int, fp, fp, fp]),
         "vd_batch_best": (C.c_int, [vp, fp]),
+        "vd_last_launches": (C.c_int, [vp, C.c_int, C.POINTER(Launch)]),
         "vd_cluster": (C.c_int, [C.c_int, C.c_int, fp, fp, C.c_double, C.c_int, ip]),
         "vd_device_alloc": (C.c_int, [vp, C.c_size_t, C.POINTER(C.c_void_p)]),
         "vd_device_free": (C.c_int, [vp, vp]),
@@ -346,6 +359,14 @@ class Context:
         out = np.zeros(len(self._batch_ids), dtype=np.float32)
         self._check(self.lib.vd_batch_best(self.h, _f(out)))
         return out
+
+    def last_launches(self):
+        """MC kernel launches of the last docking call (list of Launch)."""
+        n = self.lib.vd_last_launches(self.h, 0, None)
+        self._check(min(n, 0))
+        arr = (Launch * max(n, 1))()
+        self._check(min(self.lib.vd_last_launches(self.h, n, arr), 0))
+        return list(arr[:n])
 
 
 def version():
