@@ -93,6 +93,12 @@ int vd_ligand_info_get(vd_ctx* ctx, int id, vd_ligand_info* info);
 int vd_ligand_export(vd_ctx* ctx, int id, int* xs, int* el, int* node, float* local, int* parent,
                      int* begin, int* end, float* rel_origin, float* rel_axis, int* pairs);
 int vd_ligand_count(const vd_ctx* ctx);
+/* Device layout of the intramolecular pairs for chain groups of `lanes` lanes (16, 32, 64):
+ * chunks of `lanes` pair words (a | b << 8 | tp << 16, 0xFFFFFFFF = empty slot), each atom at most
+ * once as a and at most once as b per chunk.  Copies up to max words; returns the word count
+ * (chunks x lanes).  Layout of this implementation, no counterpart in the reference (which
+ * scatters pair forces without synchronisation, V11/inc/cuda/quasi_newton.cuh:941-944). */
+int vd_ligand_pair_chunks(vd_ctx* ctx, int id, int lanes, int max, uint32_t* words);
 
 /* Energy and gradient of n conformations of ligand `id` (model::eval_deriv,
  * V11/lib/model.cpp:1324-1334; device form V11/inc/cuda/quasi_newton.cuh:1294-1339).

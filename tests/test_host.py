@@ -179,3 +179,33 @@ def test_synthetic_receptor_recipe():
     assert np.linalg.norm(xyz, axis=1).min() >= 8.0 - 1e-5
     frac = np.bincount(xs, minlength=17) / 2000.0
     assert abs(frac[0] - 0.35) < 0.05 and abs(frac[1] - 0.28) < 0.05 and abs(frac[8] - 0.12) < 0.04
+
+
+@pytest.mark.parametrize("lanes", [16, 32, 64])
+def test_pair_chunks_cover_every_pair_once_without_conflicts(host_ctx, lanes):
+    """Device pair layout: every intramolecular pair exactly once (either way round, same type
+    pair), and within a chunk no atom is the first atom of two pairs or the second atom of two
+    pairs -- the condition under which the kernels scatter pair forces without atomics."""
+    import vdock
+    from conftest import golden
+    texts = [golden(f"{n}_ligand.pdbqt") for n in ("1u1b", "2bm2", "2er7")]
+    texts += [vdock.synth_ligand(h, min(16, h // 3), 300 + h) for h in (10, 25, 40, 60)]
+    for text in texts:
+        lid = host_ctx.add_ligand(text)
+        pairs = host_ctx.ligand_export(lid)["pairs"]          # (tp, a, b)
+        ch = host_ctx.pair_chunks(lid, lanes)
+        assert ch.shape[1] == lanes
+        words = ch.ravel()
+        used = words[words != 0xFFFFFFFF]
+        a, b, tp = used & 255, (used >> 8) & 255, used >> 16
+        got = sorted(zip(tp.tolist(), np.minimum(a, b).tolist(), np.maximum(a, b).tolist()))
+        want = sorted(zip(pairs[:, 0].tolist(), np.minimum(pairs[:, 1], pairs[:, 2]).tolist(),
+                          np.maximum(pairs[:, 1], pairs[:, 2]).tolist()))
+        assert got == want
+        for row in ch:
+            r = row[row != 0xFFFFFFFF]
+            assert len(set((r & 255).tolist())) == len(r)
+            assert len(set(((r >> 8) & 255).tolist())) == len(r)
+        # a chunk holds at most `lanes` pairs and each atom in at most two of them
+        deg = np.bincount(np.concatenate([pairs[:, 1], pairs[:, 2]]), minlength=1)
+        assert ch.shape[0] >= max(-(-len(pairs) // lanes), -(-int(deg.max()) // 2))

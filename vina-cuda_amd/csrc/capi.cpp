@@ -241,17 +241,18 @@ struct Packed {
   std::vector<vd_lig_header> headers;
   int Amax = 0, Nmax = 0, Pmax = 0, Tmax = 0;
 };
-Packed pack(vd_ctx* c, const std::vector<int>& ids, const std::vector<int>* depth = nullptr,
+// Ligand records for chain groups of G lanes; Pmax counts pair slots (chunks x G).
+Packed pack(vd_ctx* c, int G, const std::vector<int>& ids, const std::vector<int>* depth = nullptr,
             const std::vector<int>* bfgs = nullptr) {
   Packed pk;
   for (size_t k = 0; k < ids.size(); ++k) {
     const vd::Ligand& L = lig_at(c, ids[k]);
-    pk.headers.push_back(vd::pack_ligand(L, pk.blob));
+    pk.headers.push_back(vd::pack_ligand(L, pk.blob, G));
     if (depth) pk.headers.back().search_depth = (*depth)[k];
     if (bfgs) pk.headers.back().bfgs_steps = (*bfgs)[k];
     pk.Amax = std::max(pk.Amax, (int)L.atoms.size());
     pk.Nmax = std::max(pk.Nmax, (int)L.nodes.size());
-    pk.Pmax = std::max(pk.Pmax, (int)L.pairs.size());
+    pk.Pmax = std::max(pk.Pmax, pk.headers.back().n_chunks * G);
     pk.Tmax = std::max(pk.Tmax, L.n_torsions());
   }
   return pk;
@@ -294,11 +295,18 @@ void run_mc(vd_ctx* c, int G, int W, const Packed& pk, const std::vector<uint32_
   a.Amax = pk.Amax; a.Nmax = pk.Nmax; a.Pmax = pk.Pmax;
   const int lds = vdl::mc_lds_bytes(G, pk.Amax, pk.Nmax, pk.Pmax, P.local_search == 0);
   if (lds > 64 * 1024) throw ArgError(VD_ERR_CAPACITY, "ligand too large for the LDS layout");
+  const int mode = P.local_search == 0 ? 0 : (P.policy == VD_MC_QVINA2 ? 2 : 1);
+  // a tighter register budget only pays when the LDS of this launch lets the extra waves in:
+  // take the loosest budget that reaches the same residency
+  if (mode != 0) {
+    const int occ = vdl::max_resident_mc_blocks(G, W, mode, lds);
+    for (int w = 2; w < W; ++w)
+      if (vdl::max_resident_mc_blocks(G, w, mode, lds) >= occ) { W = w; break; }
+  }
   int blocks = (int)items.size();
   if (persistent) {
     a.queue = c->queue.p;
     HIP_CHECK(vdl::reset_counter(c->queue.p, c->stream));
-    const int mode = P.local_search == 0 ? 0 : (P.policy == VD_MC_QVINA2 ? 2 : 1);
     const int per_cu = std::max(1, vdl::max_resident_mc_blocks(G, W, mode, lds));
     blocks = std::min<long long>((long long)per_cu * c->n_cu, (long long)items.size());
     blocks = std::max(blocks, 1);
@@ -319,7 +327,7 @@ void run_mc(vd_ctx* c, int G, int W, const Packed& pk, const std::vector<uint32_
   vd_launch rec{};
   rec.lanes_per_chain = G; rec.waves_per_simd = W; rec.blocks = blocks; rec.items = (int)items.size();
   rec.ligands = (int)pk.headers.size(); rec.persistent = persistent ? 1 : 0;
-  rec.mode = P.local_search == 0 ? 0 : (P.policy == VD_MC_QVINA2 ? 2 : 1);
+  rec.mode = mode;
   rec.kernel_ms = timed ? elapsed_ms(c) : 0.0;
   for (size_t k = 0; k < ev.size(); ++k) {
     rec.evals += ev[k];
@@ -369,7 +377,7 @@ void autotune(vd_ctx* c, const std::vector<int>& ids, const vd_dock_params* p) {
         const int blocks = std::max(1, c->n_cu) * 4 * W;
         q.chains = blocks * cpw;
         std::vector<int> dv{2}, bv{std::max(bfgs, 1)};
-        Packed pk = pack(c, {kv.second}, &dv, &bv);
+        Packed pk = pack(c, G, {kv.second}, &dv, &bv);
         std::vector<int2> items;
         for (int c0 = 0; c0 < q.chains; c0 += cpw) items.push_back(make_int2(0, c0));
         std::vector<longlong4> off{make_longlong4(0, 0, 0, 0)};
@@ -664,7 +672,7 @@ int vd_eval(vd_ctx* c, int id, long long n, const float* conf, float* e, float* 
     if (n == 0) return;
     const int T = L.n_torsions();
     int G = pick_group(c, L, cpw);
-    Packed pk = pack(c, {id});
+    Packed pk = pack(c, G, {id});
     c->blob.ensure(pk.blob.size() / 4 + 1);
     c->headers.ensure(1);
     HIP_CHECK(hipMemcpyAsync(c->blob.p, pk.blob.data(), pk.blob.size() * 4, hipMemcpyHostToDevice, c->stream));
@@ -724,7 +732,7 @@ int vd_dock(vd_ctx* c, int id, const vd_dock_params* p, float* e, float* conf, f
     LigPlan pl = plan_for(c, id, p);
     prepare_batch(c, {pl});
     std::vector<int> dv{pl.depth}, bv{pl.bfgs};
-    Packed pk = pack(c, {id}, &dv, &bv);
+    Packed pk = pack(c, pl.G, {id}, &dv, &bv);
     const int cpw = 64 / pl.G;
     std::vector<int2> items;
     for (int c0 = 0; c0 < p->chains; c0 += cpw) items.push_back(make_int2(0, c0));
@@ -754,7 +762,7 @@ static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, 
       for (int k = 0; k < n; ++k) {
         const LigPlan& pl = plans[k];
         std::vector<int> dv{pl.depth}, bv{pl.bfgs};
-        Packed pk = pack(c, {pl.id}, &dv, &bv);
+        Packed pk = pack(c, pl.G, {pl.id}, &dv, &bv);
         const int cpw = 64 / pl.G;
         std::vector<int2> items;
         for (int c0 = 0; c0 < p->chains; c0 += cpw) items.push_back(make_int2(0, c0));
@@ -772,14 +780,14 @@ static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, 
       std::vector<int>& ks = cl.second;
       std::vector<int> lig_ids, dv, bv;
       for (int k : ks) { lig_ids.push_back(plans[k].id); dv.push_back(plans[k].depth); bv.push_back(plans[k].bfgs); }
-      Packed pk = pack(c, lig_ids, &dv, &bv);
+      Packed pk = pack(c, G, lig_ids, &dv, &bv);
       std::vector<uint32_t> streams;
       std::vector<longlong4> off;
       for (int k : ks) { streams.push_back((uint32_t)plans[k].id); off.push_back(c->batch.off[k]); }
       // cost estimate per chain: steps x iterations x (atoms + pairs)
       std::vector<std::pair<double, int>> order;
       for (size_t s = 0; s < ks.size(); ++s)
-        order.push_back({-(double)dv[s] * bv[s] * (pk.headers[s].n_atoms + 0.5 * pk.headers[s].n_pairs + 16), (int)s});
+        order.push_back({-(double)dv[s] * bv[s] * (pk.headers[s].n_atoms + 0.5 * pk.headers[s].n_chunks * G + 16), (int)s});
       std::stable_sort(order.begin(), order.end());
       const int cpw = 64 / G;
       std::vector<int2> items;
@@ -796,6 +804,18 @@ int vd_dock_batch(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, vd_
 }
 int vd_dock_batch_plain(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, vd_stats* st) {
   return dock_many(c, n, ids, p, st, false);
+}
+
+int vd_ligand_pair_chunks(vd_ctx* c, int id, int lanes, int max, uint32_t* words) {
+  if (!c || max < 0 || (max > 0 && !words)) return VD_ERR_ARG;
+  int n = 0;
+  const int rc = guard(c, [&]() {
+    if (lanes != 16 && lanes != 32 && lanes != 64) throw ArgError(VD_ERR_ARG, "lanes must be 16, 32 or 64");
+    const std::vector<uint32_t>& w = vd::pair_chunks(lig_at(c, id), lanes);
+    n = (int)w.size();
+    for (int k = 0; k < n && k < max; ++k) words[k] = w[k];
+  });
+  return rc == VD_OK ? n : rc;
 }
 
 int vd_last_launches(vd_ctx* c, int max, vd_launch* out) {

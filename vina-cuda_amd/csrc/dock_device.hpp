@@ -124,12 +124,12 @@ struct LigView {               // one ligand, staged in LDS for the wave
   const float4* atom;          // A: local xyz + meta (node | xs << 8 | heavy << 16)
   const float4* node_o;        // N: rel_origin xyz + bits (parent+1 | atom begin << 8 | atom end << 16)
   const float4* node_a;        // N: rel_axis xyz + subtree_end
-  const uint32_t* pairs;       // P, staged in LDS
-  int A, N, P, T, H, rounds;
+  const uint32_t* pairs;       // C chunks x G pair words, staged in LDS
+  int A, N, C, T, H, rounds;
 };
 struct Scratch {               // per chain, LDS
-  float* xyz;                  // 3A, coordinates relative to the root origin
-  float* frc;                  // 3A, dE/dx
+  float4* xyz;                 // A, coordinates relative to the root origin (w unused)
+  float4* frc;                 // A, dE/dx (w unused)
   float4* frame;               // 3N rows (R row, origin component)
   float4* nacc;                // 2N: per-node force and moment sums
   float* stash;                // per wave: conformations parked out of registers, [slot][field][lane]
@@ -243,7 +243,7 @@ __device__ void set_coords(const LigView& L, const Scratch& S, const Conf& c, in
   __syncthreads();
   for (int i = l; i < L.A; i += G) {
     uint32_t meta; const float3 y = atom_position(L, S, i, meta);
-    S.xyz[3 * i] = y.x; S.xyz[3 * i + 1] = y.y; S.xyz[3 * i + 2] = y.z;
+    S.xyz[i] = make_float4(y.x, y.y, y.z, 0.0f);
   }
   __syncthreads();
 }
@@ -255,7 +255,7 @@ __device__ float gyration_radius(const LigView& L, const Scratch& S, int l) {
   float acc = 0.0f, cnt = 0.0f;
   for (int i = l; i < L.A; i += G) {
     const uint32_t meta = fbits(L.atom[i].w);
-    if ((meta >> 16) & 1u) { acc += S.xyz[3 * i] * S.xyz[3 * i] + S.xyz[3 * i + 1] * S.xyz[3 * i + 1] + S.xyz[3 * i + 2] * S.xyz[3 * i + 2]; cnt += 1.0f; }
+    if ((meta >> 16) & 1u) { const float4 y = S.xyz[i]; acc += y.x * y.x + y.y * y.y + y.z * y.z; cnt += 1.0f; }
   }
   acc = gsum<G>(acc); cnt = gsum<G>(cnt);
   return cnt > 0.0f ? sqrtf(acc / cnt) : 0.0f;
@@ -292,31 +292,34 @@ __device__ float eval_deriv(const LigView& L, const Scratch& S, const EvalK& K, 
     for (int u = 0; u < 2; ++u) {
       const int i = i0 + u * G;
       if (i < L.A) {
-        S.xyz[3 * i] = y[u].x; S.xyz[3 * i + 1] = y[u].y; S.xyz[3 * i + 2] = y[u].z;
-        S.frc[3 * i] = d[u].x; S.frc[3 * i + 1] = d[u].y; S.frc[3 * i + 2] = d[u].z;
+        S.xyz[i] = make_float4(y[u].x, y[u].y, y[u].z, 0.0f);
+        S.frc[i] = make_float4(d[u].x, d[u].y, d[u].z, 0.0f);
         e_inter += ev[u];
       }
     }
   }
   __syncthreads();
-  // intramolecular pairs (eval_interacting_pairs_deriv, V11/lib/model.cpp:1241-1259), four per
-  // lane in flight: all four table reads are issued before the first is used
+  // intramolecular pairs (eval_interacting_pairs_deriv, V11/lib/model.cpp:1241-1259).  Lane l
+  // takes slot l of each chunk; within a chunk the first atoms are distinct and so are the second
+  // atoms, so the force scatter is a plain read-modify-write of LDS (first atoms, then second
+  // atoms), in chunk order: no atomics, and the same summation order in every run.  Four chunks
+  // per pass: all four table reads are issued before the first is used.
   float e_intra = 0.0f;
   constexpr int U = 4;
-  for (int p0 = l; p0 < L.P; p0 += U * G) {
+  for (int c0 = 0; c0 < L.C; c0 += U) {
     float rx[U], ry[U], rz[U], rem[U];
     int a[U], b[U];
     bool ok[U];
     float4 q[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int p = p0 + u * G;
-      ok[u] = p < L.P;
-      const uint32_t w = ok[u] ? L.pairs[p] : 0u;
-      a[u] = w & 255; b[u] = (w >> 8) & 255;
-      const int tp = w >> 16;
-      rx[u] = S.xyz[3 * b[u]] - S.xyz[3 * a[u]]; ry[u] = S.xyz[3 * b[u] + 1] - S.xyz[3 * a[u] + 1];
-      rz[u] = S.xyz[3 * b[u] + 2] - S.xyz[3 * a[u] + 2];
+      const int ch = c0 + u;
+      const uint32_t w = ch < L.C ? L.pairs[ch * G + l] : VD_NO_PAIR;
+      ok[u] = w != VD_NO_PAIR;
+      a[u] = ok[u] ? (int)(w & 255) : 0; b[u] = ok[u] ? (int)((w >> 8) & 255) : 0;
+      const int tp = ok[u] ? (int)(w >> 16) : 0;
+      const float4 pa = S.xyz[a[u]], pb = S.xyz[b[u]];
+      rx[u] = pb.x - pa.x; ry[u] = pb.y - pa.y; rz[u] = pb.z - pa.z;
       const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
       ok[u] = ok[u] && r2 < VD_CUTOFF_SQR;
       const float rf = VD_LUT_FACTOR * r2;
@@ -332,8 +335,12 @@ __device__ float eval_deriv(const LigView& L, const Scratch& S, const EvalK& K, 
       if (e > 0.0f) { const float t = (K.v_intra < kEpsFl) ? 0.0f : K.v_intra / (K.v_intra + e); e *= t; dor *= t * t; }
       e_intra += e;
       const float fx = dor * rx[u], fy = dor * ry[u], fz = dor * rz[u];
-      atomicAdd(&S.frc[3 * a[u]], -fx); atomicAdd(&S.frc[3 * a[u] + 1], -fy); atomicAdd(&S.frc[3 * a[u] + 2], -fz);
-      atomicAdd(&S.frc[3 * b[u]], fx); atomicAdd(&S.frc[3 * b[u] + 1], fy); atomicAdd(&S.frc[3 * b[u] + 2], fz);
+      float4 fa = S.frc[a[u]];
+      fa.x -= fx; fa.y -= fy; fa.z -= fz;
+      S.frc[a[u]] = fa;
+      float4 fb = S.frc[b[u]];
+      fb.x += fx; fb.y += fy; fb.z += fz;
+      S.frc[b[u]] = fb;
     }
   }
   __syncthreads();
@@ -344,8 +351,8 @@ __device__ float eval_deriv(const LigView& L, const Scratch& S, const EvalK& K, 
     const int begin = (nb >> 8) & 255, end = (nb >> 16) & 255;
     float f0 = 0, f1 = 0, f2 = 0, m0 = 0, m1 = 0, m2 = 0;
     for (int i = begin; i < end; ++i) {
-      const float fx = S.frc[3 * i], fy = S.frc[3 * i + 1], fz = S.frc[3 * i + 2];
-      const float x = S.xyz[3 * i], y = S.xyz[3 * i + 1], z = S.xyz[3 * i + 2];
+      const float4 fr = S.frc[i], p = S.xyz[i];
+      const float fx = fr.x, fy = fr.y, fz = fr.z, x = p.x, y = p.y, z = p.z;
       f0 += fx; f1 += fy; f2 += fz;
       m0 += y * fz - z * fy; m1 += z * fx - x * fz; m2 += x * fy - y * fx;
     }
@@ -391,17 +398,34 @@ __device__ __forceinline__ Conf increment(const Conf& x, float d, float factor, 
 __device__ __forceinline__ Conf csel(bool p, const Conf& a, const Conf& b) { return p ? a : b; }
 
 // Conformations that are only read now and then live in LDS instead of registers (the MC kernel
-// is register-bound); slot s of lane `lane` is fields [s][0..7][lane].
+// is register-bound).  A slot holds, per group, the position and orientation once (8 floats,
+// written by the group's lane 0) and, per lane, its torsion: [slot][group][8] then [slot][64].
+// LDS operations of one wavefront complete in order; the wavefront fences keep the compiler
+// from moving a read of a slot above the write that fills it.
 constexpr int kStashSlots = 4;
+constexpr int kStashSlotFloats = 4 * 8 + 64;
 enum { kSlotOrig = 0, kSlotPrev = 1, kSlotTmp = 2, kSlotBest = 3 };
+template <int G>
 __device__ __forceinline__ void stash_put(float* st, int slot, const Conf& c) {
-  float* p = st + slot * 512 + (threadIdx.x & 63);
-  p[0] = c.px; p[64] = c.py; p[128] = c.pz; p[192] = c.q.w; p[256] = c.q.x; p[320] = c.q.y; p[384] = c.q.z; p[448] = c.tor;
+  const int lane = threadIdx.x & 63;
+  float* p = st + slot * kStashSlotFloats;
+  if (lane % G == 0) {
+    float4* u = reinterpret_cast<float4*>(p + (lane / G) * 8);
+    u[0] = make_float4(c.px, c.py, c.pz, c.q.w);
+    u[1] = make_float4(c.q.x, c.q.y, c.q.z, 0.0f);
+  }
+  p[32 + lane] = c.tor;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 }
+template <int G>
 __device__ __forceinline__ Conf stash_get(const float* st, int slot) {
-  const float* p = st + slot * 512 + (threadIdx.x & 63);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int lane = threadIdx.x & 63;
+  const float* p = st + slot * kStashSlotFloats;
+  const float4* u = reinterpret_cast<const float4*>(p + (lane / G) * 8);
+  const float4 a = u[0], b = u[1];
   Conf c;
-  c.px = p[0]; c.py = p[64]; c.pz = p[128]; c.q = Quat{p[192], p[256], p[320], p[384]}; c.tor = p[448];
+  c.px = a.x; c.py = a.y; c.pz = a.z; c.q = Quat{a.w, b.x, b.y, b.z}; c.tor = p[32 + lane];
   return c;
 }
 
@@ -531,7 +555,7 @@ __device__ float rilc_bfgs(const LigView& L, const Scratch& S, const EvalK& K, C
   float fx = eval_deriv<G>(L, S, K, x, l, g);
   evals += enabled ? 1 : 0;
   const float fx_orig = fx;
-  stash_put(S.stash, kSlotOrig, x);
+  stash_put<G>(S.stash, kSlotOrig, x);
   bool search = enabled;
   if constexpr (QV) {
     const bool interesting = hist_interesting<G>(*hist, x, fx, g, l, L.T, enabled);
@@ -543,7 +567,7 @@ __device__ float rilc_bfgs(const LigView& L, const Scratch& S, const EvalK& K, C
   float step = 1.0f / sqrtf(gsum<G>(d * d));
   int k = 1;
   while (__any(active)) {
-    stash_put(S.stash, kSlotPrev, x);
+    stash_put<G>(S.stash, kSlotPrev, x);
     const float gp = g; const float fxp = fx;
     // line search
     const float dginit = gsum<G>(gp * d);
@@ -552,7 +576,7 @@ __device__ float rilc_bfgs(const LigView& L, const Scratch& S, const EvalK& K, C
     const float finit = fx, dgtest = 1e-4f * dginit, dstest = 0.1f * dginit;
     float mu = 0.0f, nu = 1.0e20f; bool brackt = false; int count = 0;
     while (__any(ls)) {
-      const Conf xt = increment<G>(stash_get(S.stash, kSlotPrev), d, step, l, L.T);
+      const Conf xt = increment<G>(stash_get<G>(S.stash, kSlotPrev), d, step, l, L.T);
       float gt;
       const float ft = eval_deriv<G>(L, S, K, csel(ls, xt, x), l, gt);
       const float gtd = gsum<G>(gt * d);
@@ -570,7 +594,7 @@ __device__ float rilc_bfgs(const LigView& L, const Scratch& S, const EvalK& K, C
     const float gnorm = sqrtf(gsum<G>(g * g));
     bool update = false;
     if (active) {
-      if (result < 0) { x = stash_get(S.stash, kSlotPrev); g = gp; fx = fxp; active = false; }
+      if (result < 0) { x = stash_get<G>(S.stash, kSlotPrev); g = gp; fx = fxp; active = false; }
       else if (!(gnorm >= 1e-5f)) active = false;
       else if ((max_steps != 0 && max_steps <= k) || k >= 1000) active = false;   // 1000: hard bound
       else { ++k; update = true; }
@@ -585,7 +609,7 @@ __device__ float rilc_bfgs(const LigView& L, const Scratch& S, const EvalK& K, C
     const float nd3 = nd2 + (alpha - beta) * s;
     if (update) { d = (ys > ss * sqrtf(gpgp) * 1.0e-6f) ? nd3 : nd; step = 1.0f; }
   }
-  if (enabled && !(fx <= fx_orig)) { fx = fx_orig; x = stash_get(S.stash, kSlotOrig); }
+  if (enabled && !(fx <= fx_orig)) { fx = fx_orig; x = stash_get<G>(S.stash, kSlotOrig); }
   return enabled ? fx : x_e;
 }
 

@@ -421,7 +421,46 @@ Ligand parse_ligand_pdbqt(const std::string& text) {
   return L;
 }
 
-vd_lig_header pack_ligand(const Ligand& L, std::vector<float>& blob) {
+const std::vector<uint32_t>& pair_chunks(const Ligand& L, int G) {
+  const int which = G == 16 ? 0 : G == 32 ? 1 : 2;
+  std::vector<uint32_t>& out = L.chunked[which];
+  if (!out.empty() || L.pairs.empty()) return out;
+  struct Chunk { uint64_t as[2] = {0, 0}, bs[2] = {0, 0}; int n = 0; };
+  auto has = [](const uint64_t* m, int i) { return (m[i >> 6] >> (i & 63)) & 1u; };
+  auto set = [](uint64_t* m, int i) { m[i >> 6] |= 1ull << (i & 63); };
+  std::vector<Chunk> chunks;
+  std::vector<std::vector<uint32_t>> slots;
+  size_t first_open = 0;
+  for (uint32_t p : L.pairs) {
+    const int a = p & 255, b = (p >> 8) & 255;
+    const uint32_t tp = p >> 16;
+    bool placed = false;
+    for (size_t c = first_open; c < chunks.size() && !placed; ++c) {
+      Chunk& ch = chunks[c];
+      if (ch.n >= G) continue;
+      int x = -1, y = -1;
+      if (!has(ch.as, a) && !has(ch.bs, b)) { x = a; y = b; }
+      else if (!has(ch.as, b) && !has(ch.bs, a)) { x = b; y = a; }
+      if (x < 0) continue;
+      set(ch.as, x); set(ch.bs, y); ++ch.n;
+      slots[c].push_back((uint32_t)x | ((uint32_t)y << 8) | (tp << 16));
+      placed = true;
+    }
+    if (!placed) {
+      chunks.emplace_back();
+      Chunk& ch = chunks.back();
+      set(ch.as, a); set(ch.bs, b); ch.n = 1;
+      slots.push_back({p});
+    }
+    while (first_open < chunks.size() && chunks[first_open].n >= G) ++first_open;
+  }
+  out.assign(chunks.size() * (size_t)G, VD_NO_PAIR);
+  for (size_t c = 0; c < slots.size(); ++c)
+    for (size_t k = 0; k < slots[c].size(); ++k) out[c * G + k] = slots[c][k];
+  return out;
+}
+
+vd_lig_header pack_ligand(const Ligand& L, std::vector<float>& blob, int G) {
   auto pad16 = [&]() { while (blob.size() % 4) blob.push_back(0.f); };
   auto as_f = [](uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; };
   vd_lig_header h;
@@ -449,13 +488,10 @@ vd_lig_header pack_ligand(const Ligand& L, std::vector<float>& blob) {
     blob.push_back((float)nd.rel_axis[0]); blob.push_back((float)nd.rel_axis[1]); blob.push_back((float)nd.rel_axis[2]);
     blob.push_back(as_f((uint32_t)nd.subtree_end));
   }
-  // device pair order: by diagonal (b - a, then a), so that the lanes of one pass over the list
-  // scatter forces to distinct atoms; only the order of the energy sum changes
-  std::vector<uint32_t> pairs(L.pairs);
-  std::stable_sort(pairs.begin(), pairs.end(), [](uint32_t x, uint32_t y) {
-    const int dx = (int)((x >> 8) & 255) - (int)(x & 255), dy = (int)((y >> 8) & 255) - (int)(y & 255);
-    return dx != dy ? dx < dy : (x & 255) < (y & 255);
-  });
+  // device pair order: conflict-free chunks of G slots (only the order of the energy sum changes)
+  const std::vector<uint32_t>& pairs = pair_chunks(L, G);
+  h.chunk_lanes = G;
+  h.n_chunks = (int)(pairs.size() / G);
   h.off_pairs = (int)blob.size();
   for (uint32_t p : pairs) blob.push_back(as_f(p));
   pad16();

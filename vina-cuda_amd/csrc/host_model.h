@@ -63,14 +63,23 @@ struct Ligand {
   int torsdof = 0, n_heavy = 0, max_depth = 0;
   std::vector<std::string> context; // REMARK/ROOT/BRANCH lines in file order (for output)
   int n_torsions() const { return (int)nodes.size() - 1; }
+  // conflict-free pair chunks per chain-group width (16, 32, 64), built on first use
+  mutable std::vector<uint32_t> chunked[3];
 };
+
+// Pairs in chunks of G slots such that within a chunk every atom is the first atom of at most
+// one pair and the second atom of at most one pair (each pair may be written either way round;
+// padding slots hold VD_NO_PAIR).  Greedy first fit; the chunk count is at least
+// max(ceil(P / G), ceil(max pair degree / 2)).
+const std::vector<uint32_t>& pair_chunks(const Ligand& L, int G);
 
 Ligand parse_ligand_pdbqt(const std::string& text);
 // Typed receptor atoms (rigid receptor only): xyz[3n], xs[n] (17 = no XS type)
 void parse_receptor_pdbqt(const std::string& text, std::vector<float>& xyz, std::vector<int>& xs);
 
-// Appends a ligand's packed record to a device blob (16-byte words); returns the header.
-vd_lig_header pack_ligand(const Ligand& L, std::vector<float>& blob);
+// Appends a ligand's packed record to a device blob (16-byte words), with its pairs chunked for
+// chain groups of G lanes; returns the header.
+vd_lig_header pack_ligand(const Ligand& L, std::vector<float>& blob, int G);
 
 // Synthetic inputs (recipes of the benchmark configurations).
 void synth_receptor(int n_atoms, double cube, double cavity_radius, uint64_t seed,

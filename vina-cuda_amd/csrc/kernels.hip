@@ -78,12 +78,13 @@ __global__ __launch_bounds__(256) void vd_grid_kernel(const float4* __restrict__
 }
 
 // ------------------------------------------------------------------------------------------
-// LDS layout of one wave: ligand (atoms 4A, node_o 4N, node_a 4N floats, pair words P), then per chain group
-// frame 12N, nacc 8N, xyz 3A (padded to 4), frc 3A (padded), and for dense BFGS H G*G + 2G.
+// LDS layout of one wave: ligand (atoms 4A, node_o 4N, node_a 4N floats, P pair slots), the
+// conformation stash, then per chain group frame 12N, nacc 8N, xyz 4A, frc 4A, and for dense
+// BFGS H G*G + 2G.
 __host__ __device__ inline int pad4(int n) { return (n + 3) & ~3; }
-__host__ __device__ inline int lig_floats(int A, int N, int P) { return 4 * A + 8 * N + pad4(P) + kStashSlots * 512; }
+__host__ __device__ inline int lig_floats(int A, int N, int P) { return 4 * A + 8 * N + pad4(P) + kStashSlots * kStashSlotFloats; }
 __host__ __device__ inline int group_floats(int A, int N, int G, int dense) {
-  return 12 * N + 8 * N + 2 * pad4(3 * A) + (dense ? G * G + 2 * G : 0);
+  return 12 * N + 8 * N + 8 * A + (dense ? G * G + 2 * G : 0);
 }
 
 template <int G>
@@ -93,7 +94,7 @@ __device__ __forceinline__ void stage_ligand(const float4* __restrict__ blob, co
   for (int i = lane; i < 2 * h.n_nodes; i += 64) dst[h.n_atoms + i] = blob[h.off_nodes + i];
   uint32_t* pw = reinterpret_cast<uint32_t*>(dst + h.n_atoms + 2 * h.n_nodes);
   const uint32_t* src = reinterpret_cast<const uint32_t*>(blob) + h.off_pairs;
-  for (int i = lane; i < h.n_pairs; i += 64) pw[i] = src[i];
+  for (int i = lane; i < h.n_chunks * h.chunk_lanes; i += 64) pw[i] = src[i];
 }
 
 template <int G>
@@ -102,14 +103,14 @@ __device__ __forceinline__ void make_views(const float4* __restrict__ blob, cons
   float4* base = reinterpret_cast<float4*>(lds);
   L.atom = base; L.node_o = base + h.n_atoms; L.node_a = base + h.n_atoms + h.n_nodes;
   L.pairs = reinterpret_cast<const uint32_t*>(base + h.n_atoms + 2 * h.n_nodes);
-  L.A = h.n_atoms; L.N = h.n_nodes; L.P = h.n_pairs; L.T = h.n_torsions; L.H = h.n_heavy; L.rounds = h.jump_rounds;
-  S.stash = lds + lig_floats(Amax, Nmax, Pmax) - kStashSlots * 512;
+  L.A = h.n_atoms; L.N = h.n_nodes; L.C = h.n_chunks; L.T = h.n_torsions; L.H = h.n_heavy; L.rounds = h.jump_rounds;
+  S.stash = lds + lig_floats(Amax, Nmax, Pmax) - kStashSlots * kStashSlotFloats;
   float* g = lds + lig_floats(Amax, Nmax, Pmax) + group * group_floats(Amax, Nmax, G, dense);
   S.frame = reinterpret_cast<float4*>(g);
   S.nacc = reinterpret_cast<float4*>(g + 12 * Nmax);
-  S.xyz = g + 20 * Nmax;
-  S.frc = S.xyz + pad4(3 * Amax);
-  H = S.frc + pad4(3 * Amax);
+  S.xyz = reinterpret_cast<float4*>(g + 20 * Nmax);
+  S.frc = S.xyz + Amax;
+  H = reinterpret_cast<float*>(S.frc + Amax);
 }
 
 template <int G, int MODE>
@@ -166,7 +167,7 @@ __device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const Eva
   set_coords<G>(L, S, tmp, l);
   float rg = gyration_radius<G>(L, S, l);
   float best_e = INFINITY;
-  stash_put(S.stash, kSlotBest, tmp);
+  stash_put<G>(S.stash, kSlotBest, tmp);
   const int n_ent = P.mutation_quirk ? (1 + L.T) : (2 + L.T);
   for (int step = 0; step < depth; ++step) {
     const U4 w = philox(chain, (uint32_t)step, 0, 0, k0, k1);
@@ -180,23 +181,23 @@ __device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const Eva
       if (need && vx * vx + vy * vy + vz * vz < 1.0f) { sx = vx; sy = vy; sz = vz; need = false; }
     }
     Conf cand = tmp;
-    stash_put(S.stash, kSlotTmp, tmp);
+    stash_put<G>(S.stash, kSlotTmp, tmp);
     if (which == 0) { cand.px += P.amplitude * sx; cand.py += P.amplitude * sy; cand.pz += P.amplitude * sz; }
     else if (which == 1) { if (rg > kEpsFl) { const float f = P.amplitude / rg; cand.q = qincrement(cand.q, f * sx, f * sy, f * sz); } }
     else if (l == which - 1) cand.tor = tor;
     const float ce = local_search<G, MODE>(L, S, H, K, cand, valid, 0.0f, bfgs_steps, l, evals, hist);
     const bool accept = valid && (step == 0 || ce < tmp_e || u_metro < expf((tmp_e - ce) / P.temperature));
-    if (accept) { tmp = cand; tmp_e = ce; } else tmp = stash_get(S.stash, kSlotTmp);
+    if (accept) { tmp = cand; tmp_e = ce; } else tmp = stash_get<G>(S.stash, kSlotTmp);
     const bool second = accept && tmp_e < best_e;
     const float e2 = local_search<G, MODE>(L, S, H, K, tmp, second, tmp_e, bfgs_steps, l, evals, hist);
-    if (second) { tmp_e = e2; if (tmp_e < best_e) { stash_put(S.stash, kSlotBest, tmp); best_e = tmp_e; } }
+    if (second) { tmp_e = e2; if (tmp_e < best_e) { stash_put<G>(S.stash, kSlotBest, tmp); best_e = tmp_e; } }
     if (__any(accept)) {
       set_coords<G>(L, S, tmp, l);
       const float r = gyration_radius<G>(L, S, l);
       if (accept) rg = r;
     }
   }
-  const Conf best = stash_get(S.stash, kSlotBest);
+  const Conf best = stash_get<G>(S.stash, kSlotBest);
   set_coords<G>(L, S, best, l);
   if (valid) {
     if (l == 0) {
@@ -210,7 +211,8 @@ __device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const Eva
       if ((meta >> 16) & 1u) {
         const int slot = (meta >> 17) & 255;
         float* o = out_heavy + ((size_t)chain * L.H + slot) * 3;
-        o[0] = best.px + S.xyz[3 * i]; o[1] = best.py + S.xyz[3 * i + 1]; o[2] = best.pz + S.xyz[3 * i + 2];
+        const float4 y = S.xyz[i];
+        o[0] = best.px + y.x; o[1] = best.py + y.y; o[2] = best.pz + y.z;
       }
     }
   }
@@ -270,7 +272,7 @@ __global__ __launch_bounds__(64) void vd_eval_kernel(EvalArgs a) {
   stage_ligand<G>(a.blob, h, lds, lane);
   __syncthreads();
   LigView L; Scratch S; float* H;
-  make_views<G>(a.blob, h, lds, h.n_atoms, h.n_nodes, h.n_pairs, 0, group, L, S, H);
+  make_views<G>(a.blob, h, lds, h.n_atoms, h.n_nodes, h.n_chunks * G, 0, group, L, S, H);
   const int T = h.n_torsions, nc = 7 + T, ng = 6 + T;
   for (long long base = (long long)blockIdx.x * CPW; base < a.n_conf; base += (long long)gridDim.x * CPW) {
     const long long c = base + group;
@@ -306,7 +308,7 @@ hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& 
 }
 
 hipError_t launch_eval(int G, const EvalArgs& a, int blocks, hipStream_t s) {
-  const int A = a.host_header.n_atoms, N = a.host_header.n_nodes, P = a.host_header.n_pairs;
+  const int A = a.host_header.n_atoms, N = a.host_header.n_nodes, P = a.host_header.n_chunks * G;
   const int lds = mc_lds_bytes(G, A, N, P, 0);
   switch (G) {
     case 64: hipLaunchKernelGGL(vd_eval_kernel<64>, dim3(blocks), dim3(64), lds, s, a); break;

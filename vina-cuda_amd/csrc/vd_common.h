@@ -20,7 +20,11 @@
 //             (node | xs << 8 | heavy << 16 | heavy slot << 17).
 // Node words: o = (rel_origin xyz, parent); a = (rel_axis xyz, subtree_end) -- preorder, the
 //             subtree of node k is the node range [k, subtree_end).
-// Pair word:  a | b << 8 | tp << 16.
+// Pair word:  a | b << 8 | tp << 16.  Pairs are stored in chunks of `chunk_lanes` slots (one per
+//             lane of a chain group); within a chunk every atom is the `a` of at most one pair
+//             and the `b` of at most one pair, so the force scatter of a chunk needs no atomics.
+//             Unused slots hold VD_NO_PAIR.
+#define VD_NO_PAIR 0xFFFFFFFFu
 struct vd_lig_header {
   int n_atoms, n_nodes, n_pairs, n_heavy;
   int off_atoms;   // float4 index into the blob
@@ -31,7 +35,8 @@ struct vd_lig_header {
   int n_torsions;
   int search_depth;   // MC steps for this ligand (filled per launch)
   int bfgs_steps;     // local-search iterations for this ligand (filled per launch)
-  int pad[2];
+  int n_chunks;       // pair chunks; the pair area holds n_chunks * chunk_lanes words
+  int chunk_lanes;    // lanes per chain group the chunks were built for
 };
 
 // Uniform-grid description of the 17 maps, plain FP32 layout map[t][z][y][x].

@@ -29,7 +29,7 @@ EXPORTS = (
     "vd_eval", "vd_eval_last_ms", "vd_dock_params_default", "vd_dock_heuristics", "vd_dock",
     "vd_dock_batch", "vd_dock_batch_plain", "vd_batch_results", "vd_batch_best", "vd_cluster",
     "vd_synth_receptor", "vd_synth_ligand", "vd_device_alloc", "vd_device_free", "vd_copy", "vd_random_confs",
-    "vd_last_launches",
+    "vd_last_launches", "vd_ligand_pair_chunks",
 )
 
 
@@ -119,6 +119,7 @@ def load():
         "vd_batch_results": (C.c_int, [vp, C.c_int, fp, fp, fp]),
         "vd_batch_best": (C.c_int, [vp, fp]),
         "vd_last_launches": (C.c_int, [vp, C.c_int, C.POINTER(Launch)]),
+        "vd_ligand_pair_chunks": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32)]),
         "vd_cluster": (C.c_int, [C.c_int, C.c_int, fp, fp, C.c_double, C.c_int, ip]),
         "vd_device_alloc": (C.c_int, [vp, C.c_size_t, C.POINTER(C.c_void_p)]),
         "vd_device_free": (C.c_int, [vp, vp]),
@@ -359,6 +360,14 @@ class Context:
         out = np.zeros(len(self._batch_ids), dtype=np.float32)
         self._check(self.lib.vd_batch_best(self.h, _f(out)))
         return out
+
+    def pair_chunks(self, lid, lanes):
+        """Device pair layout for chain groups of `lanes` lanes: array (chunks, lanes) of pair words."""
+        n = self.lib.vd_ligand_pair_chunks(self.h, lid, lanes, 0, None)
+        self._check(min(n, 0))
+        out = np.zeros(max(n, 1), dtype=np.uint32)
+        self._check(min(self.lib.vd_ligand_pair_chunks(self.h, lid, lanes, n, out.ctypes.data_as(C.POINTER(C.c_uint32))), 0))
+        return out[:n].reshape(-1, lanes)
 
     def last_launches(self):
         """MC kernel launches of the last docking call (list of Launch)."""
