@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--plain-subset", type=int, default=0, help="0 = all ligands")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the run, copy 256 MiB device to device twice (reference dispatch for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); ligands shard across GPUs "
@@ -229,6 +231,12 @@ def main():
                           "ms_by_chains_per_wave": {str(k): round(v, 3) for k, v in rates.items()},
                           "finite_fraction": float(np.isfinite(e_host).mean())}
 
+    if args.copy_calibration:
+        n = 256 << 20
+        src, dst = ctx.alloc(n), ctx.alloc(n)
+        for _ in range(2):
+            ctx.copy_device(dst, src, n)
+        ctx.free(src); ctx.free(dst)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = run_cpu_baseline(vdock, texts, box, xyz, xs, args.chains)

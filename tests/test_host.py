@@ -206,6 +206,11 @@ def test_pair_chunks_cover_every_pair_once_without_conflicts(host_ctx, lanes):
             r = row[row != 0xFFFFFFFF]
             assert len(set((r & 255).tolist())) == len(r)
             assert len(set(((r >> 8) & 255).tolist())) == len(r)
-        # a chunk holds at most `lanes` pairs and each atom in at most two of them
+            if host_ctx.ligand_info(lid).n_atoms <= lanes:     # each atom has a lane: slot = first atom
+                slots = np.nonzero(row != 0xFFFFFFFF)[0]
+                assert np.array_equal(slots, (row[slots] & 255))
+        # a chunk holds at most `lanes` pairs and each atom in at most two of them, so at least
+        # max(ceil(P / lanes), ceil(max degree / 2)) chunks are needed; the edge colouring
+        # (Euler orientation + equitable bipartite colouring) reaches that bound
         deg = np.bincount(np.concatenate([pairs[:, 1], pairs[:, 2]]), minlength=1)
-        assert ch.shape[0] >= max(-(-len(pairs) // lanes), -(-int(deg.max()) // 2))
+        assert ch.shape[0] == max(-(-len(pairs) // lanes), -(-int(deg.max()) // 2))

@@ -239,9 +239,9 @@ vdk::EvalK make_evalk(vd_ctx* c, float v_intra, float v_inter, float slope) {
 struct Packed {
   std::vector<float> blob;
   std::vector<vd_lig_header> headers;
-  int Amax = 0, Nmax = 0, Pmax = 0, Tmax = 0;
+  int Amax = 0, Nmax = 0, Tmax = 0;
 };
-// Ligand records for chain groups of G lanes; Pmax counts pair slots (chunks x G).
+// Ligand records for chain groups of G lanes.
 Packed pack(vd_ctx* c, int G, const std::vector<int>& ids, const std::vector<int>* depth = nullptr,
             const std::vector<int>* bfgs = nullptr) {
   Packed pk;
@@ -252,7 +252,6 @@ Packed pack(vd_ctx* c, int G, const std::vector<int>& ids, const std::vector<int
     if (bfgs) pk.headers.back().bfgs_steps = (*bfgs)[k];
     pk.Amax = std::max(pk.Amax, (int)L.atoms.size());
     pk.Nmax = std::max(pk.Nmax, (int)L.nodes.size());
-    pk.Pmax = std::max(pk.Pmax, pk.headers.back().n_chunks * G);
     pk.Tmax = std::max(pk.Tmax, L.n_torsions());
   }
   return pk;
@@ -292,8 +291,8 @@ void run_mc(vd_ctx* c, int G, int W, const Packed& pk, const std::vector<uint32_
   a.P = P;
   a.out_e = d_e; a.out_conf = d_conf; a.out_heavy = d_heavy; a.out_off = c->offs.p;
   a.slot_evals = c->evals.p;
-  a.Amax = pk.Amax; a.Nmax = pk.Nmax; a.Pmax = pk.Pmax;
-  const int lds = vdl::mc_lds_bytes(G, pk.Amax, pk.Nmax, pk.Pmax, P.local_search == 0);
+  a.Amax = pk.Amax; a.Nmax = pk.Nmax;
+  const int lds = vdl::mc_lds_bytes(G, pk.Amax, pk.Nmax, P.local_search == 0);
   if (lds > 64 * 1024) throw ArgError(VD_ERR_CAPACITY, "ligand too large for the LDS layout");
   const int mode = P.local_search == 0 ? 0 : (P.policy == VD_MC_QVINA2 ? 2 : 1);
   // a tighter register budget only pays when the LDS of this launch lets the extra waves in:
@@ -343,11 +342,15 @@ void run_mc(vd_ctx* c, int G, int W, const Packed& pk, const std::vector<uint32_
 }
 
 struct LigPlan { int id, G, W, depth, bfgs, T, H, chains; };
-LigPlan plan_for(vd_ctx* c, int id, const vd_dock_params* p) {
+// single_launch: the ligand gets a launch of its own (vd_dock, the plain per-ligand loop).  Its
+// only parallelism is its chains, so unless they fill two waves per SIMD at the tuned width, it
+// runs one chain per wavefront, the most wavefronts such a launch can have.
+LigPlan plan_for(vd_ctx* c, int id, const vd_dock_params* p, bool single_launch) {
   const vd::Ligand& L = lig_at(c, id);
   LigPlan pl;
   pl.id = id; pl.T = L.n_torsions(); pl.H = L.n_heavy; pl.chains = p->chains;
   pl.G = pick_group(c, L, p->chains_per_wave);
+  if (single_launch && p->chains_per_wave == 0 && (long long)p->chains * pl.G / 64 < 8LL * std::max(c->n_cu, 1)) pl.G = 64;
   pl.W = p->waves_per_simd ? p->waves_per_simd : pick_waves(c, L, pl.G);
   if (p->local_search != VD_LS_RILC_BFGS) pl.W = 2;
   const int rilc = p->local_search == VD_LS_RILC_BFGS;
@@ -357,37 +360,45 @@ LigPlan plan_for(vd_ctx* c, int id, const vd_dock_params* p) {
   return pl;
 }
 
-// Autotuner: for every size bucket of the batch not tuned yet, dock a representative ligand with
-// each admissible group width at full occupancy (two MC steps) and keep the width with the most
-// evaluations per millisecond.
+// Autotuner: for every size bucket of the batch not tuned yet, a representative ligand docks the
+// same fixed work -- 16 chains per CU, four MC steps -- under each admissible (lanes per chain,
+// waves per SIMD) pair in a persistent launch whose LDS layout is sized for the whole batch, as
+// the batched launch's is; the pair with the shortest kernel time wins.  Every chain follows
+// the same trajectory under every pair up to rounding (the random stream does not depend on the
+// layout), so the work is the same.
 void autotune(vd_ctx* c, const std::vector<int>& ids, const vd_dock_params* p) {
   std::map<int, int> rep;
   for (int id : ids) { const int b = size_bucket(lig_at(c, id)); if (!c->tuned.count(b) && !rep.count(b)) rep[b] = id; }
+  if (rep.empty()) return;
+  int Amax = 0, Nmax = 0;
+  for (int id : ids) {
+    const vd::Ligand& L = lig_at(c, id);
+    Amax = std::max(Amax, (int)L.atoms.size()); Nmax = std::max(Nmax, (int)L.nodes.size());
+  }
+  const int rilc = p->local_search == VD_LS_RILC_BFGS;
+  const int chains = std::max(1, c->n_cu) * 16, depth = 4;
   for (auto& kv : rep) {
     const vd::Ligand& L = lig_at(c, kv.second);
     const int gmin = min_group(L);
-    int best_g = gmin, best_w = 2; double best_rate = -1;
+    const int bfgs = std::max(1, p->bfgs_steps > 0 ? p->bfgs_steps : heur_bfgs((int)L.atoms.size(), rilc));
+    int best_g = 64, best_w = 2; double best_ms = 1e300;
+    c->tune_e.ensure(chains); c->tune_conf.ensure((size_t)chains * (7 + L.n_torsions()));
+    c->tune_heavy.ensure((size_t)chains * L.n_heavy * 3);
     for (int G = gmin; G <= 64; G *= 2)
-      for (int W = 2; W <= (p->local_search == VD_LS_RILC_BFGS ? 4 : 2); ++W) {
+      for (int W = 2; W <= (rilc ? 4 : 2); ++W) {
         vd_dock_params q = *p;
-        q.chains_per_wave = 64 / G;
-        const int rilc = p->local_search == VD_LS_RILC_BFGS;
-        const int bfgs = p->bfgs_steps > 0 ? p->bfgs_steps : heur_bfgs((int)L.atoms.size(), rilc);
+        q.chains = chains; q.chains_per_wave = 64 / G;
         const int cpw = 64 / G;
-        const int blocks = std::max(1, c->n_cu) * 4 * W;
-        q.chains = blocks * cpw;
-        std::vector<int> dv{2}, bv{std::max(bfgs, 1)};
+        std::vector<int> dv{depth}, bv{bfgs};
         Packed pk = pack(c, G, {kv.second}, &dv, &bv);
+        pk.Amax = std::max(pk.Amax, Amax); pk.Nmax = std::max(pk.Nmax, Nmax);
         std::vector<int2> items;
-        for (int c0 = 0; c0 < q.chains; c0 += cpw) items.push_back(make_int2(0, c0));
+        for (int c0 = 0; c0 < chains; c0 += cpw) items.push_back(make_int2(0, c0));
         std::vector<longlong4> off{make_longlong4(0, 0, 0, 0)};
-        c->tune_e.ensure(q.chains); c->tune_conf.ensure((size_t)q.chains * (7 + L.n_torsions()));
-        c->tune_heavy.ensure((size_t)q.chains * L.n_heavy * 3);
         vd_stats st{};
-        run_mc(c, G, W, pk, {(uint32_t)kv.second}, items, off, make_consts(c, &q, 2, bv[0]), c->tune_e.p, c->tune_conf.p,
-               c->tune_heavy.p, false, &st, true);
-        const double rate = st.kernel_ms > 0 ? st.evals / st.kernel_ms : 0;
-        if (rate > best_rate) { best_rate = rate; best_g = G; best_w = W; }
+        run_mc(c, G, W, pk, {(uint32_t)kv.second}, items, off, make_consts(c, &q, depth, bfgs), c->tune_e.p,
+               c->tune_conf.p, c->tune_heavy.p, true, &st, true);
+        if (st.kernel_ms < best_ms) { best_ms = st.kernel_ms; best_g = G; best_w = W; }
       }
     c->tuned[kv.first] = std::make_pair(best_g, best_w);
   }
@@ -729,7 +740,7 @@ int vd_dock(vd_ctx* c, int id, const vd_dock_params* p, float* e, float* conf, f
     if (st) std::memset(st, 0, sizeof(*st));
     if (p->chains_per_wave == 0 || p->waves_per_simd == 0) autotune(c, {id}, p);
     c->launches.clear();
-    LigPlan pl = plan_for(c, id, p);
+    LigPlan pl = plan_for(c, id, p, true);
     prepare_batch(c, {pl});
     std::vector<int> dv{pl.depth}, bv{pl.bfgs};
     Packed pk = pack(c, pl.G, {id}, &dv, &bv);
@@ -754,7 +765,7 @@ static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, 
     if (p->chains_per_wave == 0 || p->waves_per_simd == 0) autotune(c, std::vector<int>(ids, ids + n), p);
     c->launches.clear();
     std::vector<LigPlan> plans;
-    for (int k = 0; k < n; ++k) plans.push_back(plan_for(c, ids[k], p));
+    for (int k = 0; k < n; ++k) plans.push_back(plan_for(c, ids[k], p, !persistent));
     prepare_batch(c, plans);
     if (n == 0) return;
     if (!persistent) {

@@ -66,6 +66,36 @@ template <int G, int SRC> __device__ __forceinline__ float gbcast(float v) {
   else return __shfl(v, SRC, G);
 }
 
+// Segmented inclusive scan of six sums over the lanes of a group, segments = runs of equal
+// `seg` (-1 = no segment): DPP row shifts by 1, 2, 4, 8 inside each row of 16 lanes, then the
+// last lane of a row feeds the next row (row_bcast:15) and lane 31 feeds rows 2 and 3
+// (row_bcast:31), each added only where the segment continues.  Lanes whose source lies outside
+// the row read 0 and segment -1.
+template <int CTRL, int ROWS> __device__ __forceinline__ float dppf(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+template <int CTRL, int ROWS> __device__ __forceinline__ int dppi(int v) {
+  return __builtin_amdgcn_update_dpp(-1, v, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS> __device__ __forceinline__ void seg_step(float (&v)[6], int seg) {
+  const int s = dppi<CTRL, ROWS>(seg);
+  float t[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) t[c] = dppf<CTRL, ROWS>(v[c]);
+  if (s == seg && seg >= 0) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) v[c] += t[c];
+  }
+}
+template <int G> __device__ __forceinline__ void seg_scan(float (&v)[6], int seg) {
+  seg_step<0x111, 0xF>(v, seg);   // row_shr:1
+  seg_step<0x112, 0xF>(v, seg);   // row_shr:2
+  seg_step<0x114, 0xF>(v, seg);   // row_shr:4
+  seg_step<0x118, 0xF>(v, seg);   // row_shr:8
+  if constexpr (G >= 32) seg_step<0x142, 0xA>(v, seg);   // row_bcast:15 into rows 1 and 3
+  if constexpr (G >= 64) seg_step<0x143, 0xC>(v, seg);   // row_bcast:31 into rows 2 and 3
+}
+
 // ----------------------------------------------------------------------------------------
 // Rotations.  Quaternions are (w, x, y, z); products are Hamilton products.
 struct Quat { float w, x, y, z; };
@@ -124,8 +154,9 @@ struct LigView {               // one ligand, staged in LDS for the wave
   const float4* atom;          // A: local xyz + meta (node | xs << 8 | heavy << 16)
   const float4* node_o;        // N: rel_origin xyz + bits (parent+1 | atom begin << 8 | atom end << 16)
   const float4* node_a;        // N: rel_axis xyz + subtree_end
-  const uint32_t* pairs;       // C chunks x G pair words, staged in LDS
+  const uint4* pairs;          // C chunks of G pair words in groups of four, [group][lane][4], global
   int A, N, C, T, H, rounds;
+  int rows;                    // 1: slot l of every chunk holds a pair whose first atom is l
 };
 struct Scratch {               // per chain, LDS
   float4* xyz;                 // A, coordinates relative to the root origin (w unused)
@@ -302,62 +333,101 @@ __device__ float eval_deriv(const LigView& L, const Scratch& S, const EvalK& K, 
   // intramolecular pairs (eval_interacting_pairs_deriv, V11/lib/model.cpp:1241-1259).  Lane l
   // takes slot l of each chunk; within a chunk the first atoms are distinct and so are the second
   // atoms, so the force scatter is a plain read-modify-write of LDS (first atoms, then second
-  // atoms), in chunk order: no atomics, and the same summation order in every run.  Four chunks
-  // per pass: all four table reads are issued before the first is used.
+  // atoms), in chunk order: no atomics, and the same summation order in every run.  When every
+  // atom has its own lane (L.rows), slot l holds pairs whose first atom is l: that side is summed
+  // in registers and only the second atoms are scattered.  Four chunks per pass: one 16-byte read
+  // of the lane's four pair words (global memory, prefetched a pass ahead), then all coordinate
+  // reads, then all four table reads, before the first result is used.
   float e_intra = 0.0f;
-  constexpr int U = 4;
-  for (int c0 = 0; c0 < L.C; c0 += U) {
-    float rx[U], ry[U], rz[U], rem[U];
-    int a[U], b[U];
-    bool ok[U];
-    float4 q[U];
+  {
+    constexpr int U = 4;
+    const int groups = (L.C + U - 1) / U;
+    const uint4* pw = L.pairs + l;
+    const bool rows = L.rows != 0;
+    const float4 own = (rows && l < L.A) ? S.xyz[l] : make_float4(0, 0, 0, 0);
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f;       // rows: force on atom l as a first atom
+    uint4 w_next = groups > 0 ? pw[0] : make_uint4(0, 0, 0, 0);
+    for (int cg = 0; cg < groups; ++cg) {
+      const uint4 w4 = w_next;                     // the next group's words load during this one
+      if (cg + 1 < groups) w_next = pw[(cg + 1) * G];
+      const uint32_t w[U] = {w4.x, w4.y, w4.z, w4.w};
+      bool ok[U];
+      int a[U], b[U], idx[U];
+      float4 pa[U], pb[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int ch = c0 + u;
-      const uint32_t w = ch < L.C ? L.pairs[ch * G + l] : VD_NO_PAIR;
-      ok[u] = w != VD_NO_PAIR;
-      a[u] = ok[u] ? (int)(w & 255) : 0; b[u] = ok[u] ? (int)((w >> 8) & 255) : 0;
-      const int tp = ok[u] ? (int)(w >> 16) : 0;
-      const float4 pa = S.xyz[a[u]], pb = S.xyz[b[u]];
-      rx[u] = pb.x - pa.x; ry[u] = pb.y - pa.y; rz[u] = pb.z - pa.z;
-      const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
-      ok[u] = ok[u] && r2 < VD_CUTOFF_SQR;
-      const float rf = VD_LUT_FACTOR * r2;
-      const int i1 = ok[u] ? (int)rf : 0;
-      rem[u] = rf - (float)i1;
-      q[u] = ok[u] ? K.lutq[tp * VD_LUT_BINS + i1] : make_float4(0, 0, 0, 0);
+      for (int u = 0; u < U; ++u) {
+        ok[u] = w[u] != VD_NO_PAIR;
+        a[u] = ok[u] ? (int)(w[u] & 255) : 0;
+        b[u] = ok[u] ? (int)((w[u] >> 8) & 255) : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) { pb[u] = S.xyz[b[u]]; pa[u] = rows ? own : S.xyz[a[u]]; }
+      float rx[U], ry[U], rz[U], rem[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        rx[u] = pb[u].x - pa[u].x; ry[u] = pb[u].y - pa[u].y; rz[u] = pb[u].z - pa[u].z;
+        const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
+        ok[u] = ok[u] && r2 < VD_CUTOFF_SQR;
+        const float rf = VD_LUT_FACTOR * r2;
+        const int i1 = ok[u] ? (int)rf : 0;
+        rem[u] = rf - (float)i1;
+        idx[u] = ok[u] ? (int)(w[u] >> 16) * VD_LUT_BINS + i1 : 0;
+      }
+      float4 q[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) q[u] = K.lutq[idx[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        float e = q[u].x + rem[u] * (q[u].z - q[u].x);
+        float dor = q[u].y + rem[u] * (q[u].w - q[u].y);
+        if (e > 0.0f) { const float t = (K.v_intra < kEpsFl) ? 0.0f : K.v_intra / (K.v_intra + e); e *= t; dor *= t * t; }
+        e = ok[u] ? e : 0.0f; dor = ok[u] ? dor : 0.0f;
+        e_intra += e;
+        const float fx = dor * rx[u], fy = dor * ry[u], fz = dor * rz[u];
+        if (rows) {
+          ox -= fx; oy -= fy; oz -= fz;
+        } else if (ok[u]) {
+          float4 fa = S.frc[a[u]];
+          fa.x -= fx; fa.y -= fy; fa.z -= fz;
+          S.frc[a[u]] = fa;
+        }
+        if (ok[u]) {
+          float4 fb = S.frc[b[u]];
+          fb.x += fx; fb.y += fy; fb.z += fz;
+          S.frc[b[u]] = fb;
+        }
+      }
     }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (!ok[u]) continue;
-      float e = q[u].x + rem[u] * (q[u].z - q[u].x);
-      float dor = q[u].y + rem[u] * (q[u].w - q[u].y);
-      if (e > 0.0f) { const float t = (K.v_intra < kEpsFl) ? 0.0f : K.v_intra / (K.v_intra + e); e *= t; dor *= t * t; }
-      e_intra += e;
-      const float fx = dor * rx[u], fy = dor * ry[u], fz = dor * rz[u];
-      float4 fa = S.frc[a[u]];
-      fa.x -= fx; fa.y -= fy; fa.z -= fz;
-      S.frc[a[u]] = fa;
-      float4 fb = S.frc[b[u]];
-      fb.x += fx; fb.y += fy; fb.z += fz;
-      S.frc[b[u]] = fb;
+    if (rows && l < L.A) {
+      float4 f = S.frc[l];
+      f.x += ox; f.y += oy; f.z += oz;
+      S.frc[l] = f;
     }
   }
   __syncthreads();
-  // adjoint: per-node force and moment (about the root origin) sums; a node's atoms are the
-  // contiguous range [begin, end), summed in order by the node's lane (untyped atoms carry 0)
-  if (l < L.N) {
-    const uint32_t nb = fbits(L.node_o[l].w);
-    const int begin = (nb >> 8) & 255, end = (nb >> 16) & 255;
-    float f0 = 0, f1 = 0, f2 = 0, m0 = 0, m1 = 0, m2 = 0;
-    for (int i = begin; i < end; ++i) {
+  // adjoint: per-node force and moment (about the root origin) sums.  A node's atoms are one
+  // contiguous range, so the sums are a segmented inclusive scan over the atom lanes (segment =
+  // node; untyped atoms carry 0); the last lane of each segment adds its sums to the node's
+  // record.  Atoms beyond G lanes take further passes, each adding in turn.
+  if (l < L.N) { S.nacc[2 * l] = make_float4(0, 0, 0, 0); S.nacc[2 * l + 1] = make_float4(0, 0, 0, 0); }
+  __syncthreads();
+  for (int i0 = 0; i0 < L.A; i0 += G) {
+    const int i = i0 + l;
+    float v[6] = {0, 0, 0, 0, 0, 0};
+    int node = -1;
+    if (i < L.A) {
       const float4 fr = S.frc[i], p = S.xyz[i];
-      const float fx = fr.x, fy = fr.y, fz = fr.z, x = p.x, y = p.y, z = p.z;
-      f0 += fx; f1 += fy; f2 += fz;
-      m0 += y * fz - z * fy; m1 += z * fx - x * fz; m2 += x * fy - y * fx;
+      node = (int)(fbits(L.atom[i].w) & 255u);
+      v[0] = fr.x; v[1] = fr.y; v[2] = fr.z;
+      v[3] = p.y * fr.z - p.z * fr.y; v[4] = p.z * fr.x - p.x * fr.z; v[5] = p.x * fr.y - p.y * fr.x;
     }
-    S.nacc[2 * l] = make_float4(f0, f1, f2, 0.0f);
-    S.nacc[2 * l + 1] = make_float4(m0, m1, m2, 0.0f);
+    seg_scan<G>(v, node);
+    const int next = __shfl_down(node, 1, G);
+    if (node >= 0 && (l == G - 1 || next != node)) {
+      float4 a = S.nacc[2 * node], b = S.nacc[2 * node + 1];
+      a.x += v[0]; a.y += v[1]; a.z += v[2]; b.x += v[3]; b.y += v[4]; b.z += v[5];
+      S.nacc[2 * node] = a; S.nacc[2 * node + 1] = b;
+    }
   }
   __syncthreads();
   // subtree sums: the subtree of node k is the preorder range [k, subtree_end_k); the torque

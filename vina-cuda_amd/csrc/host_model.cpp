@@ -421,17 +421,128 @@ Ligand parse_ligand_pdbqt(const std::string& text) {
   return L;
 }
 
-const std::vector<uint32_t>& pair_chunks(const Ligand& L, int G) {
-  const int which = G == 16 ? 0 : G == 32 ? 1 : 2;
-  std::vector<uint32_t>& out = L.chunked[which];
-  if (!out.empty() || L.pairs.empty()) return out;
+namespace {
+
+// Chunks from an edge colouring: orient every pair along an Euler tour of the pair graph (a
+// dummy vertex joined to the odd-degree atoms closes it), so that each atom is the first atom of
+// at most ceil(deg/2) pairs and the second atom of at most ceil(deg/2); then colour the
+// bipartite graph (first-atom side, second-atom side) with that many colours by alternating-path
+// swaps (Konig's edge-colouring theorem).  A colour class is a valid chunk; classes larger than
+// G are cut into pieces of G.
+std::vector<std::vector<uint32_t>> chunks_by_colouring(const std::vector<uint32_t>& pairs, int A, int G) {
+  const int E = (int)pairs.size(), V = A + 1;
+  std::vector<int> ea(E), eb(E), deg(V, 0);
+  for (int e = 0; e < E; ++e) { ea[e] = pairs[e] & 255; eb[e] = (pairs[e] >> 8) & 255; ++deg[ea[e]]; ++deg[eb[e]]; }
+  // adjacency incl. dummy edges (ids >= E)
+  std::vector<int> ux(ea), vy(eb);
+  for (int v = 0; v < A; ++v) if (deg[v] & 1) { ux.push_back(A); vy.push_back(v); }
+  const int E2 = (int)ux.size();
+  std::vector<std::vector<int>> adj(V);
+  for (int e = 0; e < E2; ++e) { adj[ux[e]].push_back(e); adj[vy[e]].push_back(e); }
+  std::vector<char> used(E2, 0);
+  std::vector<size_t> next(V, 0);
+  std::vector<int> from(E, -1), to(E, -1);
+  for (int start = 0; start < V; ++start) {
+    // Hierholzer: walk unused edges; every edge is oriented in the direction it is walked
+    std::vector<int> stack{start};
+    while (!stack.empty()) {
+      const int v = stack.back();
+      while (next[v] < adj[v].size() && used[adj[v][next[v]]]) ++next[v];
+      if (next[v] == adj[v].size()) { stack.pop_back(); continue; }
+      const int e = adj[v][next[v]];
+      used[e] = 1;
+      const int w = ux[e] == v ? vy[e] : ux[e];
+      if (e < E) { from[e] = v; to[e] = w; }
+      stack.push_back(w);
+    }
+  }
+  std::vector<int> outd(A, 0), ind(A, 0);
+  for (int e = 0; e < E; ++e) { ++outd[from[e]]; ++ind[to[e]]; }
+  int D = 1;
+  for (int v = 0; v < A; ++v) D = std::max(D, std::max(outd[v], ind[v]));
+  std::vector<int> atL((size_t)A * D, -1), atR((size_t)A * D, -1), col(E, -1);
+  for (int e = 0; e < E; ++e) {
+    const int u = from[e], v = to[e];
+    int a = 0; while (atL[(size_t)u * D + a] >= 0) ++a;
+    int b = 0; while (atR[(size_t)v * D + b] >= 0) ++b;
+    if (atR[(size_t)v * D + a] >= 0) {
+      // swap colours a and b on the alternating path that leaves v by its a-coloured edge
+      std::vector<int> path;
+      int x = v; bool right = true; int c = a;
+      while (true) {
+        const int f = right ? atR[(size_t)x * D + c] : atL[(size_t)x * D + c];
+        if (f < 0) break;
+        path.push_back(f);
+        x = right ? from[f] : to[f];
+        right = !right; c = (c == a) ? b : a;
+      }
+      for (int f : path) { atL[(size_t)from[f] * D + col[f]] = -1; atR[(size_t)to[f] * D + col[f]] = -1; }
+      for (int f : path) {
+        col[f] = col[f] == a ? b : a;
+        atL[(size_t)from[f] * D + col[f]] = f; atR[(size_t)to[f] * D + col[f]] = f;
+      }
+    }
+    col[e] = a; atL[(size_t)u * D + a] = e; atR[(size_t)v * D + a] = e;
+  }
+  // Equalise: with K = max(D, ceil(E / G)) colours, move edges from the largest class to the
+  // smallest along alternating paths of the two classes that start and end with an edge of the
+  // larger one (a bipartite graph has an equitable K-edge-colouring for every K >= D).
+  const int K = std::max(D, (E + G - 1) / G);
+  {
+    std::vector<int> nL((size_t)A * K, -1), nR((size_t)A * K, -1);
+    for (int e = 0; e < E; ++e) { nL[(size_t)from[e] * K + col[e]] = e; nR[(size_t)to[e] * K + col[e]] = e; }
+    atL.swap(nL); atR.swap(nR);
+    std::vector<int> size(K, 0);
+    for (int e = 0; e < E; ++e) ++size[col[e]];
+    while (true) {
+      const int c1 = (int)(std::max_element(size.begin(), size.end()) - size.begin());
+      const int c2 = (int)(std::min_element(size.begin(), size.end()) - size.begin());
+      if (size[c1] <= G || size[c1] - size[c2] <= 1) break;
+      bool moved = false;
+      for (int e0 = 0; e0 < E && !moved; ++e0) {
+        if (col[e0] != c1) continue;
+        // a path end: the left vertex of e0 has no c2 edge (walk from the left side)
+        if (atL[(size_t)from[e0] * K + c2] >= 0) continue;
+        std::vector<int> path{e0};
+        int x = to[e0]; bool right = true; int c = c2;
+        while (true) {
+          const int f = right ? atR[(size_t)x * K + c] : atL[(size_t)x * K + c];
+          if (f < 0) break;
+          path.push_back(f);
+          x = right ? from[f] : to[f];
+          right = !right; c = (c == c1) ? c2 : c1;
+        }
+        if (col[path.back()] != c1) continue;      // balanced path: swapping gains nothing
+        for (int f : path) { atL[(size_t)from[f] * K + col[f]] = -1; atR[(size_t)to[f] * K + col[f]] = -1; }
+        for (int f : path) {
+          col[f] = col[f] == c1 ? c2 : c1;
+          atL[(size_t)from[f] * K + col[f]] = f; atR[(size_t)to[f] * K + col[f]] = f;
+        }
+        --size[c1]; ++size[c2]; moved = true;
+      }
+      if (!moved) break;
+    }
+  }
+  std::vector<std::vector<uint32_t>> out;
+  for (int c = 0; c < K; ++c) {
+    std::vector<uint32_t> cls;
+    for (int e = 0; e < E; ++e)
+      if (col[e] == c) cls.push_back((uint32_t)from[e] | ((uint32_t)to[e] << 8) | (pairs[e] & 0xFFFF0000u));
+    for (size_t k = 0; k < cls.size(); k += G)
+      out.emplace_back(cls.begin() + k, cls.begin() + std::min(cls.size(), k + (size_t)G));
+  }
+  return out;
+}
+
+// Greedy first fit over the pairs in list order.
+std::vector<std::vector<uint32_t>> chunks_first_fit(const std::vector<uint32_t>& pairs, int G) {
   struct Chunk { uint64_t as[2] = {0, 0}, bs[2] = {0, 0}; int n = 0; };
   auto has = [](const uint64_t* m, int i) { return (m[i >> 6] >> (i & 63)) & 1u; };
   auto set = [](uint64_t* m, int i) { m[i >> 6] |= 1ull << (i & 63); };
   std::vector<Chunk> chunks;
   std::vector<std::vector<uint32_t>> slots;
   size_t first_open = 0;
-  for (uint32_t p : L.pairs) {
+  for (uint32_t p : pairs) {
     const int a = p & 255, b = (p >> 8) & 255;
     const uint32_t tp = p >> 16;
     bool placed = false;
@@ -454,9 +565,24 @@ const std::vector<uint32_t>& pair_chunks(const Ligand& L, int G) {
     }
     while (first_open < chunks.size() && chunks[first_open].n >= G) ++first_open;
   }
-  out.assign(chunks.size() * (size_t)G, VD_NO_PAIR);
-  for (size_t c = 0; c < slots.size(); ++c)
-    for (size_t k = 0; k < slots[c].size(); ++k) out[c * G + k] = slots[c][k];
+  return slots;
+}
+
+}  // namespace
+
+const std::vector<uint32_t>& pair_chunks(const Ligand& L, int G) {
+  const int which = G == 16 ? 0 : G == 32 ? 1 : 2;
+  std::vector<uint32_t>& out = L.chunked[which];
+  if (!out.empty() || L.pairs.empty()) return out;
+  std::vector<std::vector<uint32_t>> a = chunks_by_colouring(L.pairs, (int)L.atoms.size(), G);
+  std::vector<std::vector<uint32_t>> b = chunks_first_fit(L.pairs, G);
+  const std::vector<std::vector<uint32_t>>& best = a.size() <= b.size() ? a : b;
+  // when every atom has a lane of its own, a pair sits in the slot of its first atom: the lane
+  // of atom i then holds every pair whose first atom is i and keeps that side in registers
+  const bool rows = (int)L.atoms.size() <= G;
+  out.assign(best.size() * (size_t)G, VD_NO_PAIR);
+  for (size_t c = 0; c < best.size(); ++c)
+    for (size_t k = 0; k < best[c].size(); ++k) out[c * G + (rows ? (best[c][k] & 255) : k)] = best[c][k];
   return out;
 }
 
@@ -488,12 +614,21 @@ vd_lig_header pack_ligand(const Ligand& L, std::vector<float>& blob, int G) {
     blob.push_back((float)nd.rel_axis[0]); blob.push_back((float)nd.rel_axis[1]); blob.push_back((float)nd.rel_axis[2]);
     blob.push_back(as_f((uint32_t)nd.subtree_end));
   }
-  // device pair order: conflict-free chunks of G slots (only the order of the energy sum changes)
+  // device pair order: conflict-free chunks of G slots (only the order of the energy sum
+  // changes), grouped by four chunks with each lane's four words adjacent ([group][lane][4], one
+  // 16-byte read per lane and group); the chunk count is padded to a multiple of four
   const std::vector<uint32_t>& pairs = pair_chunks(L, G);
   h.chunk_lanes = G;
   h.n_chunks = (int)(pairs.size() / G);
+  h.pair_rows = (int)L.atoms.size() <= G ? 1 : 0;
+  const int groups = (h.n_chunks + 3) / 4;
   h.off_pairs = (int)blob.size();
-  for (uint32_t p : pairs) blob.push_back(as_f(p));
+  for (int cg = 0; cg < groups; ++cg)
+    for (int l = 0; l < G; ++l)
+      for (int u = 0; u < 4; ++u) {
+        const int ch = cg * 4 + u;
+        blob.push_back(as_f(ch < h.n_chunks ? pairs[(size_t)ch * G + l] : VD_NO_PAIR));
+      }
   pad16();
   return h;
 }

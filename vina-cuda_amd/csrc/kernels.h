@@ -24,7 +24,7 @@ struct McArgs {
   float* out_e; float* out_conf; float* out_heavy;
   const longlong4* out_off;         // per ligand slot: (e, conf, heavy) element offsets
   unsigned long long* slot_evals;    // per ligand slot: evaluations performed (zeroed by the caller)
-  int Amax, Nmax, Pmax;
+  int Amax, Nmax;
   float* hist;                      // QuickVina2 visited histories (nullptr: Vina policy),
   long long hist_stride;            // floats per chain region; region = block * chains/wave + group
 };
@@ -39,7 +39,7 @@ struct EvalArgs {
 };
 
 namespace vdl {
-int mc_lds_bytes(int G, int Amax, int Nmax, int Pmax, int dense);
+int mc_lds_bytes(int G, int Amax, int Nmax, int dense);
 int max_resident_mc_blocks(int G, int W, int mode, int lds_bytes);
 hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& gc, const float4* lut_rows, float* maps,
                        const vd_grid_desc& gd, hipStream_t s);

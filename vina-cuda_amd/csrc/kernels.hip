@@ -78,11 +78,10 @@ __global__ __launch_bounds__(256) void vd_grid_kernel(const float4* __restrict__
 }
 
 // ------------------------------------------------------------------------------------------
-// LDS layout of one wave: ligand (atoms 4A, node_o 4N, node_a 4N floats, P pair slots), the
-// conformation stash, then per chain group frame 12N, nacc 8N, xyz 4A, frc 4A, and for dense
-// BFGS H G*G + 2G.
-__host__ __device__ inline int pad4(int n) { return (n + 3) & ~3; }
-__host__ __device__ inline int lig_floats(int A, int N, int P) { return 4 * A + 8 * N + pad4(P) + kStashSlots * kStashSlotFloats; }
+// LDS layout of one wave: ligand (atoms 4A, node_o 4N, node_a 4N floats), the conformation
+// stash, then per chain group frame 12N, nacc 8N, xyz 4A, frc 4A, and for dense BFGS H G*G + 2G.
+// The pair words stay in global memory (read once per evaluation, coalesced, L2-resident).
+__host__ __device__ inline int lig_floats(int A, int N) { return 4 * A + 8 * N + kStashSlots * kStashSlotFloats; }
 __host__ __device__ inline int group_floats(int A, int N, int G, int dense) {
   return 12 * N + 8 * N + 8 * A + (dense ? G * G + 2 * G : 0);
 }
@@ -92,20 +91,18 @@ __device__ __forceinline__ void stage_ligand(const float4* __restrict__ blob, co
   float4* dst = reinterpret_cast<float4*>(lds);
   for (int i = lane; i < h.n_atoms; i += 64) dst[i] = blob[h.off_atoms + i];
   for (int i = lane; i < 2 * h.n_nodes; i += 64) dst[h.n_atoms + i] = blob[h.off_nodes + i];
-  uint32_t* pw = reinterpret_cast<uint32_t*>(dst + h.n_atoms + 2 * h.n_nodes);
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(blob) + h.off_pairs;
-  for (int i = lane; i < h.n_chunks * h.chunk_lanes; i += 64) pw[i] = src[i];
 }
 
 template <int G>
 __device__ __forceinline__ void make_views(const float4* __restrict__ blob, const vd_lig_header& h, float* lds, int Amax,
-                                           int Nmax, int Pmax, int dense, int group, LigView& L, Scratch& S, float*& H) {
+                                           int Nmax, int dense, int group, LigView& L, Scratch& S, float*& H) {
   float4* base = reinterpret_cast<float4*>(lds);
   L.atom = base; L.node_o = base + h.n_atoms; L.node_a = base + h.n_atoms + h.n_nodes;
-  L.pairs = reinterpret_cast<const uint32_t*>(base + h.n_atoms + 2 * h.n_nodes);
+  L.pairs = reinterpret_cast<const uint4*>(reinterpret_cast<const uint32_t*>(blob) + h.off_pairs);
   L.A = h.n_atoms; L.N = h.n_nodes; L.C = h.n_chunks; L.T = h.n_torsions; L.H = h.n_heavy; L.rounds = h.jump_rounds;
-  S.stash = lds + lig_floats(Amax, Nmax, Pmax) - kStashSlots * kStashSlotFloats;
-  float* g = lds + lig_floats(Amax, Nmax, Pmax) + group * group_floats(Amax, Nmax, G, dense);
+  L.rows = h.pair_rows;
+  S.stash = lds + lig_floats(Amax, Nmax) - kStashSlots * kStashSlotFloats;
+  float* g = lds + lig_floats(Amax, Nmax) + group * group_floats(Amax, Nmax, G, dense);
   S.frame = reinterpret_cast<float4*>(g);
   S.nacc = reinterpret_cast<float4*>(g + 12 * Nmax);
   S.xyz = reinterpret_cast<float4*>(g + 20 * Nmax);
@@ -246,7 +243,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
       cur = wi.x;
     }
     LigView L; Scratch S; float* H;
-    make_views<G>(a.blob, h, lds, a.Amax, a.Nmax, a.Pmax, MODE == 0, group, L, S, H);
+    make_views<G>(a.blob, h, lds, a.Amax, a.Nmax, MODE == 0, group, L, S, H);
     const uint32_t chain = (uint32_t)(wi.y + group);
     const bool valid = (int)chain < a.P.chains;
     int evals = 0;
@@ -272,7 +269,7 @@ __global__ __launch_bounds__(64) void vd_eval_kernel(EvalArgs a) {
   stage_ligand<G>(a.blob, h, lds, lane);
   __syncthreads();
   LigView L; Scratch S; float* H;
-  make_views<G>(a.blob, h, lds, h.n_atoms, h.n_nodes, h.n_chunks * G, 0, group, L, S, H);
+  make_views<G>(a.blob, h, lds, h.n_atoms, h.n_nodes, 0, group, L, S, H);
   const int T = h.n_torsions, nc = 7 + T, ng = 6 + T;
   for (long long base = (long long)blockIdx.x * CPW; base < a.n_conf; base += (long long)gridDim.x * CPW) {
     const long long c = base + group;
@@ -296,8 +293,8 @@ __global__ __launch_bounds__(64) void vd_eval_kernel(EvalArgs a) {
 
 namespace vdl {
 
-int mc_lds_bytes(int G, int Amax, int Nmax, int Pmax, int dense) {
-  return 4 * (lig_floats(Amax, Nmax, Pmax) + (64 / G) * group_floats(Amax, Nmax, G, dense));
+int mc_lds_bytes(int G, int Amax, int Nmax, int dense) {
+  return 4 * (lig_floats(Amax, Nmax) + (64 / G) * group_floats(Amax, Nmax, G, dense));
 }
 
 hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& gc, const float4* lut_rows, float* maps,
@@ -308,8 +305,8 @@ hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& 
 }
 
 hipError_t launch_eval(int G, const EvalArgs& a, int blocks, hipStream_t s) {
-  const int A = a.host_header.n_atoms, N = a.host_header.n_nodes, P = a.host_header.n_chunks * G;
-  const int lds = mc_lds_bytes(G, A, N, P, 0);
+  const int A = a.host_header.n_atoms, N = a.host_header.n_nodes;
+  const int lds = mc_lds_bytes(G, A, N, 0);
   switch (G) {
     case 64: hipLaunchKernelGGL(vd_eval_kernel<64>, dim3(blocks), dim3(64), lds, s, a); break;
     case 32: hipLaunchKernelGGL(vd_eval_kernel<32>, dim3(blocks), dim3(64), lds, s, a); break;
@@ -331,7 +328,7 @@ static hipError_t launch_mc_mode(int G, const McArgs& a, int blocks, int lds, hi
 }
 
 hipError_t launch_mc(int G, int W, const McArgs& a, int blocks, hipStream_t s) {
-  const int lds = mc_lds_bytes(G, a.Amax, a.Nmax, a.Pmax, a.P.local_search == 0);
+  const int lds = mc_lds_bytes(G, a.Amax, a.Nmax, a.P.local_search == 0);
   if (a.P.local_search == 0) return launch_mc_mode<0, 2>(G, a, blocks, lds, s);
   if (a.P.policy == 1) {
     if (!a.hist) return hipErrorInvalidValue;

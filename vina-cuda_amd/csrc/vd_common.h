@@ -23,7 +23,9 @@
 // Pair word:  a | b << 8 | tp << 16.  Pairs are stored in chunks of `chunk_lanes` slots (one per
 //             lane of a chain group); within a chunk every atom is the `a` of at most one pair
 //             and the `b` of at most one pair, so the force scatter of a chunk needs no atomics.
-//             Unused slots hold VD_NO_PAIR.
+//             Unused slots hold VD_NO_PAIR.  When the ligand has at most chunk_lanes atoms
+//             (pair_rows = 1), the pair in slot l has first atom l.  The area is laid out in
+//             groups of four chunks, [group][lane][4].
 #define VD_NO_PAIR 0xFFFFFFFFu
 struct vd_lig_header {
   int n_atoms, n_nodes, n_pairs, n_heavy;
@@ -37,6 +39,8 @@ struct vd_lig_header {
   int bfgs_steps;     // local-search iterations for this ligand (filled per launch)
   int n_chunks;       // pair chunks; the pair area holds n_chunks * chunk_lanes words
   int chunk_lanes;    // lanes per chain group the chunks were built for
+  int pair_rows;      // 1: slot l of every chunk holds a pair whose first atom is l
+  int reserved;
 };
 
 // Uniform-grid description of the 17 maps, plain FP32 layout map[t][z][y][x].

@@ -316,6 +316,10 @@ class Context:
     def free(self, ptr):
         self._check(self.lib.vd_device_free(self.h, ptr))
 
+    def copy_device(self, dst, src, nbytes):
+        """Device-to-device copy of nbytes (synchronous)."""
+        self._check(self.lib.vd_copy(self.h, dst, src, nbytes, 2))
+
     def to_host(self, ptr, shape, dtype=np.float32):
         out = np.empty(shape, dtype=dtype)
         self._check(self.lib.vd_copy(self.h, out.ctypes.data, ptr, out.nbytes, 1))
