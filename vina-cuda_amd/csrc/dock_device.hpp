@@ -294,25 +294,29 @@ __device__ float gyration_radius(const LigView& L, const Scratch& S, int l) {
 
 // Energy and gradient of one conformation per group (model::eval_deriv,
 // V11/lib/model.cpp:1324-1334; device form quasi_newton.cuh:1294-1339).
-// Returns e (group-uniform); g = this lane's gradient component.
+// Returns e (group-uniform); g = this lane's gradient component.  rg (when given) receives the
+// gyration radius of the evaluated conformation, as gyration_radius computes it.
 template <int G>
-__device__ float eval_deriv(const LigView& L, const Scratch& S, const EvalK& K, const Conf& c, int l, float& g) {
+__device__ float eval_deriv(const LigView& L, const Scratch& S, const EvalK& K, const Conf& c, int l, float& g,
+                            float* rg = nullptr) {
   float3 ax, org;
   kinematics<G>(L, S, c, l, ax, org);
   __syncthreads();
   // intermolecular term: trilinear maps (cache::eval_deriv, V11/lib/cache.cpp:73-88)
   const float3 prel = make_float3(c.px - K.gd.init[0], c.py - K.gd.init[1], c.pz - K.gd.init[2]);
-  float e_inter = 0.0f;
+  float e_inter = 0.0f, racc = 0.0f;
   // two atoms per lane in flight: both sets of map reads are issued before either is consumed
   for (int i0 = l; i0 < L.A; i0 += 2 * G) {
     float3 d[2], y[2];
     float ev[2];
+    bool heavy[2];
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       const int i = i0 + u * G;
-      d[u] = make_float3(0, 0, 0); ev[u] = 0.0f; y[u] = make_float3(0, 0, 0);
+      d[u] = make_float3(0, 0, 0); ev[u] = 0.0f; y[u] = make_float3(0, 0, 0); heavy[u] = false;
       if (i < L.A) {
         uint32_t meta; y[u] = atom_position(L, S, i, meta);
+        heavy[u] = (meta >> 16) & 1u;
         const int xs = (meta >> 8) & 255;
         if (xs < VD_XS_SIZE)
           ev[u] = map_eval(K.maps + (size_t)xs * K.gd.plane, K.gd, make_float3(prel.x + y[u].x, prel.y + y[u].y, prel.z + y[u].z),
@@ -326,9 +330,11 @@ __device__ float eval_deriv(const LigView& L, const Scratch& S, const EvalK& K, 
         S.xyz[i] = make_float4(y[u].x, y[u].y, y[u].z, 0.0f);
         S.frc[i] = make_float4(d[u].x, d[u].y, d[u].z, 0.0f);
         e_inter += ev[u];
+        if (heavy[u]) racc += y[u].x * y[u].x + y[u].y * y[u].y + y[u].z * y[u].z;
       }
     }
   }
+  if (rg) { const float acc = gsum<G>(racc); *rg = L.H > 0 ? sqrtf(acc / (float)L.H) : 0.0f; }
   __syncthreads();
   // intramolecular pairs (eval_interacting_pairs_deriv, V11/lib/model.cpp:1241-1259).  Lane l
   // takes slot l of each chunk; within a chunk the first atoms are distinct and so are the second

@@ -215,6 +215,170 @@ __device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const Eva
   }
 }
 
+// Writes a chain's best energy, conformation and heavy-atom coordinates (call after
+// set_coords of the best conformation).
+template <int G>
+__device__ void write_chain(const LigView& L, const Scratch& S, const Conf& best, float best_e, uint32_t chain, int l,
+                            float* out_e, float* out_conf, float* out_heavy) {
+  if (l == 0) {
+    out_e[chain] = best_e;
+    float* c = out_conf + (size_t)chain * (7 + L.T);
+    c[0] = best.px; c[1] = best.py; c[2] = best.pz; c[3] = best.q.w; c[4] = best.q.x; c[5] = best.q.y; c[6] = best.q.z;
+  }
+  if (l >= 1 && l <= L.T) out_conf[(size_t)chain * (7 + L.T) + 6 + l] = best.tor;
+  for (int i = l; i < L.A; i += G) {
+    const uint32_t meta = fbits(L.atom[i].w);
+    if ((meta >> 16) & 1u) {
+      const int slot = (meta >> 17) & 255;
+      float* o = out_heavy + ((size_t)chain * L.H + slot) * 3;
+      const float4 y = S.xyz[i];
+      o[0] = best.px + y.x; o[1] = best.py + y.y; o[2] = best.pz + y.z;
+    }
+  }
+}
+
+// The same MC schedule and RILC-BFGS arithmetic as mc_chain + rilc_bfgs, run as a flat state
+// machine: every pass evaluates one conformation per chain group and each group then advances
+// its own state (start of a local search, line-search trial, end of a BFGS iteration, end of a
+// local search, Metropolis step, next mutation) as far as it can without another evaluation.
+// Chains of a wavefront thus never wait for each other's line searches or local searches; they
+// only wait at the end of the chain.  The gyration radius of each evaluated conformation comes
+// from the evaluation itself and travels with the conformation.
+// QV: QuickVina2 gate at the start of every local search (see History).
+template <int G, bool QV>
+__device__ void mc_chain_flat(const LigView& L, const Scratch& S, const EvalK& K, const vd_mc_consts& P, int depth,
+                              int max_steps, uint32_t stream, uint32_t chain, bool valid, int l, float* out_e,
+                              float* out_conf, float* out_heavy, int& evals, History* hist) {
+  const uint32_t k0 = P.seed, k1 = stream;
+  const int n_ent = P.mutation_quirk ? (1 + L.T) : (2 + L.T);
+  enum { kMutate = 0, kStart = 1, kTrial = 2, kDone = 3 };
+  // MC state; the current conformation lives in the stash (kSlotTmp)
+  {
+    const Conf c0 = initial_conf<G>(P, stream, chain, l, L.T);
+    stash_put<G>(S.stash, kSlotTmp, c0);
+    stash_put<G>(S.stash, kSlotBest, c0);
+    set_coords<G>(L, S, c0, l);
+  }
+  float rg = gyration_radius<G>(L, S, l);
+  float tmp_e = 0.0f, best_e = INFINITY, u_metro = 0.0f;
+  int step = 0, phase = (valid && depth > 0) ? kMutate : kDone, ls_kind = 1;
+  // local-search state
+  Conf x = stash_get<G>(S.stash, kSlotTmp);
+  float x_rg = rg, g = 0.0f, d = 0.0f, gp = 0.0f;
+  float fx = 0.0f, fxp = 0.0f, fx_orig = 0.0f, orig_rg = 0.0f, prev_rg = 0.0f;
+  float stp = 0.0f, finit = 0.0f, dgtest = 0.0f, dstest = 0.0f, mu = 0.0f, nu = 0.0f;
+  bool brackt = false;
+  int count = 0, k = 0;
+  while (__any(phase != kDone)) {
+    // (1) MC step start: draws and mutation (mutate.cpp:62-127); the candidate is evaluated next
+    const bool mut = phase == kMutate;
+    if (__any(mut)) {
+      const U4 w = philox(chain, (uint32_t)step, 0, 0, k0, k1);
+      const int which = min((int)(u01(w.x) * (float)n_ent), n_ent - 1);
+      const float tor = -kPi + 2.0f * kPi * u01(w.z);
+      float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+      bool need = mut && which <= 1;
+      for (uint32_t a = 0; a < 64 && __any(need); ++a) {   // P(64 rejections) ~ 1e-20
+        const U4 v = philox(chain, (uint32_t)step, 1, a, k0, k1);
+        const float vx = 2.0f * u01(v.x) - 1.0f, vy = 2.0f * u01(v.y) - 1.0f, vz = 2.0f * u01(v.z) - 1.0f;
+        if (need && vx * vx + vy * vy + vz * vz < 1.0f) { sx = vx; sy = vy; sz = vz; need = false; }
+      }
+      if (mut) {
+        Conf cand = stash_get<G>(S.stash, kSlotTmp);
+        if (which == 0) { cand.px += P.amplitude * sx; cand.py += P.amplitude * sy; cand.pz += P.amplitude * sz; }
+        else if (which == 1) { if (rg > kEpsFl) { const float f = P.amplitude / rg; cand.q = qincrement(cand.q, f * sx, f * sy, f * sz); } }
+        else if (l == which - 1) cand.tor = tor;
+        x = cand; u_metro = u01(w.y); ls_kind = 1; phase = kStart;
+      }
+    }
+    // (2) one evaluation per group: the search start point, or the line-search trial point
+    const bool trial = phase == kTrial, start = phase == kStart;
+    const Conf xt = increment<G>(stash_get<G>(S.stash, kSlotPrev), d, stp, l, L.T);
+    const Conf xe = csel(trial, xt, x);
+    float ge, rge;
+    const float fe = eval_deriv<G>(L, S, K, xe, l, ge, &rge);
+    // (3) consume it
+    bool ls_done = false, iter_end = false, begin = false;
+    int result = -1;
+    if (start || trial) { ++evals; x = xe; fx = fe; g = ge; x_rg = rge; }
+    if (trial) ++count;
+    if (start) { fx_orig = fe; orig_rg = rge; stash_put<G>(S.stash, kSlotOrig, x); }
+    bool search = start;
+    if constexpr (QV) {
+      if (__any(start)) {
+        const bool interesting = hist_interesting<G>(*hist, x, fx, g, l, L.T, start);
+        hist_add<G>(*hist, x, fx, g, l, L.T, start && interesting);
+        search = start && interesting;
+      }
+    }
+    if (__any(start)) {
+      const float gg = gsum<G>(g * g);
+      if (start) {
+        d = -g;
+        if (search && sqrtf(gg) >= 1e-5f) { stp = 1.0f / sqrtf(gg); k = 1; begin = true; }
+        else ls_done = true;
+      }
+    }
+    if (__any(trial)) {   // Lewis-Overton line search (quasi_newton.cuh:2172-2389)
+      const float gtd = gsum<G>(g * d);
+      if (trial) {
+        bool done = false;
+        if (fx > finit + stp * dgtest) { nu = stp; brackt = true; }
+        else if (gtd < dstest) { mu = stp; }
+        else { result = count; done = true; }
+        if (!done && count >= 10) { result = (fx > finit) ? -1 : count; done = true; }
+        if (!done) stp = brackt ? 0.5f * (mu + nu) : 2.0f * stp;
+        iter_end = done;
+      }
+    }
+    if (__any(iter_end)) {   // end of a BFGS iteration, direction update (quasi_newton.cuh:2342-2350)
+      const float gnorm = sqrtf(gsum<G>(g * g));
+      bool update = false;
+      if (iter_end) {
+        if (result < 0) { x = stash_get<G>(S.stash, kSlotPrev); g = gp; fx = fxp; x_rg = prev_rg; ls_done = true; }
+        else if (!(gnorm >= 1e-5f)) ls_done = true;
+        else if ((max_steps != 0 && max_steps <= k) || k >= 1000) ls_done = true;   // 1000: hard bound
+        else { ++k; update = true; }
+      }
+      const float sv = stp * d, y = g - gp;
+      const float ys = gsum<G>(y * sv), yy = gsum<G>(y * y), ss = gsum<G>(sv * sv), gpgp = gsum<G>(gp * gp);
+      const float nd = -g;
+      const float alpha = gsum<G>(sv * nd) / ys;
+      const float nd2 = (nd + alpha * y) * (ys / yy);
+      const float beta = gsum<G>(y * nd2) / ys;
+      const float nd3 = nd2 + (alpha - beta) * sv;
+      if (update) { d = (ys > ss * sqrtf(gpgp) * 1.0e-6f) ? nd3 : nd; stp = 1.0f; begin = true; }
+    }
+    if (__any(begin)) {      // a BFGS iteration starts: its line search begins at x along d
+      if (begin) { stash_put<G>(S.stash, kSlotPrev, x); gp = g; fxp = fx; prev_rg = x_rg; }
+      const float dginit = gsum<G>(g * d);
+      if (begin) {
+        if (0.0f < dginit) ls_done = true;       // no descent: the iteration fails, x stays
+        else {
+          finit = fx; dgtest = 1e-4f * dginit; dstest = 0.1f * dginit;
+          mu = 0.0f; nu = 1.0e20f; brackt = false; count = 0; phase = kTrial;
+        }
+      }
+    }
+    if (ls_done) {           // end of a local search: keep the better of its start and its result
+      if (!(fx <= fx_orig)) { fx = fx_orig; x = stash_get<G>(S.stash, kSlotOrig); x_rg = orig_rg; }
+      bool step_done = true;
+      if (ls_kind == 1) {    // Metropolis (kernel2.cu:364-392)
+        const bool accept = step == 0 || fx < tmp_e || u_metro < expf((tmp_e - fx) / P.temperature);
+        if (accept) { stash_put<G>(S.stash, kSlotTmp, x); tmp_e = fx; rg = x_rg; }
+        if (accept && tmp_e < best_e) { ls_kind = 2; phase = kStart; step_done = false; }   // second search of x
+      } else {
+        stash_put<G>(S.stash, kSlotTmp, x); tmp_e = fx; rg = x_rg;
+        if (tmp_e < best_e) { stash_put<G>(S.stash, kSlotBest, x); best_e = tmp_e; }
+      }
+      if (step_done) { ++step; phase = step < depth ? kMutate : kDone; }
+    }
+  }
+  const Conf best = stash_get<G>(S.stash, kSlotBest);
+  set_coords<G>(L, S, best, l);
+  if (valid) write_chain<G>(L, S, best, best_e, chain, l, out_e, out_conf, out_heavy);
+}
+
 // W = minimum waves per SIMD the register allocation must allow (2 = unconstrained); a tuning knob:
 // more resident waves hide more latency, at the price of register spills.
 template <int G, int MODE, int W>
@@ -249,8 +413,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
     int evals = 0;
     History hist;
     if constexpr (MODE == 2) hist.init(a.hist + ((long long)blockIdx.x * CPW + group) * a.hist_stride, h.n_torsions);
-    mc_chain<G, MODE>(L, S, H, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l, a.out_e + a.out_off[wi.x].x,
-                a.out_conf + a.out_off[wi.x].y, a.out_heavy + a.out_off[wi.x].z, evals, &hist);
+    if constexpr (MODE == 0)
+      mc_chain<G, MODE>(L, S, H, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l,
+                        a.out_e + a.out_off[wi.x].x, a.out_conf + a.out_off[wi.x].y, a.out_heavy + a.out_off[wi.x].z,
+                        evals, &hist);
+    else
+      mc_chain_flat<G, MODE == 2>(L, S, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l,
+                                  a.out_e + a.out_off[wi.x].x, a.out_conf + a.out_off[wi.x].y,
+                                  a.out_heavy + a.out_off[wi.x].z, evals, &hist);
     if (a.slot_evals) {
       int sum = 0;
       for (int gi = 0; gi < CPW; ++gi) sum += __shfl(evals, gi * G);
