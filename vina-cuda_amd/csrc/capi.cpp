@@ -361,13 +361,14 @@ LigPlan plan_for(vd_ctx* c, int id, const vd_dock_params* p, bool single_launch)
 }
 
 // Autotuner: for every size bucket of the batch not tuned yet, a representative ligand docks the
-// same fixed work -- 16 chains per CU, up to eight MC steps -- under each admissible (lanes per chain,
-// waves per SIMD) pair in a persistent launch whose LDS layout is sized for the whole batch, as
-// the batched launch's is.  Each trial runs twice and keeps its shorter time; among the pairs
-// within 2 % of the shortest, the first in a fixed preference order wins (wider groups first,
-// then 3, 4, 2 waves per SIMD), so that timing noise does not flip the choice from run to run.
-// Every chain follows the same trajectory under every pair up to rounding (the random stream
-// does not depend on the layout), so the work is the same.
+// same fixed work -- 64 chains per CU (enough to fill every wavefront slot at four chains per
+// wavefront and four wavefronts per SIMD), at most six MC steps -- under each admissible
+// (lanes per chain, waves per SIMD) pair, in a persistent launch whose LDS layout is sized for the
+// whole batch as the batched launch's is.  The pairs within 10 % of the fastest run again and
+// keep their shorter time; among those within 2 % of the fastest, the first in a fixed preference
+// order wins (wider groups first, then 3, 4, 2 waves per SIMD), so that timing noise does not flip
+// the choice between runs.  Every chain follows the same trajectory under every pair up to
+// rounding (the random stream does not depend on the layout), so the work is the same.
 void autotune(vd_ctx* c, const std::vector<int>& ids, const vd_dock_params* p) {
   std::map<int, int> rep;
   for (int id : ids) { const int b = size_bucket(lig_at(c, id)); if (!c->tuned.count(b) && !rep.count(b)) rep[b] = id; }
@@ -378,43 +379,43 @@ void autotune(vd_ctx* c, const std::vector<int>& ids, const vd_dock_params* p) {
     Amax = std::max(Amax, (int)L.atoms.size()); Nmax = std::max(Nmax, (int)L.nodes.size());
   }
   const int rilc = p->local_search == VD_LS_RILC_BFGS;
-  const int chains = std::max(1, c->n_cu) * 16;
+  const int chains = std::max(1, c->n_cu) * 64;
   for (auto& kv : rep) {
     const vd::Ligand& L = lig_at(c, kv.second);
     const int gmin = min_group(L);
     const int bfgs = std::max(1, p->bfgs_steps > 0 ? p->bfgs_steps : heur_bfgs((int)L.atoms.size(), rilc));
-    // steps: the ligand's own search depth, at most 8 (chains of one wavefront finish at
-    // different times, so a trial much shorter than the real search flatters wide groups)
-    const int depth = std::min(8, p->search_depth > 0 ? p->search_depth
+    const int depth = std::min(6, p->search_depth > 0 ? p->search_depth
                                                       : heur_depth((int)L.atoms.size(), L.n_torsions(), rilc));
     c->tune_e.ensure(chains); c->tune_conf.ensure((size_t)chains * (7 + L.n_torsions()));
     c->tune_heavy.ensure((size_t)chains * L.n_heavy * 3);
-    std::vector<std::tuple<int, int, double>> trials;   // (G, W, ms) in preference order
+    struct Trial { int G, W; double ms; };
+    std::vector<Trial> trials;   // preference order
     for (int G = 64; G >= gmin; G /= 2)
-      for (int W : {3, 4, 2}) {
-        if (!rilc && W != 2) continue;
-        vd_dock_params q = *p;
-        q.chains = chains; q.chains_per_wave = 64 / G;
-        const int cpw = 64 / G;
-        std::vector<int> dv{depth}, bv{bfgs};
-        Packed pk = pack(c, G, {kv.second}, &dv, &bv);
-        pk.Amax = std::max(pk.Amax, Amax); pk.Nmax = std::max(pk.Nmax, Nmax);
-        std::vector<int2> items;
-        for (int c0 = 0; c0 < chains; c0 += cpw) items.push_back(make_int2(0, c0));
-        std::vector<longlong4> off{make_longlong4(0, 0, 0, 0)};
-        double ms = 1e300;
-        for (int rep = 0; rep < 2; ++rep) {
-          vd_stats st{};
-          run_mc(c, G, W, pk, {(uint32_t)kv.second}, items, off, make_consts(c, &q, depth, bfgs), c->tune_e.p,
-                 c->tune_conf.p, c->tune_heavy.p, true, &st, true);
-          ms = std::min(ms, st.kernel_ms);
-        }
-        trials.emplace_back(G, W, ms);
-      }
-    double best_ms = 1e300;
-    for (auto& t : trials) best_ms = std::min(best_ms, std::get<2>(t));
-    for (auto& t : trials)
-      if (std::get<2>(t) <= 1.02 * best_ms) { c->tuned[kv.first] = std::make_pair(std::get<0>(t), std::get<1>(t)); break; }
+      for (int W : {3, 4, 2})
+        if (rilc || W == 2) trials.push_back({G, W, 1e300});
+    auto run = [&](Trial& t) {
+      vd_dock_params q = *p;
+      q.chains = chains; q.chains_per_wave = 64 / t.G;
+      const int cpw = 64 / t.G;
+      std::vector<int> dv{depth}, bv{bfgs};
+      Packed pk = pack(c, t.G, {kv.second}, &dv, &bv);
+      pk.Amax = std::max(pk.Amax, Amax); pk.Nmax = std::max(pk.Nmax, Nmax);
+      std::vector<int2> items;
+      for (int c0 = 0; c0 < chains; c0 += cpw) items.push_back(make_int2(0, c0));
+      std::vector<longlong4> off{make_longlong4(0, 0, 0, 0)};
+      vd_stats st{};
+      run_mc(c, t.G, t.W, pk, {(uint32_t)kv.second}, items, off, make_consts(c, &q, depth, bfgs), c->tune_e.p,
+             c->tune_conf.p, c->tune_heavy.p, true, &st, true);
+      t.ms = std::min(t.ms, st.kernel_ms);
+    };
+    for (Trial& t : trials) run(t);
+    double best = 1e300;
+    for (const Trial& t : trials) best = std::min(best, t.ms);
+    for (Trial& t : trials) if (t.ms <= 1.10 * best) run(t);
+    best = 1e300;
+    for (const Trial& t : trials) best = std::min(best, t.ms);
+    for (const Trial& t : trials)
+      if (t.ms <= 1.02 * best) { c->tuned[kv.first] = std::make_pair(t.G, t.W); break; }
   }
 }
 

@@ -292,6 +292,139 @@ __device__ float gyration_radius(const LigView& L, const Scratch& S, int l) {
   return cnt > 0.0f ? sqrtf(acc / cnt) : 0.0f;
 }
 
+// One pair term: value and derivative/r from the pair table quad, with the curl cap
+// (V11/lib/curl.h:4-13) above zero.
+__device__ __forceinline__ void pair_term(const float4 q, float rem, float v_intra, float& e, float& dor) {
+  e = q.x + rem * (q.z - q.x);
+  dor = q.y + rem * (q.w - q.y);
+  if (e > 0.0f) { const float t = (v_intra < kEpsFl) ? 0.0f : v_intra / (v_intra + e); e *= t; dor *= t * t; }
+}
+
+// Pair terms when every atom has its own lane (L.rows): slot l of each chunk holds pair (l, b).
+// Lane l keeps the force on atom l in registers: its own pairs give the first-atom side, and the
+// second-atom side arrives by a lane shuffle from the one slot per chunk whose pair has second
+// atom l.  Coordinates of b come from lane b by shuffle as well, so the loop touches LDS only
+// through the shuffles.  Four chunks per pass (one 16-byte read of the lane's pair words,
+// prefetched a pass ahead), all four table reads issued before the first is used.
+template <int G>
+__device__ float pair_terms_rows(const LigView& L, const Scratch& S, const EvalK& K, int l) {
+  constexpr int U = 4;
+  const int groups = (L.C + U - 1) / U;
+  const uint4* pw = L.pairs + l;
+  const float4 own = l < L.A ? S.xyz[l] : make_float4(0, 0, 0, 0);
+  float e_intra = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
+  uint4 w_next = groups > 0 ? pw[0] : make_uint4(0, 0, 0, 0);
+  for (int cg = 0; cg < groups; ++cg) {
+    const uint4 w4 = w_next;
+    if (cg + 1 < groups) w_next = pw[(cg + 1) * G];
+    const uint32_t w[U] = {w4.x, w4.y, w4.z, w4.w};
+    int b[U], src[U], idx[U];
+    bool ok[U];
+    float rx[U], ry[U], rz[U], rem[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t bb = (w[u] >> 8) & 255u, ss = w[u] & 255u;
+      ok[u] = w[u] != VD_NO_PAIR && bb != 255u;
+      b[u] = ok[u] ? (int)bb : l;
+      src[u] = (w[u] != VD_NO_PAIR && ss != 255u) ? (int)ss : l;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      rx[u] = __shfl(own.x, b[u], G) - own.x;
+      ry[u] = __shfl(own.y, b[u], G) - own.y;
+      rz[u] = __shfl(own.z, b[u], G) - own.z;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
+      ok[u] = ok[u] && r2 < VD_CUTOFF_SQR;
+      const float rf = VD_LUT_FACTOR * r2;
+      const int i1 = ok[u] ? (int)rf : 0;
+      rem[u] = rf - (float)i1;
+      idx[u] = ok[u] ? (int)(w[u] >> 16) * VD_LUT_BINS + i1 : 0;
+    }
+    float4 q[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) q[u] = K.lutq[idx[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      float e, dor;
+      pair_term(q[u], rem[u], K.v_intra, e, dor);
+      e = ok[u] ? e : 0.0f; dor = ok[u] ? dor : 0.0f;
+      e_intra += e;
+      const float fx = dor * rx[u], fy = dor * ry[u], fz = dor * rz[u];
+      ax -= fx; ay -= fy; az -= fz;
+      const float px = __shfl(fx, src[u], G), py = __shfl(fy, src[u], G), pz = __shfl(fz, src[u], G);
+      if (src[u] != l) { ax += px; ay += py; az += pz; }
+    }
+  }
+  if (l < L.A) {
+    float4 f = S.frc[l];
+    f.x += ax; f.y += ay; f.z += az;
+    S.frc[l] = f;
+  }
+  return e_intra;
+}
+
+// Pair terms in the general case: lane l takes slot l of each chunk; within a chunk the first
+// atoms are distinct and so are the second atoms, so the force scatter is a plain read-modify-
+// write of LDS (first atoms, then second atoms), in chunk order: no atomics, and the same
+// summation order in every run.
+template <int G>
+__device__ float pair_terms_scatter(const LigView& L, const Scratch& S, const EvalK& K, int l) {
+  constexpr int U = 4;
+  const int groups = (L.C + U - 1) / U;
+  const uint4* pw = L.pairs + l;
+  float e_intra = 0.0f;
+  uint4 w_next = groups > 0 ? pw[0] : make_uint4(0, 0, 0, 0);
+  for (int cg = 0; cg < groups; ++cg) {
+    const uint4 w4 = w_next;
+    if (cg + 1 < groups) w_next = pw[(cg + 1) * G];
+    const uint32_t w[U] = {w4.x, w4.y, w4.z, w4.w};
+    bool ok[U];
+    int a[U], b[U], idx[U];
+    float rx[U], ry[U], rz[U], rem[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      ok[u] = w[u] != VD_NO_PAIR;
+      a[u] = ok[u] ? (int)(w[u] & 255) : 0;
+      b[u] = ok[u] ? (int)((w[u] >> 8) & 255) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const float4 pa = S.xyz[a[u]], pb = S.xyz[b[u]];
+      rx[u] = pb.x - pa.x; ry[u] = pb.y - pa.y; rz[u] = pb.z - pa.z;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
+      ok[u] = ok[u] && r2 < VD_CUTOFF_SQR;
+      const float rf = VD_LUT_FACTOR * r2;
+      const int i1 = ok[u] ? (int)rf : 0;
+      rem[u] = rf - (float)i1;
+      idx[u] = ok[u] ? (int)(w[u] >> 16) * VD_LUT_BINS + i1 : 0;
+    }
+    float4 q[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) q[u] = K.lutq[idx[u]];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      float e, dor;
+      pair_term(q[u], rem[u], K.v_intra, e, dor);
+      if (!ok[u]) continue;
+      e_intra += e;
+      const float fx = dor * rx[u], fy = dor * ry[u], fz = dor * rz[u];
+      float4 fa = S.frc[a[u]];
+      fa.x -= fx; fa.y -= fy; fa.z -= fz;
+      S.frc[a[u]] = fa;
+      float4 fb = S.frc[b[u]];
+      fb.x += fx; fb.y += fy; fb.z += fz;
+      S.frc[b[u]] = fb;
+    }
+  }
+  return e_intra;
+}
+
 // Energy and gradient of one conformation per group (model::eval_deriv,
 // V11/lib/model.cpp:1324-1334; device form quasi_newton.cuh:1294-1339).
 // Returns e (group-uniform); g = this lane's gradient component.  rg (when given) receives the
@@ -336,80 +469,8 @@ __device__ float eval_deriv(const LigView& L, const Scratch& S, const EvalK& K, 
   }
   if (rg) { const float acc = gsum<G>(racc); *rg = L.H > 0 ? sqrtf(acc / (float)L.H) : 0.0f; }
   __syncthreads();
-  // intramolecular pairs (eval_interacting_pairs_deriv, V11/lib/model.cpp:1241-1259).  Lane l
-  // takes slot l of each chunk; within a chunk the first atoms are distinct and so are the second
-  // atoms, so the force scatter is a plain read-modify-write of LDS (first atoms, then second
-  // atoms), in chunk order: no atomics, and the same summation order in every run.  When every
-  // atom has its own lane (L.rows), slot l holds pairs whose first atom is l: that side is summed
-  // in registers and only the second atoms are scattered.  Four chunks per pass: one 16-byte read
-  // of the lane's four pair words (global memory, prefetched a pass ahead), then all coordinate
-  // reads, then all four table reads, before the first result is used.
-  float e_intra = 0.0f;
-  {
-    constexpr int U = 4;
-    const int groups = (L.C + U - 1) / U;
-    const uint4* pw = L.pairs + l;
-    const bool rows = L.rows != 0;
-    const float4 own = (rows && l < L.A) ? S.xyz[l] : make_float4(0, 0, 0, 0);
-    float ox = 0.0f, oy = 0.0f, oz = 0.0f;       // rows: force on atom l as a first atom
-    uint4 w_next = groups > 0 ? pw[0] : make_uint4(0, 0, 0, 0);
-    for (int cg = 0; cg < groups; ++cg) {
-      const uint4 w4 = w_next;                     // the next group's words load during this one
-      if (cg + 1 < groups) w_next = pw[(cg + 1) * G];
-      const uint32_t w[U] = {w4.x, w4.y, w4.z, w4.w};
-      bool ok[U];
-      int a[U], b[U], idx[U];
-      float4 pa[U], pb[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        ok[u] = w[u] != VD_NO_PAIR;
-        a[u] = ok[u] ? (int)(w[u] & 255) : 0;
-        b[u] = ok[u] ? (int)((w[u] >> 8) & 255) : 0;
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) { pb[u] = S.xyz[b[u]]; pa[u] = rows ? own : S.xyz[a[u]]; }
-      float rx[U], ry[U], rz[U], rem[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        rx[u] = pb[u].x - pa[u].x; ry[u] = pb[u].y - pa[u].y; rz[u] = pb[u].z - pa[u].z;
-        const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
-        ok[u] = ok[u] && r2 < VD_CUTOFF_SQR;
-        const float rf = VD_LUT_FACTOR * r2;
-        const int i1 = ok[u] ? (int)rf : 0;
-        rem[u] = rf - (float)i1;
-        idx[u] = ok[u] ? (int)(w[u] >> 16) * VD_LUT_BINS + i1 : 0;
-      }
-      float4 q[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) q[u] = K.lutq[idx[u]];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        float e = q[u].x + rem[u] * (q[u].z - q[u].x);
-        float dor = q[u].y + rem[u] * (q[u].w - q[u].y);
-        if (e > 0.0f) { const float t = (K.v_intra < kEpsFl) ? 0.0f : K.v_intra / (K.v_intra + e); e *= t; dor *= t * t; }
-        e = ok[u] ? e : 0.0f; dor = ok[u] ? dor : 0.0f;
-        e_intra += e;
-        const float fx = dor * rx[u], fy = dor * ry[u], fz = dor * rz[u];
-        if (rows) {
-          ox -= fx; oy -= fy; oz -= fz;
-        } else if (ok[u]) {
-          float4 fa = S.frc[a[u]];
-          fa.x -= fx; fa.y -= fy; fa.z -= fz;
-          S.frc[a[u]] = fa;
-        }
-        if (ok[u]) {
-          float4 fb = S.frc[b[u]];
-          fb.x += fx; fb.y += fy; fb.z += fz;
-          S.frc[b[u]] = fb;
-        }
-      }
-    }
-    if (rows && l < L.A) {
-      float4 f = S.frc[l];
-      f.x += ox; f.y += oy; f.z += oz;
-      S.frc[l] = f;
-    }
-  }
+  // intramolecular pairs (eval_interacting_pairs_deriv, V11/lib/model.cpp:1241-1259)
+  const float e_intra = L.rows ? pair_terms_rows<G>(L, S, K, l) : pair_terms_scatter<G>(L, S, K, l);
   __syncthreads();
   // adjoint: per-node force and moment (about the root origin) sums.  A node's atoms are one
   // contiguous range, so the sums are a segmented inclusive scan over the atom lanes (segment =

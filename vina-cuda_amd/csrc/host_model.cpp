@@ -621,13 +621,31 @@ vd_lig_header pack_ligand(const Ligand& L, std::vector<float>& blob, int G) {
   h.chunk_lanes = G;
   h.n_chunks = (int)(pairs.size() / G);
   h.pair_rows = (int)L.atoms.size() <= G ? 1 : 0;
+  // rows: slot l holds pair (l, b); its word carries, in place of the first atom (= l), the slot
+  // whose pair has second atom l in the same chunk (255: none), from which lane l pulls that
+  // pair's force on atom l
+  std::vector<uint32_t> words(pairs);
+  if (h.pair_rows) {
+    for (int ch = 0; ch < h.n_chunks; ++ch) {
+      std::vector<uint32_t> inv(G, 255u);
+      for (int l = 0; l < G; ++l) {
+        const uint32_t w = pairs[(size_t)ch * G + l];
+        if (w != VD_NO_PAIR) inv[(w >> 8) & 255] = (uint32_t)l;
+      }
+      for (int l = 0; l < G; ++l) {
+        const uint32_t w = pairs[(size_t)ch * G + l];
+        const uint32_t b = w != VD_NO_PAIR ? (w >> 8) & 255 : 255u, tp = w != VD_NO_PAIR ? w >> 16 : 0u;
+        words[(size_t)ch * G + l] = (w == VD_NO_PAIR && inv[l] == 255u) ? VD_NO_PAIR : (inv[l] | (b << 8) | (tp << 16));
+      }
+    }
+  }
   const int groups = (h.n_chunks + 3) / 4;
   h.off_pairs = (int)blob.size();
   for (int cg = 0; cg < groups; ++cg)
     for (int l = 0; l < G; ++l)
       for (int u = 0; u < 4; ++u) {
         const int ch = cg * 4 + u;
-        blob.push_back(as_f(ch < h.n_chunks ? pairs[(size_t)ch * G + l] : VD_NO_PAIR));
+        blob.push_back(as_f(ch < h.n_chunks ? words[(size_t)ch * G + l] : VD_NO_PAIR));
       }
   pad16();
   return h;

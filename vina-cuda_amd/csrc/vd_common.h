@@ -24,8 +24,10 @@
 //             lane of a chain group); within a chunk every atom is the `a` of at most one pair
 //             and the `b` of at most one pair, so the force scatter of a chunk needs no atomics.
 //             Unused slots hold VD_NO_PAIR.  When the ligand has at most chunk_lanes atoms
-//             (pair_rows = 1), the pair in slot l has first atom l.  The area is laid out in
-//             groups of four chunks, [group][lane][4].
+//             (pair_rows = 1), the pair in slot l has first atom l and its word is
+//             src | b << 8 | tp << 16 instead: b = 255 for no pair, src = the slot of the same
+//             chunk whose pair has second atom l (255: none).  The area is laid out in groups of
+//             four chunks, [group][lane][4].
 #define VD_NO_PAIR 0xFFFFFFFFu
 struct vd_lig_header {
   int n_atoms, n_nodes, n_pairs, n_heavy;
