@@ -161,6 +161,7 @@ struct LigView {               // one ligand, staged in LDS for the wave
 struct Scratch {               // per chain, LDS
   float4* xyz;                 // A, coordinates relative to the root origin (w unused)
   float4* frc;                 // A, dE/dx (w unused)
+  float4* fx;                  // G + 1: pair-force exchange row (entry G stays zero)
   float4* frame;               // 3N rows (R row, origin component)
   float4* nacc;                // 2N: per-node force and moment sums
   float* stash;                // per wave: conformations parked out of registers, [slot][field][lane]
@@ -204,8 +205,8 @@ __device__ __forceinline__ float map_eval(const float* __restrict__ m, const vd_
   float gx = (f100 - f000) * my * mz + (f110 - f010) * y * mz + (f101 - f001) * my * z + (f111 - f011) * y * z;
   float gy = (f010 - f000) * mx * mz + (f110 - f100) * x * mz + (f011 - f001) * mx * z + (f111 - f101) * x * z;
   float gz = (f001 - f000) * mx * my + (f101 - f100) * x * my + (f011 - f010) * mx * y + (f111 - f110) * x * y;
-  if (f > 0.0f) {  // curl (V11/lib/curl.h:31-38)
-    const float t = (v < kEpsFl) ? 0.0f : v / (v + f);
+  if (f > 0.0f) {  // curl (V11/lib/curl.h:31-38); hardware reciprocal for the quotient
+    const float t = (v < kEpsFl) ? 0.0f : v * __builtin_amdgcn_rcpf(v + f);
     f *= t; const float t2 = t * t; gx *= t2; gy *= t2; gz *= t2;
   }
   deriv.x = gd.factor[0] * (region[0] == 0 ? gx : 0.0f) + slope * region[0];
@@ -293,55 +294,53 @@ __device__ float gyration_radius(const LigView& L, const Scratch& S, int l) {
 }
 
 // One pair term: value and derivative/r from the pair table quad, with the curl cap
-// (V11/lib/curl.h:4-13) above zero.
+// (V11/lib/curl.h:4-13) above zero.  The cap's quotient uses the hardware reciprocal (about one
+// ulp, far inside the parity tolerance).
 __device__ __forceinline__ void pair_term(const float4 q, float rem, float v_intra, float& e, float& dor) {
   e = q.x + rem * (q.z - q.x);
   dor = q.y + rem * (q.w - q.y);
-  if (e > 0.0f) { const float t = (v_intra < kEpsFl) ? 0.0f : v_intra / (v_intra + e); e *= t; dor *= t * t; }
+  const float t = (v_intra < kEpsFl) ? 0.0f : v_intra * __builtin_amdgcn_rcpf(v_intra + e);
+  const bool cap = e > 0.0f;
+  e = cap ? e * t : e;
+  dor = cap ? dor * (t * t) : dor;
 }
 
 // Pair terms when every atom has its own lane (L.rows): slot l of each chunk holds pair (l, b).
 // Lane l keeps the force on atom l in registers: its own pairs give the first-atom side, and the
-// second-atom side arrives by a lane shuffle from the one slot per chunk whose pair has second
-// atom l.  Coordinates of b come from lane b by shuffle as well, so the loop touches LDS only
-// through the shuffles.  Four chunks per pass (one 16-byte read of the lane's pair words,
-// prefetched a pass ahead), all four table reads issued before the first is used.
+// second-atom side comes from the one slot per chunk whose pair has second atom l, through a
+// per-group LDS exchange row (every lane writes its pair force to its entry, then reads the
+// source slot's entry; entry G stays zero for lanes without a source).  Empty slots read the
+// all-zero table quad and contribute nothing, so no lane needs a select.  Four chunks per pass
+// (one 16-byte read of the lane's pair words, prefetched a pass ahead), all four table reads
+// issued before the first is used.
 template <int G>
 __device__ float pair_terms_rows(const LigView& L, const Scratch& S, const EvalK& K, int l) {
   constexpr int U = 4;
   const int groups = (L.C + U - 1) / U;
   const uint4* pw = L.pairs + l;
   const float4 own = l < L.A ? S.xyz[l] : make_float4(0, 0, 0, 0);
+  if (l == 0) S.fx[G] = make_float4(0, 0, 0, 0);
   float e_intra = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
   uint4 w_next = groups > 0 ? pw[0] : make_uint4(0, 0, 0, 0);
   for (int cg = 0; cg < groups; ++cg) {
     const uint4 w4 = w_next;
     if (cg + 1 < groups) w_next = pw[(cg + 1) * G];
     const uint32_t w[U] = {w4.x, w4.y, w4.z, w4.w};
-    int b[U], src[U], idx[U];
-    bool ok[U];
+    int src[U], idx[U];
     float rx[U], ry[U], rz[U], rem[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t bb = (w[u] >> 8) & 255u, ss = w[u] & 255u;
-      ok[u] = w[u] != VD_NO_PAIR && bb != 255u;
-      b[u] = ok[u] ? (int)bb : l;
-      src[u] = (w[u] != VD_NO_PAIR && ss != 255u) ? (int)ss : l;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      rx[u] = __shfl(own.x, b[u], G) - own.x;
-      ry[u] = __shfl(own.y, b[u], G) - own.y;
-      rz[u] = __shfl(own.z, b[u], G) - own.z;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
+      const bool has = w[u] != VD_NO_PAIR && bb != 255u;
+      src[u] = (w[u] != VD_NO_PAIR && ss != 255u) ? (int)ss : G;
+      const float4 pb = S.xyz[has ? (int)bb : l];
+      rx[u] = pb.x - own.x; ry[u] = pb.y - own.y; rz[u] = pb.z - own.z;
       const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
-      ok[u] = ok[u] && r2 < VD_CUTOFF_SQR;
+      const bool ok = has && r2 < VD_CUTOFF_SQR;
       const float rf = VD_LUT_FACTOR * r2;
-      const int i1 = ok[u] ? (int)rf : 0;
+      const int i1 = ok ? (int)rf : 0;
       rem[u] = rf - (float)i1;
-      idx[u] = ok[u] ? (int)(w[u] >> 16) * VD_LUT_BINS + i1 : 0;
+      idx[u] = ok ? (int)(w[u] >> 16) * VD_LUT_BINS + i1 : VD_ZERO_QUAD;
     }
     float4 q[U];
 #pragma unroll
@@ -350,12 +349,12 @@ __device__ float pair_terms_rows(const LigView& L, const Scratch& S, const EvalK
     for (int u = 0; u < U; ++u) {
       float e, dor;
       pair_term(q[u], rem[u], K.v_intra, e, dor);
-      e = ok[u] ? e : 0.0f; dor = ok[u] ? dor : 0.0f;
       e_intra += e;
       const float fx = dor * rx[u], fy = dor * ry[u], fz = dor * rz[u];
       ax -= fx; ay -= fy; az -= fz;
-      const float px = __shfl(fx, src[u], G), py = __shfl(fy, src[u], G), pz = __shfl(fz, src[u], G);
-      if (src[u] != l) { ax += px; ay += py; az += pz; }
+      S.fx[l] = make_float4(fx, fy, fz, 0.0f);
+      const float4 in = S.fx[src[u]];
+      ax += in.x; ay += in.y; az += in.z;
     }
   }
   if (l < L.A) {
@@ -402,7 +401,7 @@ __device__ float pair_terms_scatter(const LigView& L, const Scratch& S, const Ev
       const float rf = VD_LUT_FACTOR * r2;
       const int i1 = ok[u] ? (int)rf : 0;
       rem[u] = rf - (float)i1;
-      idx[u] = ok[u] ? (int)(w[u] >> 16) * VD_LUT_BINS + i1 : 0;
+      idx[u] = ok[u] ? (int)(w[u] >> 16) * VD_LUT_BINS + i1 : VD_ZERO_QUAD;
     }
     float4 q[U];
 #pragma unroll

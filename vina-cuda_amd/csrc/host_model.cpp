@@ -148,7 +148,8 @@ void Precalculated::grid_rows(std::vector<float>& out) const {
 }
 
 void Precalculated::pair_quads(std::vector<float>& out) const {
-  out.assign((size_t)VD_NUM_TYPE_PAIRS * VD_LUT_BINS * 4, 0.0f);
+  // one extra all-zero quad at VD_ZERO_QUAD: empty pair slots read it and contribute nothing
+  out.assign(((size_t)VD_NUM_TYPE_PAIRS * VD_LUT_BINS + 1) * 4, 0.0f);
   for (int tp = 0; tp < VD_NUM_TYPE_PAIRS; ++tp)
     for (int b = 0; b < VD_LUT_BINS; ++b) {
       const size_t s = (size_t)tp * VD_LUT_BINS;

@@ -79,11 +79,12 @@ __global__ __launch_bounds__(256) void vd_grid_kernel(const float4* __restrict__
 
 // ------------------------------------------------------------------------------------------
 // LDS layout of one wave: ligand (atoms 4A, node_o 4N, node_a 4N floats), the conformation
-// stash, then per chain group frame 12N, nacc 8N, xyz 4A, frc 4A, and for dense BFGS H G*G + 2G.
-// The pair words stay in global memory (read once per evaluation, coalesced, L2-resident).
+// stash, then per chain group frame 12N, nacc 8N, xyz 4A, frc 4A, the pair-force exchange row
+// 4(G+1), and for dense BFGS H G*G + 2G.  The pair words stay in global memory (read once per
+// evaluation, coalesced, L2-resident).
 __host__ __device__ inline int lig_floats(int A, int N) { return 4 * A + 8 * N + kStashSlots * kStashSlotFloats; }
 __host__ __device__ inline int group_floats(int A, int N, int G, int dense) {
-  return 12 * N + 8 * N + 8 * A + (dense ? G * G + 2 * G : 0);
+  return 12 * N + 8 * N + 8 * A + 4 * (G + 1) + (dense ? G * G + 2 * G : 0);
 }
 
 template <int G>
@@ -107,7 +108,8 @@ __device__ __forceinline__ void make_views(const float4* __restrict__ blob, cons
   S.nacc = reinterpret_cast<float4*>(g + 12 * Nmax);
   S.xyz = reinterpret_cast<float4*>(g + 20 * Nmax);
   S.frc = S.xyz + Amax;
-  H = reinterpret_cast<float*>(S.frc + Amax);
+  S.fx = S.frc + Amax;
+  H = reinterpret_cast<float*>(S.fx + G + 1);
 }
 
 template <int G, int MODE>

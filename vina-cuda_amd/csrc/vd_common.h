@@ -10,6 +10,7 @@
 #define VD_LUT_BINS 2051             // int(32 * 64) + 3
 #define VD_LUT_FACTOR 32.0f
 #define VD_CUTOFF_SQR 64.0f
+#define VD_ZERO_QUAD (VD_NUM_TYPE_PAIRS * VD_LUT_BINS)   // index of the all-zero quad after the pair table
 #define VD_GRID_ROW 20               // per (t1, bin) row of the map LUT: 17 t2 values padded to 20
 #define VD_MAX_ATOMS 128             // per ligand (the reference cap MAX_NUM_OF_ATOMS, V11/inc/kernel2.h:35)
 #define VD_MAX_NODES 64              // rigid nodes (torsions + 1)
