@@ -105,6 +105,23 @@ def large_box_leg(vdock, log, n_ligands=4096, chains=128):
                      "GB_per_s": round(gbytes / (g_ms / 1000.0) / 1e9, 2)}}
 
 
+def traffic_from_profile(kernel, algorithmic_bytes):
+    """Memory traffic (bytes) of one launch of `kernel`: the bytes moved per algorithmic byte that
+    the committed counter runs (FETCH_SIZE and WRITE_SIZE, calibrated on a plain copy) measured for
+    the same kernel on this workload, times this launch's algorithmic bytes; None if the committed
+    profile is for another kernel."""
+    path = os.path.join(ROOT, "profiles", "vd_mc_kernel_batched_screening.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("<!-- traffic:"):
+            fields = dict(f.split("=", 1) for f in line.strip("<!-> \n").split()[1:] if "=" in f)
+            if fields.get("kernel") != kernel.replace(" ", ""):
+                return None
+            return round(float(fields["bytes_per_algorithmic_byte"]) * algorithmic_bytes)
+    return None
+
+
 def run_cpu_baseline(vdock, texts, box, xyz, xs, chains, budget_s=15.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
@@ -202,14 +219,15 @@ def main():
         "mean_best_score": round(float(np.mean(best)), 4),
     }
     name, d, ms_by_kernel = dominant_kernel(launches)
+    traffic = traffic_from_profile(name, d["bytes"] / d["n"])
     avg_ms = d["ms"] / d["n"]
     per_launch = d["bytes"] / d["n"]
     achieved = per_launch / (avg_ms / 1000.0) / 1e9
     out["roofline"] = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2),
                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                       "traffic": None, "launches": d["n"], "avg_launch_ms": round(avg_ms, 3),
+                       "traffic": traffic, "launches": d["n"], "avg_launch_ms": round(avg_ms, 3),
                        "algorithmic_bytes_per_launch": round(per_launch), "evals_per_launch": d["evals"] // d["n"],
-                       "kernel_ms_by_instantiation": ms_by_kernel}
+                       "kernel_ms_by_instantiation": ms_by_kernel, "mc_launches_per_step": len(launches) // args.steps}
 
     if not args.no_extras:
         # per-ligand plain launch loop (one kernel_monte-style launch per ligand) on the same

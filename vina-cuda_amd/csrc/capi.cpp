@@ -388,11 +388,11 @@ void autotune(vd_ctx* c, const std::vector<int>& ids, const vd_dock_params* p) {
                                                       : heur_depth((int)L.atoms.size(), L.n_torsions(), rilc));
     c->tune_e.ensure(chains); c->tune_conf.ensure((size_t)chains * (7 + L.n_torsions()));
     c->tune_heavy.ensure((size_t)chains * L.n_heavy * 3);
-    struct Trial { int G, W; double ms; };
+    struct Trial { int G, W, W_run; double ms; };
     std::vector<Trial> trials;   // preference order
     for (int G = 64; G >= gmin; G /= 2)
       for (int W : {3, 4, 2})
-        if (rilc || W == 2) trials.push_back({G, W, 1e300});
+        if (rilc || W == 2) trials.push_back({G, W, W, 1e300});
     auto run = [&](Trial& t) {
       vd_dock_params q = *p;
       q.chains = chains; q.chains_per_wave = 64 / t.G;
@@ -407,6 +407,7 @@ void autotune(vd_ctx* c, const std::vector<int>& ids, const vd_dock_params* p) {
       run_mc(c, t.G, t.W, pk, {(uint32_t)kv.second}, items, off, make_consts(c, &q, depth, bfgs), c->tune_e.p,
              c->tune_conf.p, c->tune_heavy.p, true, &st, true);
       t.ms = std::min(t.ms, st.kernel_ms);
+      t.W_run = c->launches.back().waves_per_simd;   // the budget the launch actually took
     };
     for (Trial& t : trials) run(t);
     double best = 1e300;
@@ -415,7 +416,7 @@ void autotune(vd_ctx* c, const std::vector<int>& ids, const vd_dock_params* p) {
     best = 1e300;
     for (const Trial& t : trials) best = std::min(best, t.ms);
     for (const Trial& t : trials)
-      if (t.ms <= 1.02 * best) { c->tuned[kv.first] = std::make_pair(t.G, t.W); break; }
+      if (t.ms <= 1.02 * best) { c->tuned[kv.first] = std::make_pair(t.G, t.W_run); break; }
   }
 }
 
