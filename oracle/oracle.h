@@ -114,6 +114,22 @@ int orc_initial_conf(void* lig, const orc_mc_params* p, int chain, double* conf)
 /* ---- result clustering (coords.cpp:24-62) ----
  * feeds n results in order into an output container (min_rmsd, max_size), returns kept
  * count and their indices sorted by energy */
+/* after the search (V11/main/main.cpp:139-157, 423-470): non_cache terms on the receptor atoms,
+ * intramolecular energy with fast table values, num_tors, conf_independent, rmsd bounds,
+ * refine_structure and the whole per-ligand post-processing. Coordinates are all atoms, xyz. */
+int    orc_ligand_serials(void* lig, int* serial);
+double orc_num_tors(void* lig);
+double orc_conf_independent(void* lig, double e, double w_rot);
+double orc_noncache_eval(void* grid, void* lig, const double* xyz, double v, double slope, double* minus_forces);
+double orc_intra_fast(void* grid, void* lig, const double* xyz, double v);
+int    orc_within(void* grid, void* lig, const double* xyz);
+int    orc_rmsd_bounds(void* lig, const double* x, const double* y, double* lb, double* ub);
+double orc_noncache_eval_deriv(void* grid, void* lig, const double* conf, double v, double slope, double* grad);
+double orc_refine(void* grid, void* lig, double* conf, int max_steps, double v, double* coords_out);
+int    orc_postprocess(void* grid, void* lig, int n, const double* confs, int max_steps, double w_rot, int num_modes,
+                       double energy_range, double min_rmsd, double slope, double* conf_out, double* e_out,
+                       double* lb_out, double* ub_out, double* refined_out);
+
 int orc_cluster(int n, int heavy, const double* e, const double* heavy_coords, double min_rmsd,
                 int max_size, int* kept_idx);
 

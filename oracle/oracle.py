@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 
 # Weights of the five Vina terms (V11/main/main.cpp:649-654)
 VINA_WEIGHTS = (-0.035579, -0.005156, 0.840245, -0.035069, -0.587439)
+VINA_WEIGHTS_ROT = 0.05846  # weight_rot, V11/main/main.cpp:654
 N_XS = 17
 N_PAIRS = 153
 
@@ -64,6 +65,17 @@ def _load():
         "orc_initial_conf": (C.c_int, [vp, vp, C.c_int, dp]),
         "orc_cluster": (C.c_int, [C.c_int, C.c_int, dp, dp, C.c_double, C.c_int, ip]),
         "orc_visited_feed": (C.c_int, [C.c_int, C.c_int, dp, dp, dp, C.c_int, ip]),
+        "orc_ligand_serials": (C.c_int, [vp, ip]),
+        "orc_num_tors": (C.c_double, [vp]),
+        "orc_conf_independent": (C.c_double, [vp, C.c_double, C.c_double]),
+        "orc_noncache_eval": (C.c_double, [vp, vp, dp, C.c_double, C.c_double, dp]),
+        "orc_intra_fast": (C.c_double, [vp, vp, dp, C.c_double]),
+        "orc_within": (C.c_int, [vp, vp, dp]),
+        "orc_rmsd_bounds": (C.c_int, [vp, dp, dp, dp, dp]),
+        "orc_noncache_eval_deriv": (C.c_double, [vp, vp, dp, C.c_double, C.c_double, dp]),
+        "orc_refine": (C.c_double, [vp, vp, dp, C.c_int, C.c_double, dp]),
+        "orc_postprocess": (C.c_int, [vp, vp, C.c_int, dp, C.c_int, C.c_double, C.c_int, C.c_double,
+                                      C.c_double, C.c_double, dp, dp, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -255,6 +267,24 @@ class Ligand:
         k = lib().orc_ligand_bonds(self.h, _i(bi), _i(bj), 4096)
         return list(zip(bi[:k].tolist(), bj[:k].tolist()))
 
+    def serials(self):
+        out = np.zeros(self.n_atoms, dtype=np.int32)
+        lib().orc_ligand_serials(self.h, _i(out))
+        return out
+
+    def num_tors(self):
+        return lib().orc_num_tors(self.h)
+
+    def conf_independent(self, e, w_rot=VINA_WEIGHTS_ROT):
+        return lib().orc_conf_independent(self.h, e, w_rot)
+
+    def rmsd_bounds(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        lb, ub = C.c_double(), C.c_double()
+        lib().orc_rmsd_bounds(self.h, _d(x), _d(y), C.byref(lb), C.byref(ub))
+        return lb.value, ub.value
+
     def identity_conf(self):
         c = np.zeros(7 + self.T)
         c[:3] = self.nodes()["origin0"][0]
@@ -357,3 +387,60 @@ def visited_feed(T, x, f, g, mode=1):
     if lib().orc_visited_feed(T, len(f), _d(x), _d(f), _d(g), mode, _i(out)) != 0:
         raise RuntimeError(_err())
     return out.astype(bool)
+
+
+# --- after the search: refinement against the receptor atoms and rescoring ----------------
+
+AUTHENTIC_V = 1000.0
+
+
+def noncache_eval(grid, lig, coords, v=AUTHENTIC_V, slope=1e6, deriv=False):
+    """non_cache::eval (fast table values) or, with deriv, ::eval_deriv -> (e, minus_forces)."""
+    x = np.ascontiguousarray(coords, dtype=np.float64)
+    if not deriv:
+        return lib().orc_noncache_eval(grid.h, lig.h, _d(x), v, slope, None)
+    f = np.zeros((lig.n_atoms, 3))
+    return lib().orc_noncache_eval(grid.h, lig.h, _d(x), v, slope, _d(f)), f
+
+
+def noncache_eval_deriv(grid, lig, conf, v=AUTHENTIC_V, slope=1e6):
+    """model::eval_deriv over non_cache: inter on the receptor atoms + intra, both capped at v."""
+    c = np.ascontiguousarray(conf, dtype=np.float64)
+    g = np.zeros(6 + lig.T)
+    return lib().orc_noncache_eval_deriv(grid.h, lig.h, _d(c), v, slope, _d(g)), g
+
+
+def intra_fast(grid, lig, coords, v=AUTHENTIC_V):
+    x = np.ascontiguousarray(coords, dtype=np.float64)
+    return lib().orc_intra_fast(grid.h, lig.h, _d(x), v)
+
+
+def within(grid, lig, coords):
+    x = np.ascontiguousarray(coords, dtype=np.float64)
+    return bool(lib().orc_within(grid.h, lig.h, _d(x)))
+
+
+def refine(grid, lig, conf, max_steps, v=AUTHENTIC_V):
+    """refine_structure: returns (e, refined conf, all-atom coordinates)."""
+    c = np.array(conf, dtype=np.float64)
+    xyz = np.zeros((lig.n_atoms, 3))
+    e = lib().orc_refine(grid.h, lig.h, _d(c), max_steps, v, _d(xyz))
+    return e, c, xyz
+
+
+def refine_steps(lig, rilc_bfgs=True):
+    """BFGS step budget of the refinement (V11/main/main.cpp:383-391)."""
+    return ((30 if rilc_bfgs else 25) + lig.n_atoms) // 3
+
+
+def postprocess(grid, lig, confs, max_steps, w_rot=VINA_WEIGHTS_ROT, num_modes=9, energy_range=3.0,
+                min_rmsd=1.0, slope=1e6):
+    """Per-ligand post-processing of V11/main/main.cpp:425-470 -> list of modes
+    (conf, score, rmsd lb, rmsd ub, refined energy)."""
+    confs = np.ascontiguousarray(confs, dtype=np.float64).reshape(-1, 7 + lig.T)
+    n = len(confs)
+    co = np.zeros_like(confs)
+    e, lb, ub, rf = (np.zeros(max(n, 1)) for _ in range(4))
+    k = lib().orc_postprocess(grid.h, lig.h, n, _d(confs), max_steps, w_rot, num_modes, energy_range,
+                              min_rmsd, slope, _d(co), _d(e), _d(lb), _d(ub), _d(rf))
+    return [dict(conf=co[i].copy(), e=e[i], lb=lb[i], ub=ub[i], refined=rf[i]) for i in range(k)]

@@ -214,6 +214,7 @@ struct AtomT {
   V3 coords;           // absolute input coordinates
   V3 local;            // coordinates relative to the owning node's origin
   std::vector<int> bonds;
+  unsigned serial = 0;  // PDBQT atom serial number
   bool is_heteroatom() const {
     bool het = ad != AD_A && ad != AD_C && ad != AD_H && ad != AD_HD && ad < AD_SIZE;
     return het || xs == XS_Met_D;
@@ -314,6 +315,7 @@ struct Node { int parent = -1, begin = 0, end = 0; V3 origin0, rel_origin, rel_a
 struct Pair { int tp, a, b; };
 struct Ligand {
   std::vector<AtomT> atoms; std::vector<Node> nodes; std::vector<Pair> pairs;
+  std::vector<std::pair<int, int>> rotors;   // atom pairs across rotatable bonds (DISTANCE_ROTOR)
   int torsdof = 0; int heavy = 0;
   int T() const { return (int)nodes.size() - 1; }
 };
@@ -365,7 +367,7 @@ struct Builder {
   explicit Builder(Ligand& l) : L(l) {}
   int insert(PAtom& pa, const V3& frame_origin) {  // node_t::insert, :183-190
     for (auto* ps : pa.ps) ps->axis_begin = (int)L.atoms.size();
-    AtomT a = pa.a; a.local = a.coords - frame_origin;
+    AtomT a = pa.a; a.local = a.coords - frame_origin; a.serial = pa.number;
     L.atoms.push_back(a);
     return (int)L.atoms.size() - 1;
   }
@@ -446,6 +448,7 @@ Ligand* parse_ligand(const std::string& text) {
   for (auto& nd : L->nodes) for (int i = nd.begin; i < nd.end; ++i) for (int j = i + 1; j < nd.end; ++j) mob.set(i, j, DIST_FIXED);
   for (auto& fr : fixed_ranges) { int a = std::get<0>(fr); if (a < 0) continue; for (int i = std::get<1>(fr); i < std::get<2>(fr); ++i) mob.set(a, i, DIST_FIXED); }
   for (auto& rt : rotors) mob.set(rt.first, rt.second, DIST_ROTOR);
+  L->rotors = rotors;
   // add_bonds for axis_end is applied before set_rotor, so the rotor entry wins (order as in :430-432)
   assign_bonds(L->atoms, mob);
   assign_types(L->atoms);
@@ -617,9 +620,13 @@ void set_conf(const Ligand& L, const Conf& c, std::vector<V3>& coords, Frames& F
   }
 }
 
+fl noncache_terms(const Grid& g, const Ligand& L, const std::vector<V3>& coords, fl v, fl slope,
+                  std::vector<V3>* minus_forces);
+
 struct EvalCtx {
   const Grid* grid; const Ligand* L; fl v_intra, v_inter, slope;
   long long evals = 0;
+  bool noncache = false;  // receptor atoms directly (non_cache) instead of the grid maps
   std::vector<V3> coords, forces; Frames F;
   fl e_inter = 0;
   fl operator()(const Conf& c, std::vector<fl>& g) {
@@ -628,7 +635,8 @@ struct EvalCtx {
     const int n = (int)L->atoms.size();
     forces.assign(n, V3());
     fl e = 0;
-    for (int i = 0; i < n && g_use_inter; ++i) {  // cache::eval_deriv
+    if (noncache && g_use_inter) e = noncache_terms(*grid, *L, coords, v_inter, slope, &forces);
+    for (int i = 0; i < n && g_use_inter && !noncache; ++i) {  // cache::eval_deriv
       int t = L->atoms[i].xs; if (t >= XS_SIZE) continue;
       V3 d; e += grid->evaluate(t, coords[i], slope, v_inter, &d); forces[i] = d;
     }
@@ -840,6 +848,192 @@ fl local_search(EvalCtx& ctx, Conf& x, int max_steps, int mode) {
   fl e = mode == 1 ? rilc_bfgs(ctx, inc, x, g, max_steps) : dense_bfgs(ctx, inc, x, g, max_steps);
   x.e = e;
   return e;
+}
+
+// ---------------------------------------------------------------------------------------
+// After the search (V11/main/main.cpp:420-470): each kept pose is refined against the receptor
+// atoms themselves, and the final score is adjusted by the ligand's torsion count.
+
+// non_cache::eval and ::eval_deriv (V11/lib/non_cache.cpp:38-128): per typed movable atom, the
+// coordinates clamped to the box, slope x the distance outside it, and every typed receptor
+// atom within the cutoff through the table -- fast values for the energy alone, interpolated
+// values and dor for the derivative -- capped by curl(v).  minus_forces receives dE/dx.
+fl noncache_terms(const Grid& g, const Ligand& L, const std::vector<V3>& coords, fl v, fl slope,
+                  std::vector<V3>* minus_forces) {
+  const Lut& lut = *g.lut;
+  fl e = 0;
+  const int n = (int)L.atoms.size();
+  if (minus_forces) minus_forces->assign(n, V3());
+  for (int i = 0; i < n; ++i) {
+    const int t1 = L.atoms[i].xs;
+    if (t1 >= XS_SIZE) continue;
+    V3 adj = coords[i], oob;
+    fl pen = 0;
+    for (int j = 0; j < 3; ++j) {
+      if (g.n[j] <= 0) continue;
+      if (coords[i][j] < g.begin[j]) { adj[j] = g.begin[j]; oob[j] = -1; pen += std::fabs(coords[i][j] - g.begin[j]); }
+      else if (coords[i][j] > g.end[j]) { adj[j] = g.end[j]; oob[j] = 1; pen += std::fabs(coords[i][j] - g.end[j]); }
+    }
+    pen *= slope;
+    fl this_e = 0; V3 deriv;
+    for (size_t b = 0; b < g.rxyz.size(); ++b) {
+      const int t2 = g.rxs[b];
+      if (t2 >= XS_SIZE) continue;
+      const V3 r = adj - g.rxyz[b];
+      const fl r2 = dot(r, r);
+      if (r2 >= lut.cutoff_sqr) continue;
+      const int tp = tri_index(t1, t2);
+      if (minus_forces) { fl pe, dor; lut.eval_deriv(tp, r2, pe, dor); this_e += pe; deriv = deriv + dor * r; }
+      else this_e += lut.eval_fast(tp, r2);
+    }
+    if (minus_forces) { curl(this_e, deriv, v); (*minus_forces)[i] = deriv + slope * oob; }
+    else if (this_e > 0 && v < 0.1 * MAXFL) this_e *= (v < EPS) ? 0 : v / (v + this_e);
+    e += this_e + pen;
+  }
+  return e;
+}
+
+// eval_interacting_pairs (V11/lib/model.cpp:1201-1214): fast table values, curl on the energy.
+fl intra_fast(const Lut& lut, const Ligand& L, const std::vector<V3>& coords, fl v) {
+  fl e = 0;
+  for (const Pair& p : L.pairs) {
+    const fl r2 = dist2(coords[p.a], coords[p.b]);
+    if (r2 >= lut.cutoff_sqr) continue;
+    fl t = lut.eval_fast(p.tp, r2);
+    if (t > 0 && v < 0.1 * MAXFL) t *= (v < EPS) ? 0 : v / (v + t);
+    e += t;
+  }
+  return e;
+}
+
+// conf_independent_inputs::num_tors (V11/lib/terms.cpp:54-125): half the rotatable bonds seen
+// from each heavy atom, counting a bond only when the other atom is heavy and has more than one
+// heavy neighbour (terminal CH3, OH, NH2 are not counted).
+fl num_tors(const Ligand& L) {
+  auto rotatable = [&](int a, int b) {
+    for (auto& r : L.rotors) if ((r.first == a && r.second == b) || (r.first == b && r.second == a)) return true;
+    return false;
+  };
+  auto heavy_nb = [&](int a) { int k = 0; for (int b : L.atoms[a].bonds) if (L.atoms[b].el != EL_H) ++k; return k; };
+  fl nt = 0;
+  for (int i = 0; i < (int)L.atoms.size(); ++i) {
+    if (L.atoms[i].el == EL_H) continue;
+    int ar = 0;
+    for (int b : L.atoms[i].bonds)
+      if (rotatable(i, b) && L.atoms[b].el != EL_H && heavy_nb(b) > 1) ++ar;
+    nt += 0.5 * ar;
+  }
+  return nt;
+}
+
+// weighted_terms::conf_independent with Vina's only such term, num_tors_div
+// (V11/lib/everything.cpp:286-303, weighted_terms.cpp:83-89): the weight stored for it is
+// 5 w_rot / 0.1 - 1 (main.cpp:649-654), so the divisor is 1 + w_rot * num_tors; smooth_div.
+fl conf_independent(const Ligand& L, fl e, fl w_rot) {
+  const fl stored = 5 * w_rot / 0.1 - 1, w = 0.1 * (stored + 1);
+  const fl y = 1 + w * num_tors(L) / 5.0;
+  if (std::fabs(e) < EPS) return 0;
+  if (std::fabs(y) < EPS) return e * y > 0 ? MAXFL : -MAXFL;
+  return e / y;
+}
+
+// model::rmsd_lower_bound / rmsd_upper_bound (V11/lib/model.cpp:1453-1521), heavy atoms.
+fl rmsd_lb_asym(const Ligand& L, const std::vector<V3>& x, const std::vector<V3>& y) {
+  fl sum = 0; int cnt = 0;
+  for (size_t i = 0; i < L.atoms.size(); ++i) {
+    if (L.atoms[i].el == EL_H) continue;
+    fl r2 = MAXFL;
+    for (size_t j = 0; j < L.atoms.size(); ++j)
+      if (L.atoms[j].el == L.atoms[i].el && L.atoms[j].el != EL_H) r2 = std::min(r2, dist2(x[i], y[j]));
+    sum += r2; ++cnt;
+  }
+  return cnt ? std::sqrt(sum / cnt) : 0;
+}
+fl rmsd_ub(const Ligand& L, const std::vector<V3>& x, const std::vector<V3>& y) {
+  fl sum = 0; int cnt = 0;
+  for (size_t i = 0; i < L.atoms.size(); ++i) if (L.atoms[i].el != EL_H) { sum += dist2(x[i], y[i]); ++cnt; }
+  return cnt ? std::sqrt(sum / cnt) : 0;
+}
+
+bool within_box(const Grid& g, const Ligand& L, const std::vector<V3>& coords, fl margin = 0.0001) {  // non_cache.h:43, .cpp:58-66
+  for (size_t i = 0; i < L.atoms.size(); ++i) {
+    if (L.atoms[i].el == EL_H) continue;
+    for (int j = 0; j < 3; ++j)
+      if (g.n[j] > 0 && (coords[i][j] < g.begin[j] - margin || coords[i][j] > g.end[j] + margin)) return false;
+  }
+  return true;
+}
+
+// refine_structure (V11/main/main.cpp:139-157): dense BFGS (bfgs.h) on the receptor atoms
+// directly, inter and intra capped at v, with the out-of-box slope raised 100 -> 1e10 until the
+// heavy atoms lie within the box; a pose that never gets inside scores MAXFL.
+fl refine(const Grid& g, const Ligand& L, Conf& c, int max_steps, fl v, std::vector<V3>& coords) {
+  EvalCtx ctx{&g, &L, v, v, 0};
+  ctx.noncache = true;
+  ConfInc inc;
+  fl e = 0;
+  Frames F;
+  for (int p = 0; p < 5; ++p) {
+    ctx.slope = 100 * std::pow(10.0, 2.0 * p);
+    std::vector<fl> gr;
+    e = dense_bfgs(ctx, inc, c, gr, max_steps);
+    set_conf(L, c, coords, F);
+    if (within_box(g, L, coords)) break;
+  }
+  if (!within_box(g, L, coords)) e = MAXFL;
+  return e;
+}
+
+// main.cpp:425-470 for one ligand: refine every pose, sort, rescore against the best pose's
+// intramolecular energy (model::eval_adjusted, model.cpp:1428-1431), sort again,
+// remove_redundant (main.cpp:172-178) and keep up to num_modes poses within energy_range of
+// the best, with rmsd lower/upper bounds against the best (model.cpp:1453-1521).
+struct Mode { Conf c; fl e = 0, lb = 0, ub = 0, refined = 0; std::vector<V3> coords; };
+std::vector<Mode> postprocess(const Grid& g, const Ligand& L, std::vector<Conf> poses, int max_steps,
+                              fl w_rot, int num_modes, fl energy_range, fl min_rmsd, fl slope) {
+  const fl v = 1000;
+  std::vector<Mode> out(poses.size());
+  for (size_t i = 0; i < poses.size(); ++i) {
+    out[i].c = poses[i];
+    out[i].e = out[i].refined = refine(g, L, out[i].c, max_steps, v, out[i].coords);
+  }
+  auto by_e = [](const Mode& a, const Mode& b) { return a.e < b.e; };
+  std::stable_sort(out.begin(), out.end(), by_e);
+  if (out.empty()) return out;
+  Frames F; std::vector<V3> coords;
+  set_conf(L, out[0].c, coords, F);
+  const fl best_intra = intra_fast(*g.lut, L, coords, v);
+  for (Mode& m : out) {
+    if (!(m.e < 0.1 * MAXFL)) continue;
+    set_conf(L, m.c, coords, F);
+    const fl e = noncache_terms(g, L, coords, v, slope, nullptr) + intra_fast(*g.lut, L, coords, v);
+    m.e = conf_independent(L, e - best_intra, w_rot);
+  }
+  std::stable_sort(out.begin(), out.end(), by_e);
+  // remove_redundant: add_to_output_container with max_size = in.size(), on heavy-atom coordinates
+  auto heavy_rmsd = [&](const Mode& a, const Mode& b) {
+    fl acc = 0; int cnt = 0;
+    for (size_t i = 0; i < L.atoms.size(); ++i) if (L.atoms[i].el != EL_H) { acc += dist2(a.coords[i], b.coords[i]); ++cnt; }
+    return cnt ? std::sqrt(acc / cnt) : 0.0;
+  };
+  std::vector<Mode> kept;
+  for (const Mode& t : out) {
+    int closest = -1; fl best = MAXFL;
+    for (size_t i = 0; i < kept.size(); ++i) { fl r = heavy_rmsd(t, kept[i]); if (i == 0 || r < best) { best = r; closest = (int)i; } }
+    if (closest >= 0 && best < min_rmsd) { if (t.e < kept[closest].e) kept[closest] = t; }
+    else if (kept.size() < out.size()) kept.push_back(t);
+    else if (!kept.empty() && t.e < kept.back().e) kept.back() = t;
+    std::stable_sort(kept.begin(), kept.end(), by_e);
+  }
+  std::vector<Mode> modes;
+  for (const Mode& m : kept) {
+    if ((int)modes.size() >= num_modes || !(m.e < 0.1 * MAXFL) || m.e > kept[0].e + energy_range) break;
+    Mode r = m;
+    r.lb = std::max(rmsd_lb_asym(L, m.coords, kept[0].coords), rmsd_lb_asym(L, kept[0].coords, m.coords));
+    r.ub = rmsd_ub(L, m.coords, kept[0].coords);
+    modes.push_back(r);
+  }
+  return modes;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1147,6 +1341,67 @@ int orc_mc_run(void* gp, void* lp, const orc_mc_params* p, double* e, double* co
   for (auto& th : pool) th.join();
   if (evals) *evals = total;
   return 0;
+}
+
+int orc_ligand_serials(void* lp, int* serial) {
+  Ligand* L = (Ligand*)lp;
+  for (size_t i = 0; i < L->atoms.size(); ++i) serial[i] = (int)L->atoms[i].serial;
+  return 0;
+}
+double orc_num_tors(void* lp) { return num_tors(*(Ligand*)lp); }
+double orc_conf_independent(void* lp, double e, double w_rot) { return conf_independent(*(Ligand*)lp, e, w_rot); }
+static std::vector<V3> to_v3(const double* xyz, size_t n) {
+  std::vector<V3> c(n); for (size_t i = 0; i < n; ++i) c[i] = V3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]); return c;
+}
+double orc_noncache_eval(void* gp, void* lp, const double* xyz, double v, double slope, double* minus_forces) {
+  Grid* g = (Grid*)gp; Ligand* L = (Ligand*)lp;
+  std::vector<V3> f;
+  fl e = noncache_terms(*g, *L, to_v3(xyz, L->atoms.size()), v, slope, minus_forces ? &f : nullptr);
+  if (minus_forces) for (size_t i = 0; i < f.size(); ++i) for (int k = 0; k < 3; ++k) minus_forces[3 * i + k] = f[i][k];
+  return e;
+}
+double orc_intra_fast(void* gp, void* lp, const double* xyz, double v) {
+  Ligand* L = (Ligand*)lp; return intra_fast(*((Grid*)gp)->lut, *L, to_v3(xyz, L->atoms.size()), v);
+}
+int orc_within(void* gp, void* lp, const double* xyz) {
+  Ligand* L = (Ligand*)lp; return within_box(*(Grid*)gp, *L, to_v3(xyz, L->atoms.size())) ? 1 : 0;
+}
+int orc_rmsd_bounds(void* lp, const double* x, const double* y, double* lb, double* ub) {
+  Ligand* L = (Ligand*)lp; const size_t n = L->atoms.size();
+  std::vector<V3> a = to_v3(x, n), b = to_v3(y, n);
+  *lb = std::max(rmsd_lb_asym(*L, a, b), rmsd_lb_asym(*L, b, a));
+  *ub = rmsd_ub(*L, a, b);
+  return 0;
+}
+double orc_noncache_eval_deriv(void* gp, void* lp, const double* conf, double v, double slope, double* grad) {
+  Ligand* L = (Ligand*)lp;
+  EvalCtx ctx{(Grid*)gp, L, v, v, slope};
+  ctx.noncache = true;
+  std::vector<fl> g; Conf c = conf_from(*L, conf);
+  double e = ctx(c, g);
+  for (size_t i = 0; i < g.size(); ++i) grad[i] = g[i];
+  return e;
+}
+double orc_refine(void* gp, void* lp, double* conf, int max_steps, double v, double* coords_out) {
+  Ligand* L = (Ligand*)lp; Conf c = conf_from(*L, conf); std::vector<V3> coords;
+  fl e = refine(*(Grid*)gp, *L, c, max_steps, v, coords);
+  conf_to(c, conf);
+  if (coords_out) for (size_t i = 0; i < coords.size(); ++i) for (int k = 0; k < 3; ++k) coords_out[3 * i + k] = coords[i][k];
+  return e;
+}
+int orc_postprocess(void* gp, void* lp, int n, const double* confs, int max_steps, double w_rot, int num_modes,
+                    double energy_range, double min_rmsd, double slope, double* conf_out, double* e_out,
+                    double* lb_out, double* ub_out, double* refined_out) {
+  Ligand* L = (Ligand*)lp; const int cs = 7 + L->T();
+  std::vector<Conf> poses;
+  for (int i = 0; i < n; ++i) poses.push_back(conf_from(*L, confs + (size_t)i * cs));
+  std::vector<Mode> m = postprocess(*(Grid*)gp, *L, poses, max_steps, w_rot, num_modes, energy_range, min_rmsd, slope);
+  for (size_t i = 0; i < m.size(); ++i) {
+    conf_to(m[i].c, conf_out + i * cs);
+    e_out[i] = m[i].e; lb_out[i] = m[i].lb; ub_out[i] = m[i].ub;
+    if (refined_out) refined_out[i] = m[i].refined;
+  }
+  return (int)m.size();
 }
 
 int orc_cluster(int n, int heavy, const double* e, const double* hc, double min_rmsd, int max_size, int* kept) {
