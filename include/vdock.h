@@ -12,6 +12,8 @@
  *   main_procedure_cl V11/lib/main_procedure_cu.h:2-4 (host driver)  -> the calls below in sequence
  *   cache::populate   V11/lib/cache.h:73            -> vd_grid_populate
  *   model::eval_deriv V11/lib/model.cpp:1324-1334   -> vd_eval
+ *   refine_structure + rescoring + output, V11/main/main.cpp:139-178, 423-501
+ *                     -> vd_refine, vd_postprocess_batch / vd_postprocess_docked, vd_write_pdbqt
  */
 #ifndef VDOCK_H
 #define VDOCK_H
@@ -86,6 +88,8 @@ typedef struct {
   int n_heavy;      /* non-hydrogen atoms (the pose coordinates reported per chain) */
   int torsdof;      /* TORSDOF record */
   int tree_depth;   /* depth of the rigid-node tree */
+  float num_tors;   /* torsion count of the final score's divisor (conf_independent_inputs,
+                       V11/lib/terms.cpp:97-125) */
 } vd_ligand_info;
 int vd_ligand_info_get(vd_ctx* ctx, int id, vd_ligand_info* info);
 /* Flattened model: per atom xs, element class (0 = H), node, node-local xyz; per node parent,
@@ -175,6 +179,74 @@ int vd_batch_best(vd_ctx* ctx, float* best_e);
 /* Output pose list: add_to_output_container over chains in order (min_rmsd, max_size;
  * V11/lib/coords.cpp:47-62).  Writes kept chain indices sorted by energy; returns the count. */
 int vd_cluster(int n, int n_heavy, const float* e, const float* heavy, double min_rmsd, int max_size, int* kept);
+
+/* ---- After the search: refinement, final scores and output.  The reference runs these on the
+ * CPU after kernel_monte (V11/main/main.cpp:139-178, 423-501); here the refinement runs on the
+ * device, one pose per wavefront, many ligands per launch. ---- */
+typedef struct {
+  int max_steps;        /* BFGS steps per slope stage; 0 = the reference's bfgs_steps heuristic
+                           (V11/main/main.cpp:383-391): (30 + n_atoms) / 3 after a RILC-BFGS search,
+                           (25 + n_atoms) / 3 otherwise */
+  int local_search;     /* the search that produced the poses (selects that heuristic) */
+  int num_modes;        /* modes kept per ligand (--num_modes, 9) */
+  float energy_range;   /* kcal/mol above the best mode (--energy_range, 3) */
+  float min_rmsd;       /* remove_redundant distance (1 A, main.cpp:466) */
+  float v;              /* curl cap of the refinement and the final score (authentic_v, 1000) */
+  float slope;          /* out-of-box slope of the final score (the non_cache slope, 1e6) */
+} vd_post_params;
+int vd_post_params_default(vd_post_params* p);
+
+/* refine_structure (V11/main/main.cpp:139-157) of n poses of ligand id: dense BFGS (V11/lib/bfgs.h)
+ * on the receptor atoms themselves (non_cache::eval_deriv, V11/lib/non_cache.cpp:82-128) plus the
+ * intramolecular pairs, both capped at v, the out-of-box slope raised 1e2, 1e4, .. 1e10 until every
+ * heavy atom lies within the box (non_cache::within, margin 1e-4).  conf_in/conf_out [n][7+T] (may
+ * alias), coords_out [n][n_atoms][3] all atoms, the ligand's internal order (NULL allowed).  The
+ * box is the one of the resident maps. */
+typedef struct {
+  float e;          /* refined energy (output_type::e); FLT_MAX when the pose never got inside the box */
+  float e_inter;    /* non_cache::eval at the refined pose: fast table, cap v, slope p->slope
+                       (V11/lib/non_cache.cpp:38-56) */
+  float e_intra;    /* model::eval_intramolecular at the refined pose, cap v (V11/lib/model.cpp:1375-1417) */
+  int inside;       /* 1 when the heavy atoms are within the box */
+} vd_refined;
+int vd_refine(vd_ctx* ctx, int id, int n, const float* conf_in, const vd_post_params* p,
+              float* conf_out, float* coords_out, vd_refined* out);
+
+/* One mode of the output (output_type after main.cpp:425-470). */
+typedef struct {
+  float e;          /* final score: conf_independent(e_inter + e_intra - e_intra of the best mode),
+                       num_tors_div (V11/lib/model.cpp:1428-1431, everything.cpp:296-303) */
+  float rmsd_lb;    /* rmsd lower bound to the best mode (model.cpp:1453-1508) */
+  float rmsd_ub;    /* rmsd upper bound to the best mode (model.cpp:1510-1521) */
+  int pose;         /* index of the pose it came from */
+} vd_mode;
+/* The selection of V11/main/main.cpp:425-470 over n refined poses of ligand id (host only, no
+ * device work): sort by refined energy, eval_adjusted against the best pose's intramolecular
+ * energy, sort, remove_redundant (min_rmsd), keep up to num_modes poses within energy_range of
+ * the best; coords [n][n_atoms][3] as vd_refine returns them.  Returns the number of modes. */
+int vd_select_modes(vd_ctx* ctx, int id, int n, const vd_refined* r, const float* coords,
+                    const vd_post_params* p, vd_mode* modes);
+
+/* Both steps for many ligands, every pose in one launch: ligand ids[k] has poses[k] poses, their
+ * conformations concatenated in conf (sum poses[k] x (7+T_k) floats).  vd_dock_batch's output
+ * feeds it through vd_postprocess_docked.  Results per ligand: vd_post_results. */
+int vd_postprocess_batch(vd_ctx* ctx, int n, const int* ids, const int* poses, const float* conf,
+                         const vd_post_params* p, vd_stats* st);
+/* The same for every ligand of the last vd_dock_batch / vd_dock_batch_plain: each ligand's chains
+ * are first reduced to its pose list as the reference's host does (add_to_output_container with
+ * min_rmsd 1 and max_poses, V11/lib/main_procedure_cu.cpp:908-926; max_poses 20 = num_saved_mins). */
+int vd_postprocess_docked(vd_ctx* ctx, int max_poses, const vd_post_params* p, vd_stats* st);
+/* Modes of ligand k of the last post-processing call: modes[num_modes], their conformations
+ * conf[m][7+T] and all-atom coordinates coords[m][n_atoms][3] (NULL allowed).  Returns the count. */
+int vd_post_results(vd_ctx* ctx, int k, vd_mode* modes, float* conf, float* coords);
+
+/* PDBQT text of n modes of ligand id (write_all_output, V11/main/main.cpp:89-102; vina_remark,
+ * main.cpp:158-170; model::write_context, V11/lib/model.cpp:1063-1110): per mode "MODEL i", the
+ * "REMARK VINA RESULT:" line with score, rmsd lb, rmsd ub, then the ligand's input lines with
+ * each atom line's coordinates (columns 31-54, %8.3f) replaced, then "ENDMDL".
+ * coords [n][n_atoms][3].  Writes at most cap bytes including the terminating NUL and returns the
+ * full length without it (so cap = 0 asks for the size), or an error. */
+int vd_write_pdbqt(vd_ctx* ctx, int id, int n, const vd_mode* modes, const float* coords, char* out, int cap);
 
 /* Device memory on the context's GPU (for callers that keep inputs resident in HBM).
  * vd_copy direction: 0 host->device, 1 device->host, 2 device->device; synchronous. */

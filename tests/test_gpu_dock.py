@@ -3,7 +3,9 @@ random stream).  Docking is stochastic and FP32 vs FP64 trajectories separate af
 so parity is judged over 16 seeds:
   - best score over the 16 runs within 0.1 kcal/mol of the oracle's best over its 16 runs;
   - mean per-run best score within 0.1 kcal/mol;
-  - the two overall top poses within 2 A RMSD (heavy atoms);
+  - each side's overall top pose within 2 A RMSD (heavy atoms) of a top pose the other side
+    reached within 0.1 kcal/mol of its own best (distinct minima whose scores differ by less than
+    the search's spread are equally good answers; the argmin alone flips between them);
   - the fraction of runs whose top pose lands within 2 A of the overall top pose is no worse than
     the oracle's own fraction minus 0.25 (equal search accuracy).
 """
@@ -37,6 +39,16 @@ def rmsd(a, b):
     return float(np.sqrt(((a - b) ** 2).sum(axis=-1).mean()))
 
 
+def top_pose_distance(gb, gp, ob, op, de=0.1):
+    """Largest of: distance from the GPU's top pose to the nearest oracle run top within de of
+    the oracle's best, and the same with the sides swapped."""
+    gb, ob = np.asarray(gb), np.asarray(ob)
+    g_top, o_top = gp[int(np.argmin(gb))], op[int(np.argmin(ob))]
+    d_g = min(rmsd(g_top, op[i]) for i in range(len(ob)) if ob[i] <= ob.min() + de)
+    d_o = min(rmsd(o_top, gp[i]) for i in range(len(gb)) if gb[i] <= gb.min() + de)
+    return max(d_g, d_o)
+
+
 @pytest.mark.parametrize("heavy,tors,lseed", [(12, 2, 31), (16, 4, 32)])
 def test_docking_statistical_parity(ctx, scene, heavy, tors, lseed):
     import vdock
@@ -62,10 +74,11 @@ def test_docking_statistical_parity(ctx, scene, heavy, tors, lseed):
     succ_g = np.mean([rmsd(x, top_o) <= 2.0 for x in gp])
     succ_o = np.mean([rmsd(x, top_o) <= 2.0 for x in op_])
     say("best gpu", gb.min(), "oracle", ob.min(), "mean", gb.mean(), ob.mean(), "top-pose rmsd", rmsd(top_g, top_o),
+        "to the other side's near-best", top_pose_distance(gb, gp, ob, op_),
         "runs within 2A of top: gpu", succ_g, "oracle", succ_o)
     assert abs(gb.min() - ob.min()) <= 0.1
     assert abs(gb.mean() - ob.mean()) <= 0.1
-    assert rmsd(top_g, top_o) <= 2.0
+    assert top_pose_distance(gb, gp, ob, op_) <= 2.0
     assert succ_g >= succ_o - 0.25
 
 
@@ -108,7 +121,7 @@ def test_quickvina2_policy_parity(ctx, scene, heavy, tors, lseed):
         rmsd(top_g, top_o), "succ", succ_g, succ_o, "eval ratio to vina: gpu", r_g, "oracle", r_o)
     assert abs(gb.min() - ob.min()) <= 0.1
     assert abs(gb.mean() - ob.mean()) <= 0.1
-    assert rmsd(top_g, top_o) <= 2.0
+    assert top_pose_distance(gb, qv["gp"], ob, qv["op"]) <= 2.0
     assert succ_g >= succ_o - 0.25
     assert r_o < 0.97                      # the gate is active in this setting
     assert abs(r_g - r_o) <= 0.05

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <exception>
 #include <functional>
+#include <limits>
 #include <tuple>
 #include <map>
 #include <string>
@@ -53,6 +54,13 @@ struct BatchState {   // results of the last batch, kept on the device
   DevBuf<float> e, conf, heavy;
 };
 
+struct PostLigand {   // modes of one ligand after refinement and selection
+  int id = 0;
+  std::vector<vd_mode> modes;
+  std::vector<float> conf, coords;   // per mode: 7+T floats, 3A floats
+};
+struct PostState { std::vector<PostLigand> ligs; };
+
 }  // namespace
 
 struct vd_ctx {
@@ -63,7 +71,7 @@ struct vd_ctx {
   double weights[6] = {-0.035579, -0.005156, 0.840245, -0.035069, -0.587439, 0.05846};
   vd::Precalculated pre;
   DevBuf<float> lut_rows, lutq;
-  std::vector<float> rec_xyz;
+  std::vector<double> rec_xyz;   // receptor coordinates as parsed (FP64)
   std::vector<int> rec_xs;
   // maps
   bool maps_ready = false;
@@ -91,6 +99,20 @@ struct vd_ctx {
   DevBuf<float> tune_e, tune_conf, tune_heavy;
   DevBuf<float> hist;                        // QuickVina2 visited histories
   std::vector<vd_launch> launches;           // MC launches of the last docking call
+  // after the search: receptor cells of the direct evaluation (built on first use per box)
+  bool nc_ready = false;
+  vdr::NcD nck{};
+  DevBuf<double4> nc_atoms;
+  DevBuf<int> nc_start;
+  DevBuf<double> lutd;                       // FP64 pair table quads of the refinement
+  DevBuf<double4> ref_geo;
+  DevBuf<uint32_t> ref_pairs;
+  DevBuf<vdr::RefHeader> ref_hdr;
+  DevBuf<double> ref_in, ref_out, ref_xyz;
+  DevBuf<double4> ref_res;
+  DevBuf<int> ref_lig, ref_evals;
+  DevBuf<long long> ref_coff, ref_xoff;
+  PostState post;
 };
 
 namespace {
@@ -124,6 +146,16 @@ void upload_lut(vd_ctx* c) {
   c->lutq.ensure(quads.size());
   HIP_CHECK(hipMemcpy(c->lut_rows.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(c->lutq.p, quads.data(), quads.size() * 4, hipMemcpyHostToDevice));
+  // FP64 quads (e_i, dor_i, e_i+1, dor_i+1) for the refinement after the search
+  std::vector<double> qd((size_t)VD_NUM_TYPE_PAIRS * VD_LUT_BINS * 4);
+  for (int tp = 0; tp < VD_NUM_TYPE_PAIRS; ++tp)
+    for (int b = 0; b < VD_LUT_BINS; ++b) {
+      const size_t o = (size_t)tp * VD_LUT_BINS, b1 = std::min(b + 1, VD_LUT_BINS - 1);
+      double* q = &qd[(o + b) * 4];
+      q[0] = c->pre.smooth_e[o + b]; q[1] = c->pre.smooth_dor[o + b]; q[2] = c->pre.smooth_e[o + b1]; q[3] = c->pre.smooth_dor[o + b1];
+    }
+  c->lutd.ensure(qd.size());
+  HIP_CHECK(hipMemcpy(c->lutd.p, qd.data(), qd.size() * 8, hipMemcpyHostToDevice));
 }
 
 vd_grid_desc make_desc(const vd_box& b) {
@@ -435,6 +467,259 @@ void prepare_batch(vd_ctx* c, const std::vector<LigPlan>& plans) {
 
 }  // namespace
 
+namespace {
+
+// Receptor cells of the direct evaluation after the search: the box is cut into cells of about
+// 2 A (n per axis exactly, so the cells tile it), and each cell lists every typed receptor atom
+// within the cutoff of its brick grown by 1e-3 A (a point the device's float arithmetic puts in
+// the cell is covered even when it lies a rounding error outside), as FP64 coordinates and XS
+// type.  Coarser cells when the lists would pass 2^26 entries (very large boxes).
+void build_nc(vd_ctx* c) {
+  if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no search box: call vd_grid_populate first");
+  if (c->rec_xs.empty()) throw ArgError(VD_ERR_ARG, "no receptor atoms: call vd_set_receptor first");
+  const vd_box& b = c->box;
+  const double grow = std::sqrt((double)VD_CUTOFF_SQR) + 1e-3;
+  std::vector<int> typed;
+  for (size_t i = 0; i < c->rec_xs.size(); ++i) {
+    if (c->rec_xs[i] >= VD_XS_SIZE) continue;
+    bool near = true;
+    for (int k = 0; k < 3; ++k) {
+      const double x = c->rec_xyz[3 * i + k];
+      if (x < b.begin[k] - grow || x > b.end[k] + grow) near = false;
+    }
+    if (near) typed.push_back((int)i);
+  }
+  for (double target = 2.0;; target *= 1.5) {
+    int n[3]; double span[3], cs[3];
+    for (int k = 0; k < 3; ++k) {
+      span[k] = b.end[k] - b.begin[k];
+      n[k] = std::max(1, (int)std::ceil(span[k] / target));
+      cs[k] = span[k] / n[k];
+    }
+    // bins of the same size over the box grown by the cutoff speed up the brick tests
+    int nb[3], reach[3];
+    for (int k = 0; k < 3; ++k) {
+      nb[k] = (int)std::ceil((span[k] + 2 * grow) / cs[k]) + 1;
+      reach[k] = (int)std::ceil(2 * grow / cs[k]) + 1;
+    }
+    std::vector<std::vector<int>> bins((size_t)nb[0] * nb[1] * nb[2]);
+    for (int i : typed) {
+      int bi[3];
+      for (int k = 0; k < 3; ++k)
+        bi[k] = std::min(nb[k] - 1, std::max(0, (int)std::floor((c->rec_xyz[3 * i + k] - (b.begin[k] - grow)) / cs[k])));
+      bins[((size_t)bi[2] * nb[1] + bi[1]) * nb[0] + bi[0]].push_back(i);
+    }
+    const long long ncell = (long long)n[0] * n[1] * n[2];
+    std::vector<int> start(ncell + 1, 0);
+    std::vector<double4> list;
+    bool too_big = false;
+    for (int z = 0; z < n[2] && !too_big; ++z)
+      for (int y = 0; y < n[1] && !too_big; ++y)
+        for (int x = 0; x < n[0]; ++x) {
+          const int id[3] = {x, y, z};
+          double lo[3], hi[3];
+          for (int k = 0; k < 3; ++k) { lo[k] = b.begin[k] + cs[k] * id[k]; hi[k] = b.begin[k] + cs[k] * (id[k] + 1); }
+          for (int bz = z; bz <= std::min(nb[2] - 1, z + reach[2]); ++bz)
+            for (int by = y; by <= std::min(nb[1] - 1, y + reach[1]); ++by)
+              for (int bx = x; bx <= std::min(nb[0] - 1, x + reach[0]); ++bx)
+                for (int i : bins[((size_t)bz * nb[1] + by) * nb[0] + bx]) {
+                  double dd = 0;
+                  for (int k = 0; k < 3; ++k) {
+                    const double v = c->rec_xyz[3 * i + k];
+                    const double d = v < lo[k] ? lo[k] - v : (v > hi[k] ? v - hi[k] : 0.0);
+                    dd += d * d;
+                  }
+                  if (dd >= grow * grow) continue;
+                  list.push_back(make_double4(c->rec_xyz[3 * i], c->rec_xyz[3 * i + 1], c->rec_xyz[3 * i + 2], (double)c->rec_xs[i]));
+                }
+          const long long cell = ((long long)z * n[1] + y) * n[0] + x;
+          start[cell + 1] = (int)list.size();
+          if (list.size() > (size_t)(1 << 26)) { too_big = true; break; }
+        }
+    if (too_big) continue;
+    c->nc_atoms.ensure(list.size());
+    c->nc_start.ensure(start.size());
+    if (!list.empty())
+      HIP_CHECK(hipMemcpy(c->nc_atoms.p, list.data(), list.size() * sizeof(double4), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(c->nc_start.p, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
+    vdr::NcD& k = c->nck;
+    k.atoms = c->nc_atoms.p; k.start = c->nc_start.p;
+    for (int j = 0; j < 3; ++j) {
+      k.lo[j] = b.begin[j]; k.hi[j] = b.end[j]; k.inv_cell[j] = n[j] / span[j]; k.n[j] = n[j];
+    }
+    c->nc_ready = true;
+    return;
+  }
+}
+
+void check_post(const vd_post_params* p) {
+  if (!p) throw ArgError(VD_ERR_ARG, "post params is NULL");
+  if (p->max_steps < 0) throw ArgError(VD_ERR_ARG, "negative max_steps");
+  if (p->local_search != VD_LS_BFGS && p->local_search != VD_LS_RILC_BFGS) throw ArgError(VD_ERR_ARG, "bad local_search");
+  if (p->num_modes < 1) throw ArgError(VD_ERR_ARG, "num_modes must be >= 1");
+  if (!(p->min_rmsd >= 0) || !(p->energy_range >= 0) || !(p->v >= 0) || !(p->slope >= 0))
+    throw ArgError(VD_ERR_ARG, "negative or NaN post-processing parameter");
+}
+
+// FP64 ligand record of the refinement (vdr::RefHeader): atom words (local xyz, node | xs << 8 |
+// heavy << 16), node origin words (rel_origin, (parent + 1) | begin << 8 | end << 16), node axis
+// words (rel_axis, subtree end); pair words in the conflict-free chunks of 64.
+vdr::RefHeader pack_ref(const vd::Ligand& L, std::vector<double4>& geo, std::vector<uint32_t>& pw, int max_steps) {
+  vdr::RefHeader h{};
+  h.A = (int)L.atoms.size(); h.N = (int)L.nodes.size(); h.T = L.n_torsions(); h.heavy = L.n_heavy;
+  h.max_steps = max_steps;
+  h.off_atoms = (long long)geo.size();
+  for (const vd::LigAtom& a : L.atoms) {
+    const uint32_t meta = (uint32_t)a.node | ((uint32_t)a.xs << 8) | ((a.el != 0 ? 1u : 0u) << 16);
+    geo.push_back(make_double4(a.local[0], a.local[1], a.local[2], (double)meta));
+  }
+  for (const vd::LigNode& nd : L.nodes)
+    geo.push_back(make_double4(nd.rel_origin[0], nd.rel_origin[1], nd.rel_origin[2],
+                               (double)((uint32_t)(nd.parent + 1) | ((uint32_t)nd.begin << 8) | ((uint32_t)nd.end << 16))));
+  for (const vd::LigNode& nd : L.nodes)
+    geo.push_back(make_double4(nd.rel_axis[0], nd.rel_axis[1], nd.rel_axis[2], (double)nd.subtree_end));
+  const std::vector<uint32_t>& ch = vd::pair_chunks(L, 64);
+  h.C = (int)(ch.size() / 64);
+  h.off_pairs = (long long)pw.size();
+  pw.insert(pw.end(), ch.begin(), ch.end());
+  return h;
+}
+
+// One refinement launch over poses of several ligands: ligand ids[k] owns poses[k] poses, their
+// conformations concatenated in conf.  Fills per pose the refined conformation, all-atom
+// coordinates and vd_refined.
+struct RefineOut {
+  std::vector<float> conf, coords;
+  std::vector<vd_refined> res;
+  std::vector<long long> coff, xoff;   // per pose offsets into conf / coords
+  unsigned long long evals = 0;
+  double ms = 0;
+};
+RefineOut run_refine(vd_ctx* c, const std::vector<int>& ids, const std::vector<int>& poses, const float* conf,
+                     const vd_post_params* p) {
+  check_post(p);
+  need_device(c);
+  if (!c->nc_ready) build_nc(c);
+  std::vector<double4> geo;
+  std::vector<uint32_t> pw;
+  std::vector<vdr::RefHeader> hdr;
+  int Amax = 1, Nmax = 1, nmax = 1;
+  for (int id : ids) {
+    const vd::Ligand& L = lig_at(c, id);
+    if (L.n_torsions() + 7 > 64) throw ArgError(VD_ERR_CAPACITY, "ligand has too many torsions for the refinement");
+    const int steps = p->max_steps > 0 ? p->max_steps : heur_bfgs((int)L.atoms.size(), p->local_search);
+    hdr.push_back(pack_ref(L, geo, pw, steps));
+    Amax = std::max(Amax, (int)L.atoms.size()); Nmax = std::max(Nmax, (int)L.nodes.size());
+    nmax = std::max(nmax, 6 + L.n_torsions());
+  }
+  RefineOut r;
+  std::vector<int> pose_lig;
+  long long nc = 0, nx = 0;
+  for (size_t k = 0; k < ids.size(); ++k) {
+    const vd::Ligand& L = lig_at(c, ids[k]);
+    if (poses[k] < 0) throw ArgError(VD_ERR_ARG, "negative pose count");
+    for (int i = 0; i < poses[k]; ++i) {
+      pose_lig.push_back((int)k);
+      r.coff.push_back(nc); r.xoff.push_back(nx);
+      nc += 7 + L.n_torsions(); nx += 3 * (long long)L.atoms.size();
+    }
+  }
+  const int n = (int)pose_lig.size();
+  r.conf.assign(nc, 0.0f); r.coords.assign(nx, 0.0f); r.res.assign(n, vd_refined{});
+  if (n == 0) return r;
+  std::vector<double> cin(nc);
+  for (long long i = 0; i < nc; ++i) {
+    if (!std::isfinite(conf[i])) throw ArgError(VD_ERR_ARG, "non-finite conformation value");
+    cin[i] = conf[i];
+  }
+  const int lds = vdl::refine_lds_bytes(Amax, Nmax, nmax);
+  if (lds > 64 * 1024) throw ArgError(VD_ERR_CAPACITY, "ligand too large for the refinement's LDS layout");
+  if (pw.empty()) pw.push_back(VD_NO_PAIR);
+  c->ref_geo.ensure(geo.size()); c->ref_pairs.ensure(pw.size()); c->ref_hdr.ensure(hdr.size());
+  c->ref_lig.ensure(n); c->ref_evals.ensure(n); c->ref_res.ensure(n);
+  c->ref_coff.ensure(n); c->ref_xoff.ensure(n);
+  c->ref_in.ensure(nc); c->ref_out.ensure(nc); c->ref_xyz.ensure(nx);
+  HIP_CHECK(hipMemcpyAsync(c->ref_geo.p, geo.data(), geo.size() * sizeof(double4), hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->ref_pairs.p, pw.data(), pw.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->ref_hdr.p, hdr.data(), hdr.size() * sizeof(vdr::RefHeader), hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->ref_lig.p, pose_lig.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->ref_coff.p, r.coff.data(), n * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->ref_xoff.p, r.xoff.data(), n * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->ref_in.p, cin.data(), nc * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  RefineArgs a{};
+  a.geo = c->ref_geo.p; a.pairs = c->ref_pairs.p; a.hdr = c->ref_hdr.p; a.pose_lig = c->ref_lig.p;
+  a.conf_off = c->ref_coff.p; a.coords_off = c->ref_xoff.p;
+  a.conf_in = c->ref_in.p; a.conf_out = c->ref_out.p; a.coords_out = c->ref_xyz.p;
+  a.res = c->ref_res.p; a.evals = c->ref_evals.p;
+  a.K.lut = reinterpret_cast<const double4*>(c->lutd.p);
+  a.K.nc = c->nck;
+  a.K.v = p->v;
+  a.final_slope = p->slope;
+  a.Amax = Amax; a.Nmax = Nmax; a.nmax = nmax;
+  HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+  HIP_CHECK(vdl::launch_refine(a, n, c->stream));
+  HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+  std::vector<double4> res(n);
+  std::vector<int> ev(n);
+  std::vector<double> cout_(nc), xout(nx);
+  HIP_CHECK(hipMemcpyAsync(cout_.data(), c->ref_out.p, nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemcpyAsync(xout.data(), c->ref_xyz.p, nx * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemcpyAsync(res.data(), c->ref_res.p, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemcpyAsync(ev.data(), c->ref_evals.p, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  r.ms = elapsed_ms(c);
+  for (long long i = 0; i < nc; ++i) r.conf[i] = (float)cout_[i];
+  for (long long i = 0; i < nx; ++i) r.coords[i] = (float)xout[i];
+  const double fmax = std::numeric_limits<float>::max();
+  for (int i = 0; i < n; ++i) {
+    r.res[i].e = res[i].x < fmax ? (float)res[i].x : std::numeric_limits<float>::max();
+    r.res[i].e_inter = (float)std::min(res[i].y, fmax); r.res[i].e_intra = (float)std::min(res[i].z, fmax);
+    r.res[i].inside = res[i].w != 0.0;
+    r.evals += (unsigned long long)ev[i];
+  }
+  return r;
+}
+
+std::vector<vd_mode> select_modes(vd_ctx* c, const vd::Ligand& L, int n, const vd_refined* r, const float* coords,
+                                  const vd_post_params* p) {
+  std::vector<float> e(n), ei(n), ea(n);
+  for (int i = 0; i < n; ++i) { e[i] = r[i].e; ei[i] = r[i].e_inter; ea[i] = r[i].e_intra; }
+  std::vector<vd::Mode> m = vd::select_modes(L, n, e.data(), ei.data(), ea.data(), coords, c->weights[5], p->num_modes,
+                                             p->energy_range, p->min_rmsd);
+  std::vector<vd_mode> out(m.size());
+  for (size_t i = 0; i < m.size(); ++i) {
+    out[i].e = (float)m[i].e; out[i].rmsd_lb = (float)m[i].lb; out[i].rmsd_ub = (float)m[i].ub; out[i].pose = m[i].pose;
+  }
+  return out;
+}
+
+// Refine + select for many ligands; the results replace c->post.
+void postprocess(vd_ctx* c, const std::vector<int>& ids, const std::vector<int>& poses, const float* conf,
+                 const vd_post_params* p, vd_stats* st) {
+  RefineOut r = run_refine(c, ids, poses, conf, p);
+  PostState ps;
+  size_t first = 0;
+  for (size_t k = 0; k < ids.size(); ++k) {
+    const vd::Ligand& L = lig_at(c, ids[k]);
+    const int A = (int)L.atoms.size(), T = L.n_torsions(), n = poses[k];
+    PostLigand pl;
+    pl.id = ids[k];
+    const float* xyz = n > 0 ? r.coords.data() + r.xoff[first] : nullptr;
+    pl.modes = select_modes(c, L, n, r.res.data() + first, xyz, p);
+    for (const vd_mode& m : pl.modes) {
+      const size_t q = first + m.pose;
+      pl.conf.insert(pl.conf.end(), r.conf.begin() + r.coff[q], r.conf.begin() + r.coff[q] + 7 + T);
+      pl.coords.insert(pl.coords.end(), r.coords.begin() + r.xoff[q], r.coords.begin() + r.xoff[q] + 3 * A);
+    }
+    ps.ligs.push_back(std::move(pl));
+    first += n;
+  }
+  c->post = std::move(ps);
+  if (st) { st->kernel_ms = r.ms; st->evals = r.evals; st->chains_per_wave = 1; st->blocks = (int)r.res.size(); }
+}
+
+}  // namespace
+
 // =============================================================================================
 extern "C" {
 
@@ -480,6 +765,9 @@ void vd_destroy(vd_ctx* c) {
   c->queue.release(); c->evals.release(); c->io_a.release(); c->io_b.release(); c->io_c.release();
   c->batch.e.release(); c->batch.conf.release(); c->batch.heavy.release();
   c->tune_e.release(); c->tune_conf.release(); c->tune_heavy.release(); c->hist.release();
+  c->nc_atoms.release(); c->nc_start.release(); c->ref_in.release(); c->ref_out.release(); c->ref_xyz.release();
+  c->ref_res.release(); c->ref_lig.release(); c->ref_evals.release(); c->ref_coff.release(); c->ref_xoff.release();
+  c->lutd.release(); c->ref_geo.release(); c->ref_pairs.release(); c->ref_hdr.release();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -520,16 +808,18 @@ int vd_set_receptor(vd_ctx* c, int n, const float* xyz, const int* xs) {
     c->rec_xyz.assign(xyz, xyz + 3 * (size_t)n);
     c->rec_xs.assign(xs, xs + n);
     c->maps_ready = false;
+    c->nc_ready = false;
   });
 }
 
 int vd_set_receptor_pdbqt(vd_ctx* c, const char* text, int* n_atoms) {
   if (!c || !text) return VD_ERR_ARG;
   return guard(c, [&]() {
-    std::vector<float> xyz; std::vector<int> xs;
+    std::vector<double> xyz; std::vector<int> xs;
     vd::parse_receptor_pdbqt(text, xyz, xs);
     c->rec_xyz.swap(xyz); c->rec_xs.swap(xs);
     c->maps_ready = false;
+    c->nc_ready = false;
     if (n_atoms) *n_atoms = (int)c->rec_xs.size();
   });
 }
@@ -538,7 +828,7 @@ int vd_get_receptor(vd_ctx* c, int max, float* xyz, int* xs) {
   if (!c) return VD_ERR_ARG;
   const int n = (int)c->rec_xs.size();
   for (int i = 0; i < std::min(n, max); ++i) {
-    if (xyz) for (int k = 0; k < 3; ++k) xyz[3 * i + k] = c->rec_xyz[3 * i + k];
+    if (xyz) for (int k = 0; k < 3; ++k) xyz[3 * i + k] = (float)c->rec_xyz[3 * i + k];
     if (xs) xs[i] = c->rec_xs[i];
   }
   return n;
@@ -612,6 +902,7 @@ int vd_grid_populate(vd_ctx* c, const vd_box* box) {
     HIP_CHECK(hipStreamSynchronize(c->stream));
     c->grid_ms = elapsed_ms(c);
     c->maps_ready = true;
+    c->nc_ready = false;
   });
 }
 
@@ -636,6 +927,7 @@ int vd_grid_upload(vd_ctx* c, const vd_box* box, const float* maps) {
     c->maps.ensure((size_t)VD_XS_SIZE * c->gd.plane);
     HIP_CHECK(hipMemcpy(c->maps.p, maps, (size_t)VD_XS_SIZE * c->gd.plane * 4, hipMemcpyHostToDevice));
     c->maps_ready = true;
+    c->nc_ready = false;
   });
 }
 
@@ -655,6 +947,7 @@ int vd_ligand_info_get(vd_ctx* c, int id, vd_ligand_info* info) {
     const vd::Ligand& L = lig_at(c, id);
     info->n_atoms = (int)L.atoms.size(); info->n_torsions = L.n_torsions(); info->n_pairs = (int)L.pairs.size();
     info->n_heavy = L.n_heavy; info->torsdof = L.torsdof; info->tree_depth = L.max_depth;
+    info->num_tors = (float)L.num_tors;
   });
 }
 
@@ -888,6 +1181,105 @@ int vd_cluster(int n, int n_heavy, const float* e, const float* heavy, double mi
   std::vector<int> k = vd::cluster_poses(n, n_heavy, e, heavy, min_rmsd, max_size);
   for (size_t i = 0; i < k.size(); ++i) kept[i] = k[i];
   return (int)k.size();
+}
+
+int vd_post_params_default(vd_post_params* p) {
+  if (!p) return VD_ERR_ARG;
+  p->max_steps = 0; p->local_search = VD_LS_RILC_BFGS; p->num_modes = 9; p->energy_range = 3.0f;
+  p->min_rmsd = 1.0f; p->v = 1000.0f; p->slope = 1e6f;
+  return VD_OK;
+}
+
+int vd_refine(vd_ctx* c, int id, int n, const float* conf_in, const vd_post_params* p, float* conf_out, float* coords_out,
+              vd_refined* out) {
+  if (!c || n < 0 || (n > 0 && !conf_in)) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    RefineOut r = run_refine(c, {id}, {n}, conf_in, p);
+    if (conf_out) std::memcpy(conf_out, r.conf.data(), r.conf.size() * sizeof(float));
+    if (coords_out) std::memcpy(coords_out, r.coords.data(), r.coords.size() * sizeof(float));
+    if (out) std::memcpy(out, r.res.data(), r.res.size() * sizeof(vd_refined));
+  });
+}
+
+int vd_select_modes(vd_ctx* c, int id, int n, const vd_refined* r, const float* coords, const vd_post_params* p,
+                    vd_mode* modes) {
+  if (!c || n < 0 || (n > 0 && (!r || !coords)) || !modes) return VD_ERR_ARG;
+  int count = 0;
+  const int rc = guard(c, [&]() {
+    check_post(p);
+    std::vector<vd_mode> m = select_modes(c, lig_at(c, id), n, r, coords, p);
+    std::copy(m.begin(), m.end(), modes);
+    count = (int)m.size();
+  });
+  return rc == VD_OK ? count : rc;
+}
+
+int vd_postprocess_batch(vd_ctx* c, int n, const int* ids, const int* poses, const float* conf, const vd_post_params* p,
+                         vd_stats* st) {
+  if (!c || n < 0 || (n > 0 && (!ids || !poses))) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    std::vector<int> vi(ids, ids + n), vp(poses, poses + n);
+    long long tot = 0;
+    for (int k = 0; k < n; ++k) tot += vp[k];
+    if (tot > 0 && !conf) throw ArgError(VD_ERR_ARG, "conf is NULL");
+    postprocess(c, vi, vp, conf, p, st);
+  });
+}
+
+int vd_postprocess_docked(vd_ctx* c, int max_poses, const vd_post_params* p, vd_stats* st) {
+  if (!c || max_poses < 1) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    need_device(c);
+    BatchState& b = c->batch;
+    std::vector<int> ids, poses;
+    std::vector<float> conf;
+    for (size_t k = 0; k < b.ids.size(); ++k) {
+      const size_t ch = b.chains[k];
+      const int T = b.T[k], H = b.H[k];
+      std::vector<float> e(ch), cf(ch * (7 + T)), hv(ch * H * 3);
+      const longlong4 o = b.off[k];
+      HIP_CHECK(hipMemcpy(e.data(), b.e.p + o.x, ch * 4, hipMemcpyDeviceToHost));
+      HIP_CHECK(hipMemcpy(cf.data(), b.conf.p + o.y, cf.size() * 4, hipMemcpyDeviceToHost));
+      HIP_CHECK(hipMemcpy(hv.data(), b.heavy.p + o.z, hv.size() * 4, hipMemcpyDeviceToHost));
+      std::vector<int> kept = vd::cluster_poses((int)ch, H, e.data(), hv.data(), 1.0, max_poses);
+      ids.push_back(b.ids[k]);
+      poses.push_back((int)kept.size());
+      for (int i : kept) conf.insert(conf.end(), cf.begin() + (size_t)i * (7 + T), cf.begin() + (size_t)(i + 1) * (7 + T));
+    }
+    postprocess(c, ids, poses, conf.data(), p, st);
+  });
+}
+
+int vd_post_results(vd_ctx* c, int k, vd_mode* modes, float* conf, float* coords) {
+  if (!c) return VD_ERR_ARG;
+  int count = 0;
+  const int rc = guard(c, [&]() {
+    if (k < 0 || k >= (int)c->post.ligs.size()) throw ArgError(VD_ERR_ARG, "post-processing index out of range");
+    const PostLigand& pl = c->post.ligs[k];
+    if (modes) std::copy(pl.modes.begin(), pl.modes.end(), modes);
+    if (conf) std::copy(pl.conf.begin(), pl.conf.end(), conf);
+    if (coords) std::copy(pl.coords.begin(), pl.coords.end(), coords);
+    count = (int)pl.modes.size();
+  });
+  return rc == VD_OK ? count : rc;
+}
+
+int vd_write_pdbqt(vd_ctx* c, int id, int n, const vd_mode* modes, const float* coords, char* out, int cap) {
+  if (!c || n < 0 || (n > 0 && (!modes || !coords)) || cap < 0 || (cap > 0 && !out)) return VD_ERR_ARG;
+  int len = 0;
+  const int rc = guard(c, [&]() {
+    std::vector<double> e(n), lb(n), ub(n);
+    for (int i = 0; i < n; ++i) { e[i] = modes[i].e; lb[i] = modes[i].rmsd_lb; ub[i] = modes[i].rmsd_ub; }
+    const std::string s = vd::write_models(lig_at(c, id), n, e.data(), lb.data(), ub.data(), coords);
+    if (s.size() > (size_t)0x7FFFFFFF) throw ArgError(VD_ERR_CAPACITY, "output too large");
+    len = (int)s.size();
+    if (cap > 0) {
+      const size_t m = std::min<size_t>(s.size(), (size_t)cap - 1);
+      std::memcpy(out, s.data(), m);
+      out[m] = '\0';
+    }
+  });
+  return rc == VD_OK ? len : rc;
 }
 
 int vd_device_alloc(vd_ctx* c, size_t bytes, void** ptr) {

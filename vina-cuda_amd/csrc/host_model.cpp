@@ -163,7 +163,7 @@ void Precalculated::pair_quads(std::vector<float>& out) const {
 // ---------------------------------------------------------------------------------------
 // Receptor (V11/lib/parse_pdbqt.cpp:258-284 + model.cpp:644-797 with every pair FIXED).
 // Bond test: r^2 < (1.1 (cov_i + cov_j))^2 and no third atom nearer to both ends.
-void parse_receptor_pdbqt(const std::string& text, std::vector<float>& xyz, std::vector<int>& xs) {
+void parse_receptor_pdbqt(const std::string& text, std::vector<double>& xyz, std::vector<int>& xs) {
   std::vector<TypedAtom> at;
   std::istringstream in(text);
   std::string s;
@@ -209,7 +209,7 @@ void parse_receptor_pdbqt(const std::string& text, std::vector<float>& xyz, std:
   }
   assign_xs(at, [&](size_t i) -> const std::vector<int>& { return bonds[i]; });
   xyz.resize(3 * (size_t)n); xs.resize(n);
-  for (int i = 0; i < n; ++i) { for (int k = 0; k < 3; ++k) xyz[3 * i + k] = (float)at[i].xyz[k]; xs[i] = at[i].xs; }
+  for (int i = 0; i < n; ++i) { for (int k = 0; k < 3; ++k) xyz[3 * i + k] = at[i].xyz[k]; xs[i] = at[i].xs; }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -218,7 +218,7 @@ void parse_receptor_pdbqt(const std::string& text, std::vector<float>& xyz, std:
 // the rotation axis) and is the child's origin (V11/lib/parse_pdbqt.cpp:157-245, 406-447).
 namespace {
 struct Frag {
-  struct Entry { TypedAtom a; unsigned serial; std::string line; std::vector<std::unique_ptr<Frag>> kids; };
+  struct Entry { TypedAtom a; unsigned serial; int ctx_line; std::string line; std::vector<std::unique_ptr<Frag>> kids; };
   std::vector<Entry> entries;
   int axis_atom = -1;       // entry index of the BRANCH 'to' atom (-1 for ROOT)
   int from_index = -1, to_index = -1;  // ligand atom indices once flattened
@@ -244,7 +244,7 @@ struct LigandReader {
     return false;
   }
   void atom_into(Frag& f, const std::string& s, unsigned to) {
-    Frag::Entry e; read_atom_record(s, e.a, e.serial); e.line = s;
+    Frag::Entry e; read_atom_record(s, e.a, e.serial); e.line = s; e.ctx_line = (int)ctx->size() - 1;
     if (e.serial == to && to != 0) f.axis_atom = (int)f.entries.size();
     f.entries.push_back(std::move(e));
   }
@@ -280,7 +280,7 @@ struct Flattener {
   explicit Flattener(Ligand& l) : L(l) {}
   int push(Frag::Entry& e, const double* origin) {
     for (auto& k : e.kids) k->from_index = (int)L.atoms.size();
-    LigAtom a; a.ad = e.a.ad; a.xs = e.a.xs; a.serial = e.serial; a.line = e.line;
+    LigAtom a; a.ad = e.a.ad; a.xs = e.a.xs; a.serial = e.serial; a.line = e.line; a.ctx_line = e.ctx_line;
     for (int k = 0; k < 3; ++k) { a.xyz[k] = e.a.xyz[k]; a.local[k] = e.a.xyz[k] - origin[k]; }
     L.atoms.push_back(a);
     return (int)L.atoms.size() - 1;
@@ -390,6 +390,17 @@ Ligand parse_ligand_pdbqt(const std::string& text) {
   for (int i = 0; i < n; ++i) {
     L.atoms[i].xs = ta[i].xs; L.atoms[i].el = ta[i].el;
     if (ta[i].el != E_H) L.n_heavy++;
+  }
+  // num_tors of conf_independent_inputs (V11/lib/terms.cpp:54-125): half of each heavy atom's
+  // rotatable bonds to heavy atoms that have another heavy neighbour
+  auto heavy_neighbours = [&](int a) { int k = 0; for (int b : L.atoms[a].bonds) if (L.atoms[b].el != E_H) ++k; return k; };
+  L.num_tors = 0;
+  for (int i = 0; i < n; ++i) {
+    if (L.atoms[i].el == E_H) continue;
+    int rot = 0;
+    for (int b : L.atoms[i].bonds)
+      if (movable(i, b) == 1 && L.atoms[b].el != E_H && heavy_neighbours(b) > 1) ++rot;
+    L.num_tors += 0.5 * rot;
   }
   // intramolecular pairs: variable distance, more than 3 bonds apart, both typed
   // (model.cpp:828-896; the neighbourhood is the depth-limited DFS of model::bonded_to)

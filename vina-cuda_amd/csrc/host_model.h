@@ -49,6 +49,7 @@ struct LigAtom {
   double local[3] = {0, 0, 0};   // relative to the owning node's origin
   int node = 0;
   unsigned serial = 0;
+  int ctx_line = -1;             // index of its line in Ligand::context
   std::string line;              // the PDBQT line, for writing poses back out
   std::vector<int> bonds;
 };
@@ -61,7 +62,8 @@ struct Ligand {
   std::vector<LigNode> nodes;      // preorder, node 0 = ROOT
   std::vector<uint32_t> pairs;     // a | b << 8 | tp << 16
   int torsdof = 0, n_heavy = 0, max_depth = 0;
-  std::vector<std::string> context; // REMARK/ROOT/BRANCH lines in file order (for output)
+  std::vector<std::string> context; // every input line in file order (for output)
+  double num_tors = 0;             // torsion count of the final score's divisor (terms.cpp:97-125)
   int n_torsions() const { return (int)nodes.size() - 1; }
   // conflict-free pair chunks per chain-group width (16, 32, 64), built on first use
   mutable std::vector<uint32_t> chunked[3];
@@ -75,7 +77,7 @@ const std::vector<uint32_t>& pair_chunks(const Ligand& L, int G);
 
 Ligand parse_ligand_pdbqt(const std::string& text);
 // Typed receptor atoms (rigid receptor only): xyz[3n], xs[n] (17 = no XS type)
-void parse_receptor_pdbqt(const std::string& text, std::vector<float>& xyz, std::vector<int>& xs);
+void parse_receptor_pdbqt(const std::string& text, std::vector<double>& xyz, std::vector<int>& xs);
 
 // Appends a ligand's packed record to a device blob (16-byte words), with its pairs chunked for
 // chain groups of G lanes; returns the header.
@@ -90,5 +92,19 @@ std::string synth_ligand_pdbqt(int n_heavy, int n_rot, uint64_t seed);
 // Feeds results in the given order; returns indices kept, sorted by energy.
 std::vector<int> cluster_poses(int n, int heavy, const float* e, const float* heavy_xyz,
                                double min_rmsd, int max_size);
+
+// After the search (post.cpp).  conf_independent: the final score's torsion divisor
+// (num_tors_div, V11/lib/everything.cpp:296-303).  rmsd_bounds: rmsd lower/upper bound between
+// two all-atom poses (model.cpp:1453-1521).  select_modes: sorting, eval_adjusted,
+// remove_redundant and the num_modes / energy_range cut of V11/main/main.cpp:425-470 over n
+// refined poses (coords[n][A][3]).  write_models: the PDBQT output of n modes.
+struct Mode { int pose = 0; double e = 0, lb = 0, ub = 0; };
+double conf_independent(double e, double num_tors, double w_rot);
+void rmsd_bounds(const Ligand& L, const float* x, const float* y, double& lb, double& ub);
+std::vector<Mode> select_modes(const Ligand& L, int n, const float* e_refined, const float* e_inter,
+                               const float* e_intra, const float* coords, double w_rot, int num_modes,
+                               double energy_range, double min_rmsd);
+std::string write_models(const Ligand& L, int n, const double* e, const double* lb, const double* ub,
+                         const float* coords);
 
 }  // namespace vd

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include "vd_common.h"
 #include "dock_device.hpp"
+#include "refine_device.hpp"
 
 // Receptor cell list for the grid kernel: atoms sorted by cell (x fastest), one (begin, end)
 // range per cell; cells of `cell` A cover the box grown by the cutoff, starting at `origin`.
@@ -38,7 +39,26 @@ struct EvalArgs {
   float* e; float* grad;
 };
 
+// Refinement launch (vd_refine_kernel, FP64): one pose per workgroup.
+struct RefineArgs {
+  const double4* geo;               // ligand atom and node words (refine_device.hpp RefHeader)
+  const uint32_t* pairs;            // pair chunks of 64 words
+  const vdr::RefHeader* hdr;        // per ligand slot; max_steps = the step budget per stage
+  const int* pose_lig;              // ligand slot per pose
+  const long long* conf_off;        // per pose: offset of its conformation (7 + T doubles)
+  const long long* coords_off;      // per pose: offset of its coordinates (3 A doubles)
+  const double* conf_in;
+  double* conf_out; double* coords_out;
+  double4* res;                     // per pose: (refined e, inter, intra, inside the box)
+  int* evals;                       // per pose: evaluations
+  vdr::Consts K;
+  double final_slope;
+  int Amax, Nmax, nmax;
+};
+
 namespace vdl {
+int refine_lds_bytes(int Amax, int Nmax, int nmax);
+hipError_t launch_refine(const RefineArgs& a, int poses, hipStream_t s);
 int mc_lds_bytes(int G, int Amax, int Nmax, int dense);
 int max_resident_mc_blocks(int G, int W, int mode, int lds_bytes);
 hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& gc, const float4* lut_rows, float* maps,

@@ -7,8 +7,12 @@
 //  vd_mc_kernel       Monte Carlo + local search chains (kernel2_update, V11/inc/cuda/kernel2.cu:219-564),
 //                     either one work item per workgroup (plain per-ligand launch) or a persistent
 //                     grid pulling (ligand, chain block) items from a device queue.
+//  vd_refine_kernel   refinement of the kept poses against the receptor atoms and the parts of
+//                     their final score (refine_structure + eval_adjusted, V11/main/main.cpp:139-157,
+//                     425-440; the reference runs these on the CPU).
 #include <hip/hip_runtime.h>
 #include "dock_device.hpp"
+#include "refine_device.hpp"
 #include "kernels.h"
 
 using namespace vdk;
@@ -461,9 +465,65 @@ __global__ __launch_bounds__(64) void vd_eval_kernel(EvalArgs a) {
   }
 }
 
+// ------------------------------------------------------------------------------------------
+// After the search: refine_structure (V11/main/main.cpp:139-157) of one pose per workgroup (one
+// wavefront), in FP64: dense BFGS on the receptor atoms themselves with the out-of-box slope
+// raised 1e2 -> 1e10 until the heavy atoms lie within the box (a pose that never gets inside
+// reports max_fl), then the parts of the final score at the refined pose (fast tables, slope
+// a.final_slope).
+__global__ __launch_bounds__(64) void vd_refine_kernel(RefineArgs a) {
+  extern __shared__ double lds_d[];
+  const int pose = blockIdx.x, l = threadIdx.x;
+  const vdr::RefHeader h = a.hdr[a.pose_lig[pose]];
+  vdr::Pose P;
+  vdr::carve(P, lds_d, a.Amax, a.Nmax);
+  P.node_o = a.geo + h.off_atoms + h.A;
+  P.node_a = a.geo + h.off_atoms + h.A + h.N;
+  P.pairs = a.pairs + h.off_pairs;
+  P.A = h.A; P.N = h.N; P.T = h.T; P.C = h.C; P.n = 6 + h.T;
+  for (int i = l; i < h.A; i += 64) const_cast<double4*>(P.atom)[i] = a.geo[h.off_atoms + i];
+  const double* ci = a.conf_in + a.conf_off[pose];
+  vdr::Conf x;
+  x.pos = vdr::D3{ci[0], ci[1], ci[2]};
+  x.q = vdr::DQ{ci[3], ci[4], ci[5], ci[6]};
+  x.tor = (l >= 6 && l < 6 + h.T) ? ci[7 + (l - 6)] : 0.0;
+  __syncthreads();
+  double e = 0, slope = 100;
+  bool inside = false;
+  int evals = 0;
+  for (int p = 0; p < 5; ++p, slope *= 100) {
+    e = vdr::bfgs(P, a.K, slope, x, h.max_steps, l, evals);
+    vdr::set_coords(P, x, l);
+    inside = vdr::within(P, a.K, l);
+    if (inside) break;
+  }
+  if (!inside) e = vdr::kMaxFl;
+  const double e_inter = vdr::lane_sum<64>(vdr::nc_inter<true>(P, a.K, a.final_slope, l));
+  const double e_intra = vdr::lane_sum<64>(vdr::pair_terms<true>(P, a.K, l));
+  double* co = a.conf_out + a.conf_off[pose];
+  if (l == 0) {
+    co[0] = x.pos.x; co[1] = x.pos.y; co[2] = x.pos.z; co[3] = x.q.a; co[4] = x.q.b; co[5] = x.q.c; co[6] = x.q.d;
+    a.res[pose] = make_double4(e, e_inter, e_intra, inside ? 1.0 : 0.0);
+    a.evals[pose] = evals;
+  }
+  if (l >= 6 && l < 6 + h.T) co[7 + (l - 6)] = x.tor;
+  double* xyz = a.coords_out + a.coords_off[pose];
+  for (int i = l; i < h.A; i += 64) {
+    const double4 y = P.xyz[i];
+    xyz[3 * i] = y.x; xyz[3 * i + 1] = y.y; xyz[3 * i + 2] = y.z;
+  }
+}
+
 }  // namespace
 
 namespace vdl {
+
+int refine_lds_bytes(int Amax, int Nmax, int nmax) { return 8 * vdr::pose_lds_doubles(Amax, Nmax, nmax); }
+
+hipError_t launch_refine(const RefineArgs& a, int poses, hipStream_t s) {
+  hipLaunchKernelGGL(vd_refine_kernel, dim3(poses), dim3(64), refine_lds_bytes(a.Amax, a.Nmax, a.nmax), s, a);
+  return hipGetLastError();
+}
 
 int mc_lds_bytes(int G, int Amax, int Nmax, int dense) {
   return 4 * (lig_floats(Amax, Nmax) + (64 / G) * group_floats(Amax, Nmax, G, dense));

@@ -30,6 +30,8 @@ EXPORTS = (
     "vd_dock_batch", "vd_dock_batch_plain", "vd_batch_results", "vd_batch_best", "vd_cluster",
     "vd_synth_receptor", "vd_synth_ligand", "vd_device_alloc", "vd_device_free", "vd_copy", "vd_random_confs",
     "vd_last_launches", "vd_ligand_pair_chunks",
+    "vd_post_params_default", "vd_refine", "vd_select_modes", "vd_postprocess_batch", "vd_postprocess_docked",
+    "vd_post_results", "vd_write_pdbqt",
 )
 
 
@@ -52,7 +54,7 @@ class Box(C.Structure):
 
 class LigandInfo(C.Structure):
     _fields_ = [("n_atoms", C.c_int), ("n_torsions", C.c_int), ("n_pairs", C.c_int), ("n_heavy", C.c_int),
-                ("torsdof", C.c_int), ("tree_depth", C.c_int)]
+                ("torsdof", C.c_int), ("tree_depth", C.c_int), ("num_tors", C.c_float)]
 
 
 class DockParams(C.Structure):
@@ -65,6 +67,19 @@ class DockParams(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("evals", C.c_ulonglong), ("chains_per_wave", C.c_int), ("blocks", C.c_int)]
+
+
+class PostParams(C.Structure):
+    _fields_ = [("max_steps", C.c_int), ("local_search", C.c_int), ("num_modes", C.c_int), ("energy_range", C.c_float),
+                ("min_rmsd", C.c_float), ("v", C.c_float), ("slope", C.c_float)]
+
+
+class Refined(C.Structure):
+    _fields_ = [("e", C.c_float), ("e_inter", C.c_float), ("e_intra", C.c_float), ("inside", C.c_int)]
+
+
+class Mode(C.Structure):
+    _fields_ = [("e", C.c_float), ("rmsd_lb", C.c_float), ("rmsd_ub", C.c_float), ("pose", C.c_int)]
 
 
 class Launch(C.Structure):
@@ -127,6 +142,14 @@ def load():
         "vd_random_confs": (C.c_int, [vp, C.c_int, C.c_longlong, C.c_uint32, vp]),
         "vd_synth_receptor": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_uint64, fp, ip]),
         "vd_synth_ligand": (C.c_int, [C.c_int, C.c_int, C.c_uint64, C.c_char_p, C.c_int]),
+        "vd_post_params_default": (C.c_int, [C.POINTER(PostParams)]),
+        "vd_refine": (C.c_int, [vp, C.c_int, C.c_int, fp, C.POINTER(PostParams), fp, fp, C.POINTER(Refined)]),
+        "vd_select_modes": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(Refined), fp, C.POINTER(PostParams),
+                                      C.POINTER(Mode)]),
+        "vd_postprocess_batch": (C.c_int, [vp, C.c_int, ip, ip, fp, C.POINTER(PostParams), C.POINTER(Stats)]),
+        "vd_postprocess_docked": (C.c_int, [vp, C.c_int, C.POINTER(PostParams), C.POINTER(Stats)]),
+        "vd_post_results": (C.c_int, [vp, C.c_int, C.POINTER(Mode), fp, fp]),
+        "vd_write_pdbqt": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(Mode), fp, C.c_char_p, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -380,6 +403,80 @@ class Context:
         arr = (Launch * max(n, 1))()
         self._check(min(self.lib.vd_last_launches(self.h, n, arr), 0))
         return list(arr[:n])
+
+    # -- after the search: refinement, final scores, output
+    def refine(self, lid, conf, params=None):
+        """refine_structure of every row of conf -> (conf, all-atom coords, list of Refined)."""
+        p = params or post_params()
+        inf = self.ligand_info(lid)
+        c = np.ascontiguousarray(conf, dtype=np.float32).reshape(-1, 7 + inf.n_torsions)
+        n = len(c)
+        co = np.zeros_like(c)
+        xyz = np.zeros((n, inf.n_atoms, 3), dtype=np.float32)
+        res = (Refined * max(n, 1))()
+        self._check(self.lib.vd_refine(self.h, lid, n, _f(c), C.byref(p), _f(co), _f(xyz), res))
+        return co, xyz, list(res[:n])
+
+    def select_modes(self, lid, refined, coords, params=None):
+        """Mode selection over refined poses (host only) -> list of Mode."""
+        p = params or post_params()
+        n = len(refined)
+        r = (Refined * max(n, 1))(*refined)
+        xyz = np.ascontiguousarray(coords, dtype=np.float32)
+        modes = (Mode * max(p.num_modes, 1))()
+        k = self.lib.vd_select_modes(self.h, lid, n, r, _f(xyz), C.byref(p), modes)
+        self._check(min(k, 0))
+        return list(modes[:k])
+
+    def postprocess(self, ids, confs, params=None):
+        """Refinement + selection for several ligands in one launch; confs[k] = poses of ids[k]."""
+        p = params or post_params()
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        counts = np.array([len(np.atleast_2d(c)) if len(c) else 0 for c in confs], dtype=np.int32)
+        flat = np.concatenate([np.asarray(c, dtype=np.float32).reshape(-1) for c in confs]) if len(confs) else \
+            np.zeros(1, dtype=np.float32)
+        flat = np.ascontiguousarray(flat, dtype=np.float32)
+        st = Stats()
+        self._check(self.lib.vd_postprocess_batch(self.h, len(ids), _i(ids), _i(counts), _f(flat), C.byref(p), C.byref(st)))
+        self._post_ids = list(ids)
+        return st
+
+    def postprocess_docked(self, max_poses=20, params=None):
+        p = params or post_params()
+        st = Stats()
+        self._check(self.lib.vd_postprocess_docked(self.h, max_poses, C.byref(p), C.byref(st)))
+        self._post_ids = list(self._batch_ids)
+        return st
+
+    def post_results(self, k, num_modes=9):
+        """Modes of ligand k of the last post-processing call -> (modes, conf, coords)."""
+        self._check(min(self.lib.vd_post_results(self.h, k, None, None, None), 0))
+        inf = self.ligand_info(self._post_ids[k])
+        modes = (Mode * num_modes)()
+        conf = np.zeros((num_modes, 7 + inf.n_torsions), dtype=np.float32)
+        xyz = np.zeros((num_modes, inf.n_atoms, 3), dtype=np.float32)
+        m = self.lib.vd_post_results(self.h, k, modes, _f(conf), _f(xyz))
+        self._check(min(m, 0))
+        return list(modes[:m]), conf[:m], xyz[:m]
+
+    def write_pdbqt(self, lid, modes, coords):
+        """PDBQT text of the given modes (coords[m][n_atoms][3])."""
+        n = len(modes)
+        arr = (Mode * max(n, 1))(*modes)
+        xyz = np.ascontiguousarray(coords, dtype=np.float32)
+        size = self.lib.vd_write_pdbqt(self.h, lid, n, arr, _f(xyz), None, 0)
+        self._check(min(size, 0))
+        buf = C.create_string_buffer(size + 1)
+        self._check(min(self.lib.vd_write_pdbqt(self.h, lid, n, arr, _f(xyz), buf, size + 1), 0))
+        return buf.value.decode()
+
+
+def post_params(**kw):
+    p = PostParams()
+    load().vd_post_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
 
 
 def version():
