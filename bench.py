@@ -13,6 +13,9 @@ Printed beside it on the same JSON line:
   adjoint           forward + adjoint evaluations of 2^20 random conformations of a 40-atom,
                     12-torsion ligand (evals/s)
   roofline          the dominant kernel (the MC kernel) against HBM peak, algorithmic bytes
+  post              the step after the search for the docked batch: pose lists, FP64 refinement of
+                    every pose against the receptor atoms in one launch, final scores; beside it
+                    the oracle's FP64 refinement of a sample of the same poses on host threads
   large_box         large-box batched screening with the QuickVina2 policy (20,000-atom receptor,
                     188^3 maps, 4,096 ligands of 10-80 heavy atoms, 128 chains)
   cpu_baseline      the FP64 oracle's Monte Carlo (same schedule, same random stream) on host
@@ -65,7 +68,7 @@ def dominant_kernel(launches):
         d["n"] += 1; d["ms"] += rec.kernel_ms; d["bytes"] += rec.eval_bytes; d["evals"] += rec.evals
     name = max(by, key=lambda k: by[k]["ms"])
     d = by[name]
-    return name, d, {k: round(v["ms"], 2) for k, v in by.items()}
+    return name, d, by
 
 
 def large_box_leg(vdock, log, n_ligands=4096, chains=128):
@@ -109,16 +112,15 @@ def traffic_from_profile(kernel, algorithmic_bytes):
     """Memory traffic (bytes) of one launch of `kernel`: the bytes moved per algorithmic byte that
     the committed counter runs (FETCH_SIZE and WRITE_SIZE, calibrated on a plain copy) measured for
     the same kernel on this workload, times this launch's algorithmic bytes; None if the committed
-    profile is for another kernel."""
+    profile has no figure for this kernel."""
     path = os.path.join(ROOT, "profiles", "vd_mc_kernel_batched_screening.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
         if line.startswith("<!-- traffic:"):
             fields = dict(f.split("=", 1) for f in line.strip("<!-> \n").split()[1:] if "=" in f)
-            if fields.get("kernel") != kernel.replace(" ", ""):
-                return None
-            return round(float(fields["bytes_per_algorithmic_byte"]) * algorithmic_bytes)
+            if fields.get("kernel") == kernel.replace(" ", ""):
+                return round(float(fields["bytes_per_algorithmic_byte"]) * algorithmic_bytes)
     return None
 
 
@@ -151,6 +153,53 @@ def run_cpu_baseline(vdock, texts, box, xyz, xs, chains, budget_s=15.0):
                 evals_per_s=evals / t_mc)
 
 
+def post_leg(vdock, ctx, log, texts, box, xyz, xs, max_poses=20, cpu_budget_s=10.0):
+    """The step after the search for every ligand of the last docking batch: each ligand's chains
+    reduced to at most max_poses poses (add_to_output_container), every pose refined against the
+    receptor atoms in one FP64 launch, final scores and mode selection.  Beside it, the oracle's
+    FP64 refinement (the reference runs this step on the CPU) of a bounded sample of the same
+    poses on host threads."""
+    log("post: refinement + final scores of the docked batch")
+    t0 = time.perf_counter()
+    st = ctx.postprocess_docked(max_poses)
+    wall = time.perf_counter() - t0
+    n_lig = len(ctx._post_ids)
+    modes = [ctx.post_results(k)[0] for k in range(n_lig)]
+    best = [m[0].e for m in modes if m]
+    out = {"ligands": n_lig, "poses": st.blocks, "kernel_ms": round(st.kernel_ms, 2), "wall_ms": round(1000 * wall, 2),
+           "poses_per_s_kernel": round(st.blocks / (st.kernel_ms / 1000.0), 1),
+           "ligands_per_s_wall": round(n_lig / wall, 2), "evals": int(st.evals),
+           "evals_per_s_kernel": round(st.evals / (st.kernel_ms / 1000.0), 1),
+           "mean_best_final_score": round(float(np.mean(best)), 4), "dtype": "f64"}
+    # CPU: the oracle's refine_structure on the same poses, ctypes calls on host threads
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    from concurrent.futures import ThreadPoolExecutor
+    b, e, n = box.as_tuple()
+    grid = orc.Grid(orc.Lut(), b, e, n)
+    grid.set_receptor(xyz.astype(np.float64), xs)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    jobs = []
+    for k, lid in enumerate(ctx._post_ids[:64]):
+        lig = orc.Ligand(texts[k])
+        _, conf, _ = ctx.post_results(k)
+        for c in conf:
+            jobs.append((lig, c.astype(np.float64)))
+    done, t_cpu = 0, 0.0
+    with ThreadPoolExecutor(threads) as pool:
+        t1 = time.perf_counter()
+        for chunk in range(0, len(jobs), threads):
+            list(pool.map(lambda j: orc.refine(grid, j[0], j[1], orc.refine_steps(j[0])), jobs[chunk:chunk + threads]))
+            done = min(len(jobs), chunk + threads)
+            if time.perf_counter() - t1 >= cpu_budget_s:
+                break
+        t_cpu = time.perf_counter() - t1
+    out["cpu_refine"] = {"poses_per_s": round(done / t_cpu, 2), "cores": threads, "kind": "port",
+                         "sample": f"{done} refined poses of the first ligands, FP64 oracle refine_structure"}
+    out["speedup_vs_cpu_poses_per_s"] = round(out["poses_per_s_kernel"] / out["cpu_refine"]["poses_per_s"], 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +211,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-large-box", action="store_true", help="skip the large-box QuickVina2 leg")
+    ap.add_argument("--no-post", action="store_true", help="skip the refinement + final-score leg")
+    ap.add_argument("--grid-only", action="store_true",
+                    help="only the 17-map precompute: screening scene x5, large-box scene x3 (for a kernel profile)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, copy 256 MiB device to device twice (reference dispatch for HBM counters)")
     args = ap.parse_args()
@@ -172,6 +224,15 @@ def main():
 
     import vdock
     ctx = vdock.Context(0)
+    if args.grid_only:
+        xyz, xs, box, _ = build_scene(vdock, ctx)
+        ms = [ctx.grid_populate(box) for _ in range(5)]
+        x2, s2 = vdock.synth_receptor(20000, 90.0, 25.0, 4242)
+        ctx.set_receptor(x2, s2)
+        b2 = vdock.box_from_center([0, 0, 0], [70.0, 70.0, 70.0], 0.375)
+        ms2 = [ctx.grid_populate(b2) for _ in range(3)]
+        print(json.dumps({"grid_only": True, "screening_ms": ms, "large_box_ms": ms2}), flush=True)
+        return
     log("scene: receptor + maps")
     xyz, xs, box, _ = build_scene(vdock, ctx)
     log("ligands")
@@ -195,6 +256,7 @@ def main():
     wall = sum(times)
     value = args.ligands * args.steps / wall
     best = ctx.batch_best()
+    post = None if args.no_post else post_leg(vdock, ctx, log, texts, box, xyz, xs)
     out = {
         "metric": "ligands/sec on 1 MI355X, batched vs per-ligand at equal score acc; grid GB/s",
         "value": round(value, 3),
@@ -218,7 +280,8 @@ def main():
         "evals_per_s": round(evals / (kernel_ms / 1000.0), 1),
         "mean_best_score": round(float(np.mean(best)), 4),
     }
-    name, d, ms_by_kernel = dominant_kernel(launches)
+    name, d, by_kernel = dominant_kernel(launches)
+    ms_by_kernel = {k: round(v["ms"], 2) for k, v in by_kernel.items()}
     traffic = traffic_from_profile(name, d["bytes"] / d["n"])
     avg_ms = d["ms"] / d["n"]
     per_launch = d["bytes"] / d["n"]
@@ -227,8 +290,12 @@ def main():
                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                        "traffic": traffic, "launches": d["n"], "avg_launch_ms": round(avg_ms, 3),
                        "algorithmic_bytes_per_launch": round(per_launch), "evals_per_launch": d["evals"] // d["n"],
-                       "kernel_ms_by_instantiation": ms_by_kernel, "mc_launches_per_step": len(launches) // args.steps}
+                       "kernel_ms_by_instantiation": ms_by_kernel, "mc_launches_per_step": len(launches) // args.steps,
+                       "algorithmic_bytes_by_instantiation": {k: round(v["bytes"] / v["n"]) for k, v in by_kernel.items()},
+                       "launches_by_instantiation": {k: v["n"] for k, v in by_kernel.items()}}
 
+    if post is not None:
+        out["post"] = post
     if not args.no_extras:
         # per-ligand plain launch loop (one kernel_monte-style launch per ligand) on the same
         # ligands and parameters as the headline step; equal accuracy = same best scores

@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <mutex>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -50,6 +53,7 @@ struct DevBuf {
 struct BatchState {   // results of the last batch, kept on the device
   std::vector<int> ids;
   std::vector<longlong4> off;      // per slot: e, conf, heavy offsets
+  longlong4 total{};               // element counts of e, conf, heavy
   std::vector<int> T, H, chains;
   DevBuf<float> e, conf, heavy;
 };
@@ -463,6 +467,25 @@ void prepare_batch(vd_ctx* c, const std::vector<LigPlan>& plans) {
     oe += pl.chains; oc += (long long)pl.chains * (7 + pl.T); oh += (long long)pl.chains * pl.H * 3;
   }
   b.e.ensure(oe); b.conf.ensure(oc); b.heavy.ensure(oh);
+  b.total = make_longlong4(oe, oc, oh, 0);
+}
+
+// f(0..n-1) on up to 16 host threads (independent items); rethrows the first failure.
+void parallel_for(int n, const std::function<void(int)>& f) {
+  const int threads = std::min({n, 16, (int)std::max(1u, std::thread::hardware_concurrency())});
+  if (threads <= 1) { for (int i = 0; i < n; ++i) f(i); return; }
+  std::atomic<int> next{0};
+  std::exception_ptr failure;
+  std::mutex m;
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t)
+    pool.emplace_back([&]() {
+      for (int i; (i = next++) < n;) {
+        try { f(i); } catch (...) { std::lock_guard<std::mutex> g(m); if (!failure) failure = std::current_exception(); }
+      }
+    });
+  for (std::thread& t : pool) t.join();
+  if (failure) std::rethrow_exception(failure);
 }
 
 }  // namespace
@@ -698,22 +721,22 @@ void postprocess(vd_ctx* c, const std::vector<int>& ids, const std::vector<int>&
                  const vd_post_params* p, vd_stats* st) {
   RefineOut r = run_refine(c, ids, poses, conf, p);
   PostState ps;
-  size_t first = 0;
-  for (size_t k = 0; k < ids.size(); ++k) {
-    const vd::Ligand& L = lig_at(c, ids[k]);
+  ps.ligs.resize(ids.size());
+  std::vector<size_t> first(ids.size() + 1, 0);
+  for (size_t k = 0; k < ids.size(); ++k) first[k + 1] = first[k] + poses[k];
+  parallel_for((int)ids.size(), [&](int k) {
+    const vd::Ligand& L = c->ligs[ids[k]];
     const int A = (int)L.atoms.size(), T = L.n_torsions(), n = poses[k];
-    PostLigand pl;
+    PostLigand& pl = ps.ligs[k];
     pl.id = ids[k];
-    const float* xyz = n > 0 ? r.coords.data() + r.xoff[first] : nullptr;
-    pl.modes = select_modes(c, L, n, r.res.data() + first, xyz, p);
+    const float* xyz = n > 0 ? r.coords.data() + r.xoff[first[k]] : nullptr;
+    pl.modes = select_modes(c, L, n, r.res.data() + first[k], xyz, p);
     for (const vd_mode& m : pl.modes) {
-      const size_t q = first + m.pose;
+      const size_t q = first[k] + m.pose;
       pl.conf.insert(pl.conf.end(), r.conf.begin() + r.coff[q], r.conf.begin() + r.coff[q] + 7 + T);
       pl.coords.insert(pl.coords.end(), r.coords.begin() + r.xoff[q], r.coords.begin() + r.xoff[q] + 3 * A);
     }
-    ps.ligs.push_back(std::move(pl));
-    first += n;
-  }
+  });
   c->post = std::move(ps);
   if (st) { st->kernel_ms = r.ms; st->evals = r.evals; st->chains_per_wave = 1; st->blocks = (int)r.res.size(); }
 }
@@ -1231,20 +1254,23 @@ int vd_postprocess_docked(vd_ctx* c, int max_poses, const vd_post_params* p, vd_
   return guard(c, [&]() {
     need_device(c);
     BatchState& b = c->batch;
-    std::vector<int> ids, poses;
-    std::vector<float> conf;
-    for (size_t k = 0; k < b.ids.size(); ++k) {
-      const size_t ch = b.chains[k];
-      const int T = b.T[k], H = b.H[k];
-      std::vector<float> e(ch), cf(ch * (7 + T)), hv(ch * H * 3);
+    const int nl = (int)b.ids.size();
+    std::vector<float> e(b.total.x), cf(b.total.y), hv(b.total.z);
+    if (b.total.x) HIP_CHECK(hipMemcpy(e.data(), b.e.p, e.size() * 4, hipMemcpyDeviceToHost));
+    if (b.total.y) HIP_CHECK(hipMemcpy(cf.data(), b.conf.p, cf.size() * 4, hipMemcpyDeviceToHost));
+    if (b.total.z) HIP_CHECK(hipMemcpy(hv.data(), b.heavy.p, hv.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<std::vector<int>> kept(nl);
+    parallel_for(nl, [&](int k) {
       const longlong4 o = b.off[k];
-      HIP_CHECK(hipMemcpy(e.data(), b.e.p + o.x, ch * 4, hipMemcpyDeviceToHost));
-      HIP_CHECK(hipMemcpy(cf.data(), b.conf.p + o.y, cf.size() * 4, hipMemcpyDeviceToHost));
-      HIP_CHECK(hipMemcpy(hv.data(), b.heavy.p + o.z, hv.size() * 4, hipMemcpyDeviceToHost));
-      std::vector<int> kept = vd::cluster_poses((int)ch, H, e.data(), hv.data(), 1.0, max_poses);
-      ids.push_back(b.ids[k]);
-      poses.push_back((int)kept.size());
-      for (int i : kept) conf.insert(conf.end(), cf.begin() + (size_t)i * (7 + T), cf.begin() + (size_t)(i + 1) * (7 + T));
+      kept[k] = vd::cluster_poses(b.chains[k], b.H[k], e.data() + o.x, hv.data() + o.z, 1.0, max_poses);
+    });
+    std::vector<int> ids(b.ids), poses(nl);
+    std::vector<float> conf;
+    for (int k = 0; k < nl; ++k) {
+      const int T = b.T[k];
+      poses[k] = (int)kept[k].size();
+      const float* base = cf.data() + b.off[k].y;
+      for (int i : kept[k]) conf.insert(conf.end(), base + (size_t)i * (7 + T), base + (size_t)(i + 1) * (7 + T));
     }
     postprocess(c, ids, poses, conf.data(), p, st);
   });
