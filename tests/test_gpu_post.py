@@ -7,6 +7,8 @@ energies within 1e-4 kcal/mol and poses within 1e-3 A for poses near a minimum; 
 BFGS moves far, a knife-edge line-search decision can part the two runs, so the bound there is
 on the distribution; final scores within 1e-3 and as the reference prints them (0.1 kcal/mol).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -146,3 +148,45 @@ def test_dock_then_postprocess_1u1b(ctx, site):
     assert modes[0].rmsd_lb == 0 and modes[0].rmsd_ub == 0
     text = ctx.write_pdbqt(lid, modes, xyz)
     assert text.startswith("MODEL 1\nREMARK VINA RESULT:") and text.count("ENDMDL") == len(modes)
+
+
+def test_driver_runs_the_reference_example(ctx, site, tmp_path):
+    """The reference's example run (1u1b_config.txt: its box and 8192 chains) through the
+    command-line flow: the output file has the reference output's layout line for line (every
+    line but the coordinates and the scores equal to V11/docking_result/1u1b_docking_result.pdbqt's
+    first model), and a best mode as good as the printed -7.9 to within the search's spread.
+    Directory mode writes one <name>_out.pdbqt per ligand into <dir>_out."""
+    import shutil
+    from conftest import GOLDEN
+    from vdock import driver
+    lig = str(tmp_path / "1u1b_ligand.pdbqt")
+    shutil.copy(f"{GOLDEN}/1u1b_ligand.pdbqt", lig)
+    opts = driver.resolve(f"{GOLDEN}/1u1b_config.txt", receptor=f"{GOLDEN}/1u1b_protein.pdbqt", ligand=lig,
+                          out=str(tmp_path / "out.pdbqt"), seed=11)
+    res = driver.run(opts, ctx=ctx, log=lambda *a: None)
+    text = open(tmp_path / "out.pdbqt").read()
+    assert text == res[0]["text"] and res[0]["modes"][0].e <= -7.0
+    ref = golden("1u1b_docking_result.pdbqt").splitlines()
+    ref_model = ref[:ref.index("ENDMDL") + 1]
+    got = text.splitlines()
+    got_model = got[:got.index("ENDMDL") + 1]
+    assert len(got_model) == len(ref_model)
+    for a, b in zip(got_model, ref_model):
+        if a.startswith("REMARK VINA RESULT:"):
+            assert b.startswith("REMARK VINA RESULT:") and len(a) == len(b)
+        elif a.startswith(("ATOM", "HETATM")):
+            assert a[:30] == b[:30] and a[54:] == b[54:]
+        else:
+            assert a == b
+    d = tmp_path / "ligs"
+    d.mkdir()
+    shutil.copy(f"{GOLDEN}/1u1b_ligand.pdbqt", d / "a.pdbqt")
+    shutil.copy(f"{GOLDEN}/2bm2_ligand.pdbqt", d / "b.pdbqt")
+    (d / "bad.pdbqt").write_text("ROOT\nnot a pdbqt line\n")
+    opts = driver.resolve(receptor=f"{GOLDEN}/1u1b_protein.pdbqt", ligand_directory=str(d), thread=2048,
+                          center_x=-30.661, center_y=-29.540, center_z=17.010, size_x=24.0, size_y=26.25,
+                          size_z=22.5, seed=3)
+    res = driver.run(opts, ctx=ctx, log=lambda *a: None)
+    assert [os.path.basename(r["out"]) for r in res] == ["a_out.pdbqt", "b_out.pdbqt"]
+    assert all(os.path.exists(r["out"]) and r["modes"] for r in res)
+    assert os.path.dirname(res[0]["out"]) == str(d) + "_out"
