@@ -333,7 +333,9 @@ __device__ float pair_terms_rows(const LigView& L, const Scratch& S, const EvalK
       const uint32_t bb = (w[u] >> 8) & 255u, ss = w[u] & 255u;
       const bool has = w[u] != VD_NO_PAIR && bb != 255u;
       src[u] = (w[u] != VD_NO_PAIR && ss != 255u) ? (int)ss : G;
-      const float4 pb = S.xyz[has ? (int)bb : l];
+      // a slot without a pair reads a written coordinate (its own atom, or atom 0 for lanes
+      // beyond the ligand) so that the unused distance stays finite
+      const float4 pb = S.xyz[has ? (int)bb : (l < L.A ? l : 0)];
       rx[u] = pb.x - own.x; ry[u] = pb.y - own.y; rz[u] = pb.z - own.z;
       const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
       const bool ok = has && r2 < VD_CUTOFF_SQR;

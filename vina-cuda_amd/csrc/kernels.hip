@@ -83,12 +83,15 @@ __global__ __launch_bounds__(256) void vd_grid_kernel(const float4* __restrict__
 
 // ------------------------------------------------------------------------------------------
 // LDS layout of one wave: ligand (atoms 4A, node_o 4N, node_a 4N floats), the conformation
-// stash, then per chain group frame 12N, nacc 8N, xyz 4A, frc 4A, the pair-force exchange row
-// 4(G+1), and for dense BFGS H G*G + 2G.  The pair words stay in global memory (read once per
-// evaluation, coalesced, L2-resident).
+// stash, then per chain group the node area 12N, xyz 4A, frc 4A, the pair-force exchange row
+// 4(G+1), and for dense BFGS H G*G + 2G.  The node area holds the frame rows from the kinematics
+// until the atoms are placed, then the per-node force and moment sums of the adjoint (8N): an
+// evaluation reads the frames only in its intermolecular pass, which ends (with a fence) before
+// the sums are cleared.  The pair words stay in global memory (read once per evaluation,
+// coalesced, L2-resident).
 __host__ __device__ inline int lig_floats(int A, int N) { return 4 * A + 8 * N + kStashSlots * kStashSlotFloats; }
 __host__ __device__ inline int group_floats(int A, int N, int G, int dense) {
-  return 12 * N + 8 * N + 8 * A + 4 * (G + 1) + (dense ? G * G + 2 * G : 0);
+  return 12 * N + 8 * A + 4 * (G + 1) + (dense ? G * G + 2 * G : 0);
 }
 
 template <int G>
@@ -109,8 +112,8 @@ __device__ __forceinline__ void make_views(const float4* __restrict__ blob, cons
   S.stash = lds + lig_floats(Amax, Nmax) - kStashSlots * kStashSlotFloats;
   float* g = lds + lig_floats(Amax, Nmax) + group * group_floats(Amax, Nmax, G, dense);
   S.frame = reinterpret_cast<float4*>(g);
-  S.nacc = reinterpret_cast<float4*>(g + 12 * Nmax);
-  S.xyz = reinterpret_cast<float4*>(g + 20 * Nmax);
+  S.nacc = S.frame;
+  S.xyz = reinterpret_cast<float4*>(g + 12 * Nmax);
   S.frc = S.xyz + Amax;
   S.fx = S.frc + Amax;
   H = reinterpret_cast<float*>(S.fx + G + 1);
