@@ -117,13 +117,13 @@ double vd_eval_last_ms(const vd_ctx* ctx);
 
 /* Docking parameters (monte_carlo, V11/lib/monte_carlo.h:60-70; kernel_monte arguments). */
 enum { VD_LS_BFGS = 0, VD_LS_RILC_BFGS = 1 };
-enum { VD_MC_VINA = 0, VD_MC_QVINA2 = 1 };
+enum { VD_MC_VINA = 0, VD_MC_QVINA2 = 1, VD_MC_QVINAW = 2 };
 typedef struct {
   int chains;            /* MC chains per ligand (reference: --thread) */
   int search_depth;      /* MC steps per chain; 0 = heuristic of V11/main/main.cpp:354-380 */
   int bfgs_steps;        /* local-search iterations; 0 = heuristic of V11/main/main.cpp:383-391 */
   int local_search;      /* VD_LS_RILC_BFGS (reference --rilc_bfgs 1) or VD_LS_BFGS */
-  int policy;            /* VD_MC_VINA or VD_MC_QVINA2 */
+  int policy;            /* VD_MC_VINA, VD_MC_QVINA2 or VD_MC_QVINAW (the last two with RILC-BFGS) */
   int mutation_quirk;    /* 1: mutate one of 1+T entities like the reference GPU host tables */
   int chains_per_wave;   /* 1, 2, 4; 0 = autotuned per ligand size bucket */
   int waves_per_simd;    /* register budget of the MC kernel: 2, 3 or 4 resident waves per SIMD; 0 = autotuned */

@@ -93,6 +93,13 @@ double orc_opt_test(int kind, int n, double* x, const double* coef, int max_step
  * Feeds m points of a ligand with T torsions into one fresh history in order: x[m][7+T]
  * conformations, f[m] energies, g[m][6+T] derivatives.  interesting[k] = the gate's answer for
  * point k; mode 1 adds a point when it is interesting (the gate), mode 2 adds every point. */
+/* QuickVina-W state (per-chain individual buffers, one global buffer of 2^levels cells per axis
+ * over the box) driven by a sequence of operations: op 0 gate query (out = 1 interesting),
+ * 1 add to chain's individual buffer, 2 add to the global buffer (out = cell), 3 G-check alone
+ * (out = -1 interesting, else entries examined). */
+int orc_qvw_feed(int T, int m, const double* x, const double* f, const double* g, const int* chain, const int* op,
+                 const double box_lo[3], const double box_hi[3], int levels, int* out);
+int orc_qvw_levels(long long chains, long long depth);
 int orc_visited_feed(int T, int m, const double* x, const double* f, const double* g, int mode, int* interesting);
 
 /* ---- Monte Carlo (kernel2.cu:219-426 schedule; mutate.cpp:62-127; monte_carlo.cpp:35-39) ---- */
@@ -103,7 +110,7 @@ typedef struct {
   double box_lo[3], box_hi[3];
   double v_intra, v_inter, slope, temperature, amplitude;
   int    threads;
-  int    policy;                                   /* 0 Vina, 1 QuickVina2 visited-history gate */
+  int    policy;                                   /* 0 Vina, 1 QuickVina2 visited-history gate, 2 QuickVina-W */
 } orc_mc_params;
 /* per chain: e[c], conf[c*(7+T)], heavy coords[c*3*H]; returns total evals through *evals */
 int orc_mc_run(void* grid, void* lig, const orc_mc_params* p, double* e, double* conf,

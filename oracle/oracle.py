@@ -65,6 +65,8 @@ def _load():
         "orc_initial_conf": (C.c_int, [vp, vp, C.c_int, dp]),
         "orc_cluster": (C.c_int, [C.c_int, C.c_int, dp, dp, C.c_double, C.c_int, ip]),
         "orc_visited_feed": (C.c_int, [C.c_int, C.c_int, dp, dp, dp, C.c_int, ip]),
+        "orc_qvw_feed": (C.c_int, [C.c_int, C.c_int, dp, dp, dp, ip, ip, dp, dp, C.c_int, ip]),
+        "orc_qvw_levels": (C.c_int, [C.c_longlong, C.c_longlong]),
         "orc_ligand_serials": (C.c_int, [vp, ip]),
         "orc_num_tors": (C.c_double, [vp]),
         "orc_conf_independent": (C.c_double, [vp, C.c_double, C.c_double]),
@@ -339,6 +341,7 @@ class MCParams(C.Structure):
 
 
 def mc_params(chains, depth, bfgs_steps, box_lo, box_hi, seed=42, stream=0, mode=1, quirk=0, threads=0, policy=0):
+    """policy: 0 Vina, 1 QuickVina2, 2 QuickVina-W."""
     p = MCParams()
     p.chains, p.search_depth, p.bfgs_steps, p.mode, p.mutation_quirk = chains, depth, bfgs_steps, mode, quirk
     p.seed, p.stream = seed, stream
@@ -444,3 +447,24 @@ def postprocess(grid, lig, confs, max_steps, w_rot=VINA_WEIGHTS_ROT, num_modes=9
     k = lib().orc_postprocess(grid.h, lig.h, n, _d(confs), max_steps, w_rot, num_modes, energy_range,
                               min_rmsd, slope, _d(co), _d(e), _d(lb), _d(ub), _d(rf))
     return [dict(conf=co[i].copy(), e=e[i], lb=lb[i], ub=ub[i], refined=rf[i]) for i in range(k)]
+
+
+def qvw_feed(T, ops, box_lo, box_hi, levels):
+    """Runs QuickVina-W gate operations: ops = [(op, chain, x, f, g)] with op 0 gate query,
+    1 individual add, 2 global add, 3 G-check alone; returns the per-op outputs."""
+    m = len(ops)
+    x = np.ascontiguousarray([o[2] for o in ops], dtype=np.float64).reshape(m, 7 + T)
+    f = np.ascontiguousarray([o[3] for o in ops], dtype=np.float64)
+    g = np.ascontiguousarray([o[4] for o in ops], dtype=np.float64).reshape(m, 6 + T)
+    ch = np.ascontiguousarray([o[1] for o in ops], dtype=np.int32)
+    op = np.ascontiguousarray([o[0] for o in ops], dtype=np.int32)
+    lo = np.ascontiguousarray(box_lo, dtype=np.float64)
+    hi = np.ascontiguousarray(box_hi, dtype=np.float64)
+    out = np.zeros(max(m, 1), dtype=np.int32)
+    if lib().orc_qvw_feed(T, m, _d(x), _d(f), _d(g), _i(ch), _i(op), _d(lo), _d(hi), levels, _i(out)) != 0:
+        raise RuntimeError(_err())
+    return out[:m].tolist()
+
+
+def qvw_levels(chains, depth):
+    return lib().orc_qvw_levels(chains, depth)

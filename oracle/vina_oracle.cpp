@@ -25,6 +25,8 @@
 #include <sstream>
 #include <algorithm>
 #include <thread>
+#include <mutex>
+#include <map>
 #include <atomic>
 #include <limits>
 #include <stdexcept>
@@ -702,8 +704,14 @@ int line_search_lo(F& f, Inc& inc, fl& stp, P& x, fl& fx, std::vector<fl>& g, co
   }
 }
 
-template <class P, class F, class Inc>
-fl rilc_bfgs(F& f, Inc& inc, P& x, std::vector<fl>& g, int max_steps) {
+struct NoHooks {   // observers of a RILC-BFGS run (QuickVina-W records points through them)
+  template <class P> void iter(const P&, fl, const std::vector<fl>&) {}   // an iteration ended, search continues
+  template <class P> void stop(const P&, fl, const std::vector<fl>&) {}   // line search failed / step budget spent
+  template <class P> void done(const P&, fl, const std::vector<fl>&) {}   // the loop ended by one of those two
+};
+
+template <class P, class F, class Inc, class H = NoHooks>
+fl rilc_bfgs(F& f, Inc& inc, P& x, std::vector<fl>& g, int max_steps, H* hooks = nullptr) {
   fl fx = f(x, g);
   const int n = (int)g.size();
   fl fx_orig = fx; std::vector<fl> g_orig = g; P x_orig = x;
@@ -715,9 +723,9 @@ fl rilc_bfgs(F& f, Inc& inc, P& x, std::vector<fl>& g, int max_steps) {
   while (true) {
     P xp = x; std::vector<fl> gp = g; fl fxp = fx;
     int ls = line_search_lo(f, inc, step, x, fx, g, d, xp, gp);
-    if (ls < 0) { x = xp; g = gp; fx = fxp; break; }
+    if (ls < 0) { x = xp; g = gp; fx = fxp; if (hooks) hooks->stop(x, fx, g); break; }
     if (!(std::sqrt(vdot(g, g)) >= 1e-5)) return fx;
-    if (max_steps != 0 && max_steps <= k) break;
+    if (max_steps != 0 && max_steps <= k) { if (hooks) hooks->stop(x, fx, g); break; }
     ++k;
     for (int i = 0; i < n; ++i) { s[i] = step * d[i]; y[i] = g[i] - gp[i]; }
     fl ys = vdot(y, s), yy = vdot(y, y);
@@ -731,7 +739,9 @@ fl rilc_bfgs(F& f, Inc& inc, P& x, std::vector<fl>& g, int max_steps) {
       for (int i = 0; i < n; ++i) d[i] = d[i] + (alpha - beta) * s[i];
     }
     step = 1.0;
+    if (hooks) hooks->iter(x, fx, g);
   }
+  if (hooks) hooks->done(x, fx, g);
   if (!(fx <= fx_orig)) { fx = fx_orig; x = x_orig; g = g_orig; }
   return fx;
 }
@@ -786,6 +796,7 @@ fl local_search(EvalCtx& ctx, Conf& x, int max_steps, int mode);
 struct VEntry { std::vector<fl> x; fl f; uint32_t zero = 0, pos = 0; };
 struct Visited {
   int n = 0, count = 0, p = 0; bool full = false;
+  int rounds = 10;   // capacity = rounds x n: 10 for QuickVina2, 5 for QuickVina-W's individual buffer
   std::vector<VEntry> list;
   static std::vector<fl> flat(const Conf& c) {
     std::vector<fl> v{c.pos[0], c.pos[1], c.pos[2], c.q.a, c.q.b, c.q.c, c.q.d};
@@ -798,9 +809,9 @@ struct Visited {
   void add(const Conf& c, fl f, const std::vector<fl>& g) {
     VEntry e; e.x = flat(c); e.f = f;
     for (size_t i = 0; i < g.size(); ++i) { if (g[i] == 0) e.zero |= 1u << i; else if (g[i] > 0) e.pos |= 1u << i; }
-    if (count == 0) { n = (int)g.size(); list.assign(10 * n, VEntry()); }
-    if (!full) { list[count++] = e; if (count >= 10 * n) { full = true; p = 0; } }
-    else { list[p] = e; p = (p + 1) % (10 * n); }
+    if (count == 0) { n = (int)g.size(); list.assign(rounds * n, VEntry()); }
+    if (!full) { list[count++] = e; if (count >= rounds * n) { full = true; p = 0; } }
+    else { list[p] = e; p = (p + 1) % (rounds * n); }
   }
   bool check(const std::vector<fl>& x, fl f, const std::vector<fl>& g, const VEntry& nb) const {
     const bool y_bigger = (f - nb.f) > 0;
@@ -816,20 +827,107 @@ struct Visited {
   }
   bool interesting(const Conf& c, fl f, const std::vector<fl>& g) const {
     if (!full) return true;
-    const std::vector<fl> x = flat(c);
-    const int len = (int)list.size();
+    return nearest_check(list, flat(c), f, g, n) < 0;
+  }
+  // The up-to-max_checks nearest of entries (squared distance over the 7+T conformation numbers,
+  // lowest index among equals) in order of distance: -1 as soon as one does not explain the
+  // point (interesting), else the number examined.
+  int nearest_check(const std::vector<VEntry>& entries, const std::vector<fl>& x, fl f, const std::vector<fl>& g,
+                    int max_checks) const {
+    const int len = (int)entries.size();
     std::vector<fl> dist(len);
-    for (int k = 0; k < len; ++k) { fl d = 0; for (size_t j = 0; j < x.size(); ++j) d += sqr(list[k].x[j] - x[j]); dist[k] = d; }
+    for (int k = 0; k < len; ++k) { fl d = 0; for (size_t j = 0; j < x.size(); ++j) d += sqr(entries[k].x[j] - x[j]); dist[k] = d; }
     std::vector<char> picked(len, 0);
-    for (int it = 0; it < n; ++it) {
+    int it = 0;
+    for (; it < std::min(max_checks, len); ++it) {
       int best = -1;
       for (int k = 0; k < len; ++k) if (!picked[k] && (best < 0 || dist[k] < dist[best])) best = k;
       picked[best] = 1;
-      if (check(x, f, g, list[best])) return true;
+      if (check(x, f, g, entries[best])) return -1;
     }
-    return false;
+    return it;
+  }
+  // QuickVina-W's individual check (individual_interesting_cl): interesting until the buffer is
+  // full, then the up-to-(4n - excluded) nearest entries decide.
+  int interesting_qvw(const Conf& c, fl f, const std::vector<fl>& g, int excluded) const {
+    if (!full) return -1;
+    return nearest_check(list, flat(c), f, g, 4 * n - excluded);
   }
 };
+
+// QuickVina-W's global buffer (QVW/inc/cuda/visited.cuh: add_to_global_buffer,
+// getPointsWithinCutoff_cl, global_interesting_cl; get_n and set_Binary_Id_cl for the cell),
+// intended semantics: one buffer per ligand shared by all its chains; the box is cut into 2^L
+// cells per axis, L = (get_n(chains x search depth) - 3) / 3 levels (get_n: the smallest multiple
+// of 3, i, with chains x depth < 2^(i+1)), capped at 4; each cell keeps a ring of the last 8
+// local-search end points added in it.  The G-check takes the cell holding the point's position
+// (the reference takes the cell of the last point added and never adds, as its only call
+// passes global_flag = true) and examines up to n = 6+T of its entries, nearest first.
+struct GlobalVisited {
+  int per_axis = 1; fl lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+  std::vector<std::vector<VEntry>> cells; std::vector<long long> counter;
+  std::mutex m;
+  static int levels_for(long long chains, long long depth) {
+    int i = 0;
+    while (!(chains * depth < (2LL << i))) i += 3;
+    return std::max(0, std::min(4, (i - 3) / 3));
+  }
+  void init(const fl* b, const fl* e, int levels) {
+    per_axis = 1 << levels;
+    for (int k = 0; k < 3; ++k) { lo[k] = b[k]; hi[k] = e[k]; }
+    cells.assign((size_t)per_axis * per_axis * per_axis, {});
+    counter.assign(cells.size(), 0);
+  }
+  int cell_of(const Conf& c) const {
+    int id[3];
+    for (int k = 0; k < 3; ++k) {
+      const fl t = (c.pos[k] - lo[k]) / (hi[k] - lo[k]) * per_axis;
+      id[k] = t >= 0 ? std::min(per_axis - 1, (int)t) : 0;
+    }
+    return (id[2] * per_axis + id[1]) * per_axis + id[0];
+  }
+  void add(const Conf& c, fl f, const std::vector<fl>& g) {
+    VEntry e; e.x = Visited::flat(c); e.f = f;
+    for (size_t i = 0; i < g.size(); ++i) { if (g[i] == 0) e.zero |= 1u << i; else if (g[i] > 0) e.pos |= 1u << i; }
+    std::lock_guard<std::mutex> lk(m);
+    const int cl = cell_of(c);
+    if (cells[cl].size() < 8) cells[cl].push_back(e); else cells[cl][counter[cl] % 8] = e;
+    ++counter[cl];
+  }
+  int interesting(const Visited& rule, const Conf& c, fl f, const std::vector<fl>& g) {
+    std::vector<VEntry> here;
+    { std::lock_guard<std::mutex> lk(m); here = cells[cell_of(c)]; }
+    return rule.nearest_check(here, Visited::flat(c), f, g, (int)g.size());
+  }
+};
+
+// QuickVina-W's local search (QVW/inc/cuda/quasi_newton.cuh:2180-2420): the G-check, then the
+// individual check with the examined count excluded; a point neither finds interesting is not
+// searched.  A searched start point, every iteration's end point and the point where the search
+// stops go to the individual buffer; the search's last point goes to the global buffer.
+struct QvwHooks {
+  Visited* ind; GlobalVisited* glob;
+  template <class P> void iter(const P& x, fl f, const std::vector<fl>& g) { ind->add(x, f, g); }
+  template <class P> void stop(const P& x, fl f, const std::vector<fl>& g) { ind->add(x, f, g); }
+  template <class P> void done(const P& x, fl f, const std::vector<fl>& g) { glob->add(x, f, g); }
+};
+bool qvw_gate(Visited& ind, GlobalVisited& glob, const Conf& x, fl f, const std::vector<fl>& g) {
+  const int ret = glob.interesting(ind, x, f, g);
+  return !(ret >= 0 && ind.interesting_qvw(x, f, g, ret) >= 0);
+}
+
+fl local_search_qvw(EvalCtx& ctx, Conf& x, int max_steps, Visited& ind, GlobalVisited& glob) {
+  std::vector<fl> g;
+  const fl f = ctx(x, g);
+  if (!qvw_gate(ind, glob, x, f, g)) { x.e = f; return f; }
+  ind.add(x, f, g);
+  --ctx.evals;             // the search below starts by evaluating the same point: count it once
+  ConfInc inc;
+  QvwHooks h{&ind, &glob};
+  const fl e = rilc_bfgs(ctx, inc, x, g, max_steps, &h);
+  x.e = e;
+  return e;
+}
 
 fl local_search(EvalCtx& ctx, Conf& x, int max_steps, int mode, Visited* visited) {
   if (visited) {
@@ -1083,52 +1181,62 @@ fl gyration_radius(const Ligand& L, const std::vector<V3>& coords, const V3& ori
 // One Monte Carlo chain with the reference GPU schedule (V11/inc/cuda/kernel2.cu:290-426):
 // mutate -> local search -> Metropolis(T) -> on accept, if better than best: second local
 // search, keep best.  Mutation per V11/lib/mutate.cpp:62-127.
-void mc_chain(EvalCtx& ctx, const orc_mc_params& p, int chain, Conf& best, std::vector<V3>& best_heavy) {
+// One MC chain's state (monte_carlo.cpp:93-127 / kernel2.cu:219-426 as a step function).
+struct McState {
+  Conf tmp, best; fl rg = 0, best_e = 0; std::vector<V3> best_heavy; Visited vis;
+};
+void mc_init(EvalCtx& ctx, const orc_mc_params& p, int chain, McState& st) {
+  const Ligand& L = *ctx.L;
+  st.tmp = initial_conf(L, p, chain);
+  st.tmp.e = 0;
+  Frames F; std::vector<V3> coords;
+  set_conf(L, st.tmp, coords, F);
+  st.rg = gyration_radius(L, coords, st.tmp.pos);
+  st.best_e = std::numeric_limits<fl>::infinity();
+  if (p.policy == 2) st.vis.rounds = 5;
+}
+void mc_step(EvalCtx& ctx, const orc_mc_params& p, int chain, int step, McState& st, GlobalVisited* glob) {
   const Ligand& L = *ctx.L;
   uint32_t key[2] = {p.seed, p.stream}, w[4];
-  Conf tmp = initial_conf(L, p, chain);
-  tmp.e = 0;
-  Frames F; std::vector<V3> coords;
-  set_conf(L, tmp, coords, F);
-  fl rg = gyration_radius(L, coords, tmp.pos);
-  fl best_e = std::numeric_limits<fl>::infinity();
   const int T = L.T();
   const int n_ent = p.mutation_quirk ? (1 + T) : (2 + T);
-  Visited vis;
-  Visited* vp = p.policy == 1 ? &vis : nullptr;
-  for (int step = 0; step < p.search_depth; ++step) {
-    Conf cand = tmp;
-    uint32_t c0[4] = {(uint32_t)chain, (uint32_t)step, 0, 0}; philox(c0, key, w);
-    int which = std::min(int(u01(w[0]) * n_ent), n_ent - 1);
-    fl u_metro = u01(w[1]); fl tor = -PI + 2 * PI * u01(w[2]);
-    if (which <= 1) {
-      V3 sph;
-      for (uint32_t a = 0;; ++a) {
-        uint32_t c1[4] = {(uint32_t)chain, (uint32_t)step, 1, a}; philox(c1, key, w);
-        sph = V3(2 * u01(w[0]) - 1, 2 * u01(w[1]) - 1, 2 * u01(w[2]) - 1);
-        if (dot(sph, sph) < 1) break;
-      }
-      if (which == 0) cand.pos = cand.pos + p.amplitude * sph;
-      else if (rg > EPS) quaternion_increment(cand.q, (p.amplitude / rg) * sph);
-    } else {
-      cand.tors[which - 2] = tor;
+  Visited* vp = p.policy == 1 ? &st.vis : nullptr;
+  auto search = [&](Conf& c) {
+    if (p.policy == 2) local_search_qvw(ctx, c, p.bfgs_steps, st.vis, *glob);
+    else local_search(ctx, c, p.bfgs_steps, p.mode, vp);
+  };
+  Conf cand = st.tmp;
+  uint32_t c0[4] = {(uint32_t)chain, (uint32_t)step, 0, 0}; philox(c0, key, w);
+  int which = std::min(int(u01(w[0]) * n_ent), n_ent - 1);
+  fl u_metro = u01(w[1]); fl tor = -PI + 2 * PI * u01(w[2]);
+  if (which <= 1) {
+    V3 sph;
+    for (uint32_t a = 0;; ++a) {
+      uint32_t c1[4] = {(uint32_t)chain, (uint32_t)step, 1, a}; philox(c1, key, w);
+      sph = V3(2 * u01(w[0]) - 1, 2 * u01(w[1]) - 1, 2 * u01(w[2]) - 1);
+      if (dot(sph, sph) < 1) break;
     }
-    local_search(ctx, cand, p.bfgs_steps, p.mode, vp);
-    bool accept = (step == 0) || (cand.e < tmp.e) || (u_metro < std::exp((tmp.e - cand.e) / p.temperature));  // monte_carlo.cpp:35-39
-    if (accept) {
-      tmp = cand;
-      if (tmp.e < best_e) {
-        local_search(ctx, tmp, p.bfgs_steps, p.mode, vp);
-        if (tmp.e < best_e) {
-          best = tmp; best_e = tmp.e;
-          set_conf(L, best, coords, F);
-          best_heavy.clear();
-          for (size_t i = 0; i < L.atoms.size(); ++i) if (L.atoms[i].el != EL_H) best_heavy.push_back(coords[i]);
-        }
+    if (which == 0) cand.pos = cand.pos + p.amplitude * sph;
+    else if (st.rg > EPS) quaternion_increment(cand.q, (p.amplitude / st.rg) * sph);
+  } else {
+    cand.tors[which - 2] = tor;
+  }
+  search(cand);
+  bool accept = (step == 0) || (cand.e < st.tmp.e) || (u_metro < std::exp((st.tmp.e - cand.e) / p.temperature));  // monte_carlo.cpp:35-39
+  if (accept) {
+    st.tmp = cand;
+    Frames F; std::vector<V3> coords;
+    if (st.tmp.e < st.best_e) {
+      search(st.tmp);
+      if (st.tmp.e < st.best_e) {
+        st.best = st.tmp; st.best_e = st.tmp.e;
+        set_conf(L, st.best, coords, F);
+        st.best_heavy.clear();
+        for (size_t i = 0; i < L.atoms.size(); ++i) if (L.atoms[i].el != EL_H) st.best_heavy.push_back(coords[i]);
       }
-      set_conf(L, tmp, coords, F);
-      rg = gyration_radius(L, coords, tmp.pos);
     }
+    set_conf(L, st.tmp, coords, F);
+    st.rg = gyration_radius(L, coords, st.tmp.pos);
   }
 }
 
@@ -1295,6 +1403,34 @@ int orc_visited_feed(int T, int m, const double* x, const double* f, const doubl
   } catch (std::exception& e) { return fail(e.what()); }
 }
 
+int orc_qvw_feed(int T, int m, const double* x, const double* f, const double* g, const int* chain, const int* op,
+                 const double box_lo[3], const double box_hi[3], int levels, int* out) {
+  try {
+    if (T < 0 || m < 0 || levels < 0 || levels > 4) throw std::runtime_error("bad arguments");
+    GlobalVisited glob;
+    glob.init(box_lo, box_hi, levels);
+    std::map<int, Visited> ind;
+    const int n = 6 + T;
+    for (int k = 0; k < m; ++k) {
+      Conf c;
+      const double* xk = x + (size_t)k * (7 + T);
+      c.pos = V3(xk[0], xk[1], xk[2]); c.q = Q{xk[3], xk[4], xk[5], xk[6]};
+      c.tors.assign(xk + 7, xk + 7 + T);
+      std::vector<fl> gk(g + (size_t)k * n, g + (size_t)(k + 1) * n);
+      Visited& v = ind[chain[k]];
+      v.rounds = 5;
+      out[k] = 0;
+      if (op[k] == 0) out[k] = qvw_gate(v, glob, c, f[k], gk) ? 1 : 0;
+      else if (op[k] == 1) v.add(c, f[k], gk);
+      else if (op[k] == 2) { glob.add(c, f[k], gk); out[k] = glob.cell_of(c); }
+      else if (op[k] == 3) out[k] = glob.interesting(v, c, f[k], gk);
+      else throw std::runtime_error("bad op");
+    }
+    return 0;
+  } catch (std::exception& e) { return fail(e.what()); }
+}
+int orc_qvw_levels(long long chains, long long depth) { return GlobalVisited::levels_for(chains, depth); }
+
 double orc_opt_test(int kind, int n, double* x, const double* coef, int max_steps, int mode, int* n_evals) {
   int evals = 0;
   auto f = [&](const std::vector<fl>& p, std::vector<fl>& g) -> fl {
@@ -1326,19 +1462,46 @@ int orc_mc_run(void* gp, void* lp, const orc_mc_params* p, double* e, double* co
   const int T = L->T(), H = L->heavy;
   std::atomic<int> next(0); std::atomic<long long> total(0);
   int nth = p->threads > 0 ? p->threads : (int)std::max(1u, std::thread::hardware_concurrency());
-  std::vector<std::thread> pool;
-  for (int t = 0; t < nth; ++t) pool.emplace_back([&]() {   // parallel_mc.cpp:65-77: one task per chain
-    EvalCtx ctx{G, L, p->v_intra, p->v_inter, p->slope};
-    for (int c = next++; c < p->chains; c = next++) {
-      Conf best; std::vector<V3> heavy;
-      mc_chain(ctx, *p, c, best, heavy);
-      e[c] = best.e;
-      conf_to(best, conf + (size_t)c * (7 + T));
-      for (int i = 0; i < H; ++i) for (int k = 0; k < 3; ++k) heavy_coords[((size_t)c * H + i) * 3 + k] = i < (int)heavy.size() ? heavy[i][k] : 0;
+  std::vector<McState> st(p->chains);
+  auto output = [&](int c) {
+    e[c] = st[c].best.e;
+    conf_to(st[c].best, conf + (size_t)c * (7 + T));
+    const std::vector<V3>& heavy = st[c].best_heavy;
+    for (int i = 0; i < H; ++i) for (int k = 0; k < 3; ++k) heavy_coords[((size_t)c * H + i) * 3 + k] = i < (int)heavy.size() ? heavy[i][k] : 0;
+  };
+  if (p->policy != 2) {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nth; ++t) pool.emplace_back([&]() {   // parallel_mc.cpp:65-77: one task per chain
+      EvalCtx ctx{G, L, p->v_intra, p->v_inter, p->slope};
+      for (int c = next++; c < p->chains; c = next++) {
+        mc_init(ctx, *p, c, st[c]);
+        for (int s = 0; s < p->search_depth; ++s) mc_step(ctx, *p, c, s, st[c], nullptr);
+        output(c);
+      }
+      total += ctx.evals;
+    });
+    for (auto& th : pool) th.join();
+  } else {
+    // QuickVina-W: the chains share the global buffer, so they advance together, one MC step of
+    // every chain before the next step (as the device's resident chains do); threads take the
+    // chains of a step in any order
+    GlobalVisited glob;
+    glob.init(p->box_lo, p->box_hi, GlobalVisited::levels_for(p->chains, p->search_depth));
+    for (int s = -1; s < p->search_depth; ++s) {
+      next = 0;
+      std::vector<std::thread> pool;
+      for (int t = 0; t < nth; ++t) pool.emplace_back([&]() {
+        EvalCtx ctx{G, L, p->v_intra, p->v_inter, p->slope};
+        for (int c = next++; c < p->chains; c = next++) {
+          if (s < 0) mc_init(ctx, *p, c, st[c]);
+          else mc_step(ctx, *p, c, s, st[c], &glob);
+        }
+        total += ctx.evals;
+      });
+      for (auto& th : pool) th.join();
     }
-    total += ctx.evals;
-  });
-  for (auto& th : pool) th.join();
+    for (int c = 0; c < p->chains; ++c) output(c);
+  }
   if (evals) *evals = total;
   return 0;
 }

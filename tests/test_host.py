@@ -162,9 +162,9 @@ def test_invalid_arguments(host_ctx):
         host_ctx.set_receptor(np.array([[np.nan, 0, 0]], dtype=np.float32), np.array([0], dtype=np.int32))
     with pytest.raises(vdock.VdockError):
         host_ctx.ligand_info(10 ** 6)
-    # parameters are checked before the device: the QuickVina2 policy runs with RILC-BFGS only
+    # parameters are checked before the device: the QuickVina2 and QuickVina-W policies run with RILC-BFGS only
     lid = host_ctx.add_ligand(vdock.synth_ligand(12, 2, 1))
-    for bad in (dict(policy=1, local_search=0), dict(chains=0), dict(chains_per_wave=3), dict(waves_per_simd=5)):
+    for bad in (dict(policy=1, local_search=0), dict(policy=2, local_search=0), dict(policy=3), dict(chains=0), dict(chains_per_wave=3), dict(waves_per_simd=5)):
         with pytest.raises(vdock.VdockError) as ei:
             host_ctx.dock(lid, vdock.default_params(**bad))
         assert ei.value.code == vdock.VD_ERR_ARG, bad

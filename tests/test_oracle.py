@@ -479,3 +479,94 @@ def test_visited_order_of_history_does_not_matter():
         assert a == b
         answers.append(a)
     assert 0 < sum(answers) < len(answers)          # both answers occur
+
+
+# ---- QuickVina-W (QVW/inc/cuda/visited.cuh, quasi_newton.cuh:2180-2420, intended semantics) ----
+# Same bowl points.  Individual buffers hold 5n entries; the global buffer is shared by the
+# chains of a ligand, a ring of 8 entries per cell.
+QIND = 5 * VN
+QBOX = ([-4.0, -4.0, -4.0], [4.0, 4.0, 4.0])
+
+
+def _qops(op, chain, pts):
+    x, f, g = pts
+    return [(op, chain, x[i], f[i], g[i]) for i in range(len(f))]
+
+
+def _qfeed(ops, levels=0):
+    return orc.qvw_feed(VT, ops, *QBOX, levels)
+
+
+def test_qvw_levels_follow_get_n():
+    """get_n (QVW visited.cuh): the smallest multiple of 3, i, with chains x depth < 2^(i+1);
+    (i - 3) / 3 levels of halving per axis, at least 0, capped at 4 here."""
+    assert orc.qvw_levels(256, 20) == 3          # 5120 < 8192 = 2^13
+    assert orc.qvw_levels(8192, 30) == 4         # 245760 < 2^19 -> 5 levels, capped
+    assert orc.qvw_levels(10, 10) == 1           # 100 < 128 = 2^7
+    assert orc.qvw_levels(3, 5) == 0             # 15 < 16 = 2^4
+    assert orc.qvw_levels(1, 1) == 0
+
+
+def test_qvw_gate_needs_a_full_individual_buffer():
+    rng = np.random.default_rng(21)
+    right, q = _bowl(rng, QIND - 1, +1), _bowl(rng, 1, +1)
+    ops = _qops(1, 0, right) + _qops(0, 0, q)
+    assert _qfeed(ops)[-1] == 1                  # 5n - 1 explaining entries: still interesting
+    ops = _qops(1, 0, _bowl(rng, QIND, +1)) + _qops(0, 0, q) + _qops(0, 0, _bowl(rng, 1, -1))
+    assert _qfeed(ops)[-2:] == [0, 1]            # full: the same side is skipped, the other searched
+
+
+def test_qvw_global_buffer_is_shared_and_consulted_first():
+    rng = np.random.default_rng(22)
+    full_right = _qops(1, 0, _bowl(rng, QIND, +1))
+    q = _qops(0, 0, _bowl(rng, 1, +1))
+    # another chain's search ended across the minimum: chain 0's point becomes interesting
+    assert _qfeed(full_right + _qops(2, 1, _bowl(rng, 1, -1)) + q)[-1] == 1
+    # same-side global entries explain it: the G-check examines them all, the individual
+    # buffer the remaining 4n - 8 nearest, and the point is skipped
+    out = _qfeed(full_right + _qops(2, 1, _bowl(rng, 8, +1)) + _qops(3, 0, _bowl(rng, 1, +1)) + q)
+    assert out[-2] == min(8, VN) and out[-1] == 0
+
+
+def test_qvw_global_ring_keeps_the_last_8():
+    rng = np.random.default_rng(23)
+    full_right = _qops(1, 0, _bowl(rng, QIND, +1))
+    q = _qops(0, 0, _bowl(rng, 1, +1))
+    left8 = _qops(2, 1, _bowl(rng, 8, -1))
+    assert _qfeed(full_right + left8 + _qops(2, 1, _bowl(rng, 8, +1)) + q)[-1] == 0
+    assert _qfeed(full_right + left8 + _qops(2, 1, _bowl(rng, 7, +1)) + q)[-1] == 1
+
+
+def test_qvw_global_cells():
+    """Cells of 8 A / 2^levels over the box, x fastest; outside the box clamps to the border cell;
+    an entry in another cell is not consulted."""
+    rng = np.random.default_rng(24)
+    x, f, g = _bowl(rng, 2, -1)
+    x[0, :3] = [-3.9, 0.1, 3.9]
+    x[1, :3] = [10.0, -10.0, 0.0]
+    out = _qfeed(_qops(2, 1, (x, f, g)), levels=2)
+    assert out == [(3 * 4 + 2) * 4 + 0, (2 * 4 + 0) * 4 + 3]
+    full_right = _qops(1, 0, _bowl(rng, QIND, +1))
+    qx, qf, qg = _bowl(rng, 1, +1)
+    qx[0, :3] = [-3.9, 0.1, 3.9]
+    left = _bowl(rng, 1, -1)
+    for pos, expect in (([-3.8, 0.2, 3.8], 1), ([3.8, 0.2, 3.8], 0)):
+        lx = left[0].copy()
+        lx[0, :3] = pos
+        ops = full_right + _qops(2, 1, (lx, left[1], left[2])) + _qops(0, 0, (qx, qf, qg))
+        assert _qfeed(ops, levels=2)[-1] == expect
+
+
+def test_qvw_policy_runs_fewer_searches(scene):
+    """The MC with the QuickVina-W gate skips searches once buffers fill: fewer evaluations than
+    the Vina policy on the same chains, and a comparable best score."""
+    import vdock
+    lig = orc.Ligand(vdock.synth_ligand(12, 2, 9))
+    b, e = [-6.0, -6.0, -6.0], [6.0, 6.0, 6.0]
+    res = {}
+    for pol in (0, 2):
+        p = orc.mc_params(32, 96, 8, b, e, seed=3, threads=8, policy=pol)
+        en, _, _, ev = orc.mc_run(scene, lig, p)
+        res[pol] = (float(en.min()), ev)
+    assert res[2][1] < 0.97 * res[0][1]
+    assert res[2][0] <= res[0][0] + 0.5

@@ -101,7 +101,10 @@ struct vd_ctx {
   int n_cu = 0;
   std::map<int, std::pair<int, int>> tuned;  // size bucket -> (lanes per chain, waves per SIMD)
   DevBuf<float> tune_e, tune_conf, tune_heavy;
-  DevBuf<float> hist;                        // QuickVina2 visited histories
+  DevBuf<float> hist;                        // QuickVina2 / QuickVina-W per-chain histories
+  DevBuf<float> glob;                        // QuickVina-W global buffers of the last launch
+  DevBuf<long long> glob_meta_off;
+  DevBuf<int> glob_meta_axis, glob_meta_T;
   std::vector<vd_launch> launches;           // MC launches of the last docking call
   // after the search: receptor cells of the direct evaluation (built on first use per box)
   bool nc_ready = false;
@@ -202,13 +205,21 @@ void check_params(const vd_dock_params* p) {
   if (p->chains < 1) throw ArgError(VD_ERR_ARG, "chains must be >= 1");
   if (p->search_depth < 0 || p->bfgs_steps < 0) throw ArgError(VD_ERR_ARG, "negative search_depth or bfgs_steps");
   if (p->local_search != VD_LS_BFGS && p->local_search != VD_LS_RILC_BFGS) throw ArgError(VD_ERR_ARG, "bad local_search");
-  if (p->policy != VD_MC_VINA && p->policy != VD_MC_QVINA2) throw ArgError(VD_ERR_ARG, "bad policy");
-  if (p->policy == VD_MC_QVINA2 && p->local_search != VD_LS_RILC_BFGS)
-    throw ArgError(VD_ERR_ARG, "the QuickVina2 policy runs with RILC-BFGS only");
+  if (p->policy != VD_MC_VINA && p->policy != VD_MC_QVINA2 && p->policy != VD_MC_QVINAW) throw ArgError(VD_ERR_ARG, "bad policy");
+  if (p->policy != VD_MC_VINA && p->local_search != VD_LS_RILC_BFGS)
+    throw ArgError(VD_ERR_ARG, "the QuickVina2 and QuickVina-W policies run with RILC-BFGS only");
   if (p->chains_per_wave != 0 && p->chains_per_wave != 1 && p->chains_per_wave != 2 && p->chains_per_wave != 4)
     throw ArgError(VD_ERR_ARG, "chains_per_wave must be 0, 1, 2 or 4");
   if (p->waves_per_simd != 0 && (p->waves_per_simd < 2 || p->waves_per_simd > 4))
     throw ArgError(VD_ERR_ARG, "waves_per_simd must be 0, 2, 3 or 4");
+}
+
+// QuickVina-W global buffer levels (QVW/inc/cuda/visited.cuh get_n): the smallest multiple of 3,
+// i, with chains x depth < 2^(i+1); (i - 3) / 3 halvings per axis, at least 0, at most 4.
+int qvw_levels(long long chains, long long depth) {
+  int i = 0;
+  while (!(chains * depth < (2LL << i))) i += 3;
+  return std::max(0, std::min(4, (i - 3) / 3));
 }
 
 int heur_depth(int A, int T, int rilc) {
@@ -314,11 +325,32 @@ void run_mc(vd_ctx* c, int G, int W, const Packed& pk, const std::vector<uint32_
   c->offs.ensure(off.size());
   c->queue.ensure(1);
   c->evals.ensure(pk.headers.size());
+  std::vector<vd_lig_header> headers(pk.headers);
+  std::vector<longlong4> offs(off);
+  long long glob_floats = 0;
+  std::vector<long long> g_off;
+  std::vector<int> g_axis, g_T;
+  if (P.policy == VD_MC_QVINAW) {
+    // one global buffer per ligand slot, 2^levels cells per axis (levels from chains x depth)
+    for (size_t k = 0; k < headers.size(); ++k) {
+      const int axis = 1 << qvw_levels(P.chains, headers[k].search_depth);
+      headers[k].reserved = axis;
+      offs[k].w = glob_floats;
+      g_off.push_back(glob_floats); g_axis.push_back(axis); g_T.push_back(headers[k].n_torsions);
+      glob_floats += (vdk::GlobalHist::floats(axis, headers[k].n_torsions) + 63) / 64 * 64;
+    }
+    c->glob.ensure(glob_floats);
+    c->glob_meta_off.ensure(g_off.size()); c->glob_meta_axis.ensure(g_axis.size()); c->glob_meta_T.ensure(g_T.size());
+    HIP_CHECK(hipMemcpyAsync(c->glob_meta_off.p, g_off.data(), g_off.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(c->glob_meta_axis.p, g_axis.data(), g_axis.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(hipMemcpyAsync(c->glob_meta_T.p, g_T.data(), g_T.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_CHECK(vdl::glob_init(c->glob.p, c->glob_meta_off.p, c->glob_meta_axis.p, c->glob_meta_T.p, (int)g_off.size(), c->stream));
+  }
   HIP_CHECK(hipMemcpyAsync(c->blob.p, pk.blob.data(), pk.blob.size() * 4, hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->headers.p, pk.headers.data(), pk.headers.size() * sizeof(vd_lig_header), hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->headers.p, headers.data(), headers.size() * sizeof(vd_lig_header), hipMemcpyHostToDevice, c->stream));
   HIP_CHECK(hipMemcpyAsync(c->streams.p, streams.data(), streams.size() * 4, hipMemcpyHostToDevice, c->stream));
   HIP_CHECK(hipMemcpyAsync(c->items.p, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->offs.p, off.data(), off.size() * sizeof(longlong4), hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->offs.p, offs.data(), offs.size() * sizeof(longlong4), hipMemcpyHostToDevice, c->stream));
   HIP_CHECK(hipMemsetAsync(c->evals.p, 0, pk.headers.size() * sizeof(unsigned long long), c->stream));
   McArgs a{};
   a.blob = c->blob.p; a.headers = c->headers.p; a.streams = c->streams.p; a.items = c->items.p;
@@ -330,7 +362,7 @@ void run_mc(vd_ctx* c, int G, int W, const Packed& pk, const std::vector<uint32_
   a.Amax = pk.Amax; a.Nmax = pk.Nmax;
   const int lds = vdl::mc_lds_bytes(G, pk.Amax, pk.Nmax, P.local_search == 0);
   if (lds > 64 * 1024) throw ArgError(VD_ERR_CAPACITY, "ligand too large for the LDS layout");
-  const int mode = P.local_search == 0 ? 0 : (P.policy == VD_MC_QVINA2 ? 2 : 1);
+  const int mode = P.local_search == 0 ? 0 : (P.policy == VD_MC_QVINA2 ? 2 : P.policy == VD_MC_QVINAW ? 3 : 1);
   // a tighter register budget only pays when the LDS of this launch lets the extra waves in:
   // take the loosest budget that reaches the same residency
   if (mode != 0) {
@@ -346,13 +378,14 @@ void run_mc(vd_ctx* c, int G, int W, const Packed& pk, const std::vector<uint32_
     blocks = std::min<long long>((long long)per_cu * c->n_cu, (long long)items.size());
     blocks = std::max(blocks, 1);
   }
-  if (P.policy == VD_MC_QVINA2) {
+  if (P.policy == VD_MC_QVINA2 || P.policy == VD_MC_QVINAW) {
     // one visited-history region per chain slot of every launched block (blocks keep their
     // slot for all the items they take from the queue)
-    a.hist_stride = vdk::History::floats_per_chain(pk.Tmax);
+    a.hist_stride = vdk::History::floats_per_chain(pk.Tmax, P.policy == VD_MC_QVINAW ? 5 : 10);
     c->hist.ensure((size_t)blocks * (64 / G) * a.hist_stride);
     a.hist = c->hist.p;
   }
+  if (P.policy == VD_MC_QVINAW) a.glob = c->glob.p;
   if (timed) HIP_CHECK(hipEventRecord(c->ev0, c->stream));
   HIP_CHECK(vdl::launch_mc(G, W, a, blocks, c->stream));
   if (timed) HIP_CHECK(hipEventRecord(c->ev1, c->stream));
@@ -788,6 +821,7 @@ void vd_destroy(vd_ctx* c) {
   c->queue.release(); c->evals.release(); c->io_a.release(); c->io_b.release(); c->io_c.release();
   c->batch.e.release(); c->batch.conf.release(); c->batch.heavy.release();
   c->tune_e.release(); c->tune_conf.release(); c->tune_heavy.release(); c->hist.release();
+  c->glob.release(); c->glob_meta_off.release(); c->glob_meta_axis.release(); c->glob_meta_T.release();
   c->nc_atoms.release(); c->nc_start.release(); c->ref_in.release(); c->ref_out.release(); c->ref_xyz.release();
   c->ref_res.release(); c->ref_lig.release(); c->ref_evals.release(); c->ref_coff.release(); c->ref_xoff.release();
   c->lutd.release(); c->ref_geo.release(); c->ref_pairs.release(); c->ref_hdr.release();

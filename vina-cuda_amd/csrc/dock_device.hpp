@@ -587,12 +587,20 @@ constexpr int kHistMaxRounds = 10;                // cap = 10 (6+T) <= 10 G entr
 struct History {
   float* X; float* F; unsigned long long* Z; unsigned long long* Pm;
   int cap, count, p;
-  __host__ __device__ static long long floats_per_chain(int T) { const long long cap = 10LL * (6 + T); return cap * (12 + T); }
-  __device__ void init(float* base, int T) {
-    cap = 10 * (6 + T); count = 0; p = 0;
+  // rounds x (6+T) entries: 10 for QuickVina2, 5 for QuickVina-W's individual buffer
+  __host__ __device__ static long long floats_per_chain(int T, int rounds = 10) {
+    const long long cap = (long long)rounds * (6 + T); return cap * (12 + T);
+  }
+  __device__ void init(float* base, int T, int rounds = 10) {
+    cap = rounds * (6 + T); count = 0; p = 0;
     X = base; F = base + (long long)cap * (7 + T);
     Z = reinterpret_cast<unsigned long long*>(F + cap); Pm = Z + cap;
   }
+};
+// Entries to examine: the first `avail` of a store laid out as History (row stride cap).
+struct HistView {
+  const float* X; const float* F; const unsigned long long* Z; const unsigned long long* Pm;
+  int cap, avail;
 };
 template <int G> __device__ __forceinline__ int hist_comp(int l, int T) {
   return l <= T ? l : (l >= G - 6 ? T + 1 + (l - (G - 6)) : -1);
@@ -625,11 +633,14 @@ __device__ void hist_add(History& h, const Conf& x, float f, float g, int l, int
   __syncthreads();                                 // entries visible to the whole wave
 }
 
-// true when (x, f, g) is worth a local search; every lane of the wave calls it.
+// Examines, nearest first (lowest index among equals), up to max_checks of the view's entries:
+// true as soon as one does not explain (x, f, g), else false; checked = entries examined.
+// Groups with need == false examine nothing.  Every lane of the wave calls it.
 template <int G>
-__device__ bool hist_interesting(const History& h, const Conf& x, float f, float g, int l, int T, bool on) {
-  const bool need = on && h.count == h.cap;
-  if (!__any(need)) return true;
+__device__ bool hist_check(const HistView& h, const Conf& x, float f, float g, int l, int T, bool need,
+                           int max_checks, int& checked) {
+  checked = 0;
+  if (!__any(need)) return false;
   const float coord = conf_coord<G>(x, l, T);
   float d[kHistMaxRounds];
 #pragma unroll
@@ -640,27 +651,28 @@ __device__ bool hist_interesting(const History& h, const Conf& x, float f, float
 #pragma unroll
     for (int r = 0; r < kHistMaxRounds; ++r) {
       const int e = l + G * r;
-      if (need && e < h.cap) { const float t = row[e] - xc; d[r] += t * t; }
+      if (need && e < h.avail) { const float t = row[e] - xc; d[r] += t * t; }
     }
   }
   const bool dim = is_dim_lane<G>(l, T);
   const int cl = hist_comp<G>(l, T);
   uint32_t picked = 0;
   bool found = false;
-  for (int it = 0; it < 6 + T && __any(need && !found); ++it) {
+  for (int it = 0; __any(need && !found && it < max_checks); ++it) {
     // next nearest entry: smallest distance, lowest index among equals
     float bd = INFINITY; int be = h.cap;
 #pragma unroll
     for (int r = 0; r < kHistMaxRounds; ++r) {
       const int e = l + G * r;
-      if (e < h.cap && !((picked >> r) & 1u) && d[r] < bd) { bd = d[r]; be = e; }
+      if (e < h.avail && !((picked >> r) & 1u) && d[r] < bd) { bd = d[r]; be = e; }
     }
 #pragma unroll
     for (int off = 1; off < G; off <<= 1) {
       const float od = __shfl_xor(bd, off, G); const int oe = __shfl_xor(be, off, G);
       if (od < bd || (od == bd && oe < be)) { bd = od; be = oe; }
     }
-    const bool pick = need && !found && be < h.cap;
+    const bool pick = need && !found && it < max_checks && be < h.avail;
+    if (pick) checked = it + 1;
     if (pick && be % G == l) picked |= 1u << (be / G);
     bool explains = false;
     if (pick && dim && !(g == 0.0f)) {
@@ -677,7 +689,88 @@ __device__ bool hist_interesting(const History& h, const Conf& x, float f, float
     const bool any_explains = group_ballot<G>(explains) != 0ull;
     if (pick && !any_explains) found = true;
   }
+  return need && found;
+}
+
+// QuickVina2's gate: true when (x, f, g) is worth a local search -- always until the history is
+// full, then unless each of the 6+T nearest entries explains it.  Every lane of the wave calls it.
+template <int G>
+__device__ bool hist_interesting(const History& h, const Conf& x, float f, float g, int l, int T, bool on) {
+  const bool need = on && h.count == h.cap;
+  int checked;
+  const bool found = hist_check<G>(HistView{h.X, h.F, h.Z, h.Pm, h.cap, h.cap}, x, f, g, l, T, need, 6 + T, checked);
   return !need || found;
+}
+
+// QuickVina-W's global buffer of one ligand: per_axis^3 cells over the box (x fastest), each a
+// ring of 8 entries laid out as History with cap 8, after an int counter per cell (adds so far).
+// Shared by every chain of the ligand through global memory: a slot is claimed with an atomic
+// add, then written; a reader may meet an entry while it is being written (the search heuristic
+// tolerates it; unwritten entries carry all-zero-derivative masks and never explain a point).
+struct GlobalHist {
+  float* base; int per_axis, T;
+  float lo[3], hi[3];
+  __device__ int cells() const { return per_axis * per_axis * per_axis; }
+  __device__ int cell_of(float px, float py, float pz) const {
+    const float p[3] = {px, py, pz};
+    int id[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float t = (p[k] - lo[k]) / (hi[k] - lo[k]) * (float)per_axis;   // NaN lands in cell 0
+      id[k] = t >= 0.0f ? (int)fminf(t, (float)(per_axis - 1)) : 0;
+    }
+    return (id[2] * per_axis + id[1]) * per_axis + id[0];
+  }
+  __device__ int* counter(int cell) const { return reinterpret_cast<int*>(base) + cell; }
+  __device__ HistView view(int cell, int avail) const {
+    float* c = base + cells() + (long long)cell * 8 * (12 + T);
+    const float* F = c + 8 * (7 + T);
+    const unsigned long long* Z = reinterpret_cast<const unsigned long long*>(F + 8);
+    return HistView{c, F, Z, Z + 8, 8, avail};
+  }
+  __host__ __device__ static long long floats(int per_axis, int T) {
+    const long long n = (long long)per_axis * per_axis * per_axis;
+    return n + n * 8 * (12 + T);
+  }
+};
+
+// Adds (x, f, g) to the global buffer cell of x's position when on.  Every lane calls it.
+template <int G>
+__device__ void glob_add(const GlobalHist& gh, const Conf& x, float f, float g, int l, int T, bool on) {
+  const bool dim = is_dim_lane<G>(l, T);
+  const unsigned long long z = group_ballot<G>(dim && g == 0.0f), ps = group_ballot<G>(dim && g > 0.0f);
+  const int cell = gh.cell_of(x.px, x.py, x.pz);
+  int slot = 0;
+  if (on && l == 0) slot = atomicAdd(gh.counter(cell), 1) & 7;
+  slot = ggeti<G>(slot, 0);
+  if (on) {
+    const HistView v = gh.view(cell, 8);
+    float* X = const_cast<float*>(v.X);
+    const int c = hist_comp<G>(l, T);
+    if (c >= 0) X[c * 8 + slot] = conf_coord<G>(x, l, T);
+    if (l == 0) {
+      const_cast<float*>(v.F)[slot] = f;
+      const_cast<unsigned long long*>(v.Z)[slot] = z;
+      const_cast<unsigned long long*>(v.Pm)[slot] = ps;
+    }
+  }
+}
+
+// QuickVina-W's gate (QVW/inc/cuda/quasi_newton.cuh:2236-2245, intended semantics): the G-check
+// examines up to 6+T nearest entries of the cell holding x; if none says interesting, the
+// individual buffer decides -- interesting until full, then by its 4(6+T) - (examined) nearest.
+template <int G>
+__device__ bool qvw_interesting(const History& h, const GlobalHist& gh, const Conf& x, float f, float g, int l, int T,
+                                bool on) {
+  const int cell = gh.cell_of(x.px, x.py, x.pz);
+  const int avail = on ? min(__atomic_load_n(gh.counter(cell), __ATOMIC_RELAXED), 8) : 0;
+  int checked = 0;
+  const bool glob = hist_check<G>(gh.view(cell, avail), x, f, g, l, T, on && avail > 0, 6 + T, checked);
+  const bool full = h.count == h.cap;
+  int c2;
+  const bool ind = hist_check<G>(HistView{h.X, h.F, h.Z, h.Pm, h.cap, h.cap}, x, f, g, l, T, on && !glob && full,
+                                 4 * (6 + T) - checked, c2);
+  return glob || !full || ind;
 }
 
 // ----------------------------------------------------------------------------------------

@@ -26,8 +26,10 @@ struct McArgs {
   const longlong4* out_off;         // per ligand slot: (e, conf, heavy) element offsets
   unsigned long long* slot_evals;    // per ligand slot: evaluations performed (zeroed by the caller)
   int Amax, Nmax;
-  float* hist;                      // QuickVina2 visited histories (nullptr: Vina policy),
+  float* hist;                      // QuickVina2 / QuickVina-W per-chain histories (nullptr: Vina policy),
   long long hist_stride;            // floats per chain region; region = block * chains/wave + group
+  float* glob;                      // QuickVina-W global buffers: ligand slot k at glob + out_off[k].w,
+                                    // header.reserved cells per axis
 };
 
 struct EvalArgs {
@@ -66,5 +68,6 @@ hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& 
 hipError_t launch_eval(int G, const EvalArgs& a, int blocks, hipStream_t s);
 hipError_t launch_mc(int G, int W, const McArgs& a, int blocks, hipStream_t s);
 hipError_t reset_counter(int* p, hipStream_t s);
+hipError_t glob_init(float* base, const long long* off, const int* per_axis, const int* T, int n, hipStream_t s);
 hipError_t random_confs(float* conf, long long n, int T, uint32_t seed, const float lo[3], const float hi[3], hipStream_t s);
 }  // namespace vdl

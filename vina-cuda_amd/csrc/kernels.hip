@@ -253,11 +253,14 @@ __device__ void write_chain(const LigView& L, const Scratch& S, const Conf& best
 // Chains of a wavefront thus never wait for each other's line searches or local searches; they
 // only wait at the end of the chain.  The gyration radius of each evaluated conformation comes
 // from the evaluation itself and travels with the conformation.
-// QV: QuickVina2 gate at the start of every local search (see History).
-template <int G, bool QV>
+// POL: 0 Vina; 1 QuickVina2 gate at the start of every local search (see History);
+// 2 QuickVina-W gate (qvw_interesting) with its records: the searched start point, each BFGS
+// iteration's end point and the point where the search stops go to the chain's own buffer, the
+// search's last point to the ligand's global buffer (QVW/inc/cuda/quasi_newton.cuh:2236-2410).
+template <int G, int POL>
 __device__ void mc_chain_flat(const LigView& L, const Scratch& S, const EvalK& K, const vd_mc_consts& P, int depth,
                               int max_steps, uint32_t stream, uint32_t chain, bool valid, int l, float* out_e,
-                              float* out_conf, float* out_heavy, int& evals, History* hist) {
+                              float* out_conf, float* out_heavy, int& evals, History* hist, const GlobalHist* gh) {
   const uint32_t k0 = P.seed, k1 = stream;
   const int n_ent = P.mutation_quirk ? (1 + L.T) : (2 + L.T);
   enum { kMutate = 0, kStart = 1, kTrial = 2, kDone = 3 };
@@ -313,13 +316,21 @@ __device__ void mc_chain_flat(const LigView& L, const Scratch& S, const EvalK& K
     if (trial) ++count;
     if (start) { fx_orig = fe; orig_rg = rge; stash_put<G>(S.stash, kSlotOrig, x); }
     bool search = start;
-    if constexpr (QV) {
+    if constexpr (POL == 1) {
       if (__any(start)) {
         const bool interesting = hist_interesting<G>(*hist, x, fx, g, l, L.T, start);
         hist_add<G>(*hist, x, fx, g, l, L.T, start && interesting);
         search = start && interesting;
       }
     }
+    if constexpr (POL == 2) {
+      if (__any(start)) {
+        const bool interesting = qvw_interesting<G>(*hist, *gh, x, fx, g, l, L.T, start);
+        hist_add<G>(*hist, x, fx, g, l, L.T, start && interesting);
+        search = start && interesting;
+      }
+    }
+    bool rec_own = false, rec_glob = false;   // QuickVina-W records of this pass
     if (__any(start)) {
       const float gg = gsum<G>(g * g);
       if (start) {
@@ -344,10 +355,13 @@ __device__ void mc_chain_flat(const LigView& L, const Scratch& S, const EvalK& K
       const float gnorm = sqrtf(gsum<G>(g * g));
       bool update = false;
       if (iter_end) {
-        if (result < 0) { x = stash_get<G>(S.stash, kSlotPrev); g = gp; fx = fxp; x_rg = prev_rg; ls_done = true; }
+        if (result < 0) {
+          x = stash_get<G>(S.stash, kSlotPrev); g = gp; fx = fxp; x_rg = prev_rg; ls_done = true;
+          rec_own = rec_glob = true;
+        }
         else if (!(gnorm >= 1e-5f)) ls_done = true;
-        else if ((max_steps != 0 && max_steps <= k) || k >= 1000) ls_done = true;   // 1000: hard bound
-        else { ++k; update = true; }
+        else if ((max_steps != 0 && max_steps <= k) || k >= 1000) { ls_done = true; rec_own = rec_glob = true; }   // 1000: hard bound
+        else { ++k; update = true; rec_own = true; }
       }
       const float sv = stp * d, y = g - gp;
       const float ys = gsum<G>(y * sv), yy = gsum<G>(y * y), ss = gsum<G>(sv * sv), gpgp = gsum<G>(gp * gp);
@@ -362,12 +376,16 @@ __device__ void mc_chain_flat(const LigView& L, const Scratch& S, const EvalK& K
       if (begin) { stash_put<G>(S.stash, kSlotPrev, x); gp = g; fxp = fx; prev_rg = x_rg; }
       const float dginit = gsum<G>(g * d);
       if (begin) {
-        if (0.0f < dginit) ls_done = true;       // no descent: the iteration fails, x stays
+        if (0.0f < dginit) { ls_done = true; rec_own = rec_glob = true; }   // no descent: the iteration fails, x stays
         else {
           finit = fx; dgtest = 1e-4f * dginit; dstest = 0.1f * dginit;
           mu = 0.0f; nu = 1.0e20f; brackt = false; count = 0; phase = kTrial;
         }
       }
+    }
+    if constexpr (POL == 2) {
+      if (__any(rec_own)) hist_add<G>(*hist, x, fx, g, l, L.T, rec_own);
+      if (__any(rec_glob)) glob_add<G>(*gh, x, fx, g, l, L.T, rec_glob);
     }
     if (ls_done) {           // end of a local search: keep the better of its start and its result
       if (!(fx <= fx_orig)) { fx = fx_orig; x = stash_get<G>(S.stash, kSlotOrig); x_rg = orig_rg; }
@@ -422,14 +440,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
     int evals = 0;
     History hist;
     if constexpr (MODE == 2) hist.init(a.hist + ((long long)blockIdx.x * CPW + group) * a.hist_stride, h.n_torsions);
+    if constexpr (MODE == 3) hist.init(a.hist + ((long long)blockIdx.x * CPW + group) * a.hist_stride, h.n_torsions, 5);
+    GlobalHist gh{};
+    if constexpr (MODE == 3) {
+      gh.base = a.glob + a.out_off[wi.x].w; gh.per_axis = h.reserved; gh.T = h.n_torsions;
+      for (int k = 0; k < 3; ++k) { gh.lo[k] = a.P.box_lo[k]; gh.hi[k] = a.P.box_hi[k]; }
+    }
     if constexpr (MODE == 0)
       mc_chain<G, MODE>(L, S, H, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l,
                         a.out_e + a.out_off[wi.x].x, a.out_conf + a.out_off[wi.x].y, a.out_heavy + a.out_off[wi.x].z,
                         evals, &hist);
     else
-      mc_chain_flat<G, MODE == 2>(L, S, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l,
-                                  a.out_e + a.out_off[wi.x].x, a.out_conf + a.out_off[wi.x].y,
-                                  a.out_heavy + a.out_off[wi.x].z, evals, &hist);
+      mc_chain_flat<G, MODE - 1>(L, S, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l,
+                                 a.out_e + a.out_off[wi.x].x, a.out_conf + a.out_off[wi.x].y,
+                                 a.out_heavy + a.out_off[wi.x].z, evals, &hist, &gh);
     if (a.slot_evals) {
       int sum = 0;
       for (int gi = 0; gi < CPW; ++gi) sum += __shfl(evals, gi * G);
@@ -573,6 +597,14 @@ hipError_t launch_mc(int G, int W, const McArgs& a, int blocks, hipStream_t s) {
       default: return launch_mc_mode<2, 2>(G, a, blocks, lds, s);
     }
   }
+  if (a.P.policy == 2) {
+    if (!a.hist || !a.glob) return hipErrorInvalidValue;
+    switch (W) {
+      case 3: return launch_mc_mode<3, 3>(G, a, blocks, lds, s);
+      case 4: return launch_mc_mode<3, 4>(G, a, blocks, lds, s);
+      default: return launch_mc_mode<3, 2>(G, a, blocks, lds, s);
+    }
+  }
   switch (W) {
     case 3: return launch_mc_mode<1, 3>(G, a, blocks, lds, s);
     case 4: return launch_mc_mode<1, 4>(G, a, blocks, lds, s);
@@ -581,6 +613,31 @@ hipError_t launch_mc(int G, int W, const McArgs& a, int blocks, hipStream_t s) {
 }
 
 __global__ void vd_fill_int(int* p, int v) { *p = v; }
+
+// Clears QuickVina-W global buffers: counters 0, conformation numbers 1e30 (never near),
+// energies 0, zero-derivative masks all ones (an entry nobody wrote never explains a point).
+__global__ void vd_glob_init_kernel(float* base, const long long* off, const int* per_axis, const int* T, int n) {
+  for (int k = blockIdx.y; k < n; k += gridDim.y) {
+  const long long cells = (long long)per_axis[k] * per_axis[k] * per_axis[k];
+  const int t = T[k];
+  float* b = base + off[k];
+  const long long per_cell = 8LL * (12 + t);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < cells + cells * per_cell;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (i < cells) { reinterpret_cast<int*>(b)[i] = 0; continue; }
+    const long long j = (i - cells) % per_cell;      // position inside the cell's record
+    float v = 0.0f;
+    if (j < 8LL * (7 + t)) v = 1e30f;
+    else if (j >= 8LL * (8 + t) && j < 8LL * (8 + t) + 16) v = __uint_as_float(0xFFFFFFFFu);   // Z masks
+    b[i] = v;
+  }
+  }
+}
+hipError_t glob_init(float* base, const long long* off, const int* per_axis, const int* T, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(vd_glob_init_kernel, dim3(64, std::min(n, 1024)), dim3(256), 0, s, base, off, per_axis, T, n);
+  return hipGetLastError();
+}
 
 // Random conformations of the adjoint benchmark: translation uniform in the box, orientation a
 // normalised 4-Gaussian (uniform on rotations), torsions uniform in [-pi, pi).
@@ -629,10 +686,11 @@ static int occupancy(int G, int lds_bytes) {
   return e == hipSuccess ? n : 0;
 }
 
-// mode: 0 dense BFGS, 1 RILC-BFGS, 2 RILC-BFGS with the QuickVina2 history
+// mode: 0 dense BFGS, 1 RILC-BFGS, 2 RILC-BFGS with the QuickVina2 history, 3 with QuickVina-W's
 int max_resident_mc_blocks(int G, int W, int mode, int lds_bytes) {
   if (mode == 0) return occupancy<0, 2>(G, lds_bytes);
   if (mode == 2) return W == 3 ? occupancy<2, 3>(G, lds_bytes) : W == 4 ? occupancy<2, 4>(G, lds_bytes) : occupancy<2, 2>(G, lds_bytes);
+  if (mode == 3) return W == 3 ? occupancy<3, 3>(G, lds_bytes) : W == 4 ? occupancy<3, 4>(G, lds_bytes) : occupancy<3, 2>(G, lds_bytes);
   return W == 3 ? occupancy<1, 3>(G, lds_bytes) : W == 4 ? occupancy<1, 4>(G, lds_bytes) : occupancy<1, 2>(G, lds_bytes);
 }
 

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libvdock.so")
 
 VD_OK, VD_ERR_ARG, VD_ERR_CAPACITY, VD_ERR_HIP, VD_ERR_PARSE, VD_ERR_NO_DEVICE = 0, -1, -2, -3, -4, -5
 VD_LS_BFGS, VD_LS_RILC_BFGS = 0, 1
-VD_MC_VINA, VD_MC_QVINA2 = 0, 1
+VD_MC_VINA, VD_MC_QVINA2, VD_MC_QVINAW = 0, 1, 2
 N_XS = 17
 N_TYPE_PAIRS = 153
 LUT_BINS = 2051
