@@ -16,6 +16,8 @@ Printed beside it on the same JSON line:
   post              the step after the search for the docked batch: pose lists, FP64 refinement of
                     every pose against the receptor atoms in one launch, final scores; beside it
                     the oracle's FP64 refinement of a sample of the same poses on host threads
+  quickvina_w       the same screening step with the QuickVina-W policy (evaluations saved by its
+                    gate, throughput, mean best score)
   large_box         large-box batched screening with the QuickVina2 policy (20,000-atom receptor,
                     188^3 maps, 4,096 ligands of 10-80 heavy atoms, 128 chains)
   cpu_baseline      the FP64 oracle's Monte Carlo (same schedule, same random stream) on host
@@ -355,6 +357,19 @@ def main():
                           "evals_per_s": round(n / (rates[best_cpw] / 1000.0), 1),
                           "ms_by_chains_per_wave": {str(k): round(v, 3) for k, v in rates.items()},
                           "finite_fraction": float(np.isfinite(e_host).mean())}
+
+    if not args.no_extras:
+        # the QuickVina-W policy on the same ligands and schedule (global + per-chain buffers)
+        log("QuickVina-W policy")
+        pq = vdock.default_params(chains=args.chains, seed=7, policy=vdock.VD_MC_QVINAW)
+        ctx.dock_batch(ids, pq)
+        t0 = time.perf_counter()
+        st_q = ctx.dock_batch(ids, pq)
+        t_q = time.perf_counter() - t0
+        best_q = ctx.batch_best()
+        out["quickvina_w"] = {"ligands_per_s": round(len(ids) / t_q, 3), "kernel_ms": round(st_q.kernel_ms, 2),
+                              "evals": int(st_q.evals), "evals_vs_vina_policy": round(st_q.evals / (evals / args.steps), 4),
+                              "mean_best_score": round(float(np.mean(best_q)), 4)}
 
     if not args.no_extras and not args.no_large_box:
         out["large_box"] = large_box_leg(vdock, log)
