@@ -190,3 +190,38 @@ def test_driver_runs_the_reference_example(ctx, site, tmp_path):
     assert [os.path.basename(r["out"]) for r in res] == ["a_out.pdbqt", "b_out.pdbqt"]
     assert all(os.path.exists(r["out"]) and r["modes"] for r in res)
     assert os.path.dirname(res[0]["out"]) == str(d) + "_out"
+
+
+@pytest.mark.parametrize("heavy,tors,seed", [(75, 14, 5), (12, 0, 6), (40, 30, 7)])
+def test_refine_matches_oracle_on_synthetic_ligands(ctx, heavy, tors, seed):
+    """Shapes beyond the reference example: more atoms than a wavefront has lanes (pairs in the
+    general chunk layout), a rigid ligand (no torsions), many torsions.  Docked poses of each,
+    refined on the device and by the oracle from the same FP32 starts."""
+    import vdock
+    xyz, xs = vdock.synth_receptor(3000, 40.0, 8.0, 11)
+    ctx.set_receptor(xyz, xs)
+    box = vdock.box_from_center([0, 0, 0], [18.0, 18.0, 18.0], 0.375)
+    ctx.grid_populate(box)
+    text = vdock.synth_ligand(heavy, tors, seed)
+    lid = ctx.add_ligand(text)
+    lig = orc.Ligand(text)
+    b, e, n = box.as_tuple()
+    g = orc.Grid(orc.Lut(), b, e, n)
+    g.set_receptor(xyz.astype(np.float64), xs)
+    p = vdock.default_params(chains=64, search_depth=4, bfgs_steps=8, seed=seed)
+    en, conf, _, _ = ctx.dock(lid, p)
+    starts = conf[np.argsort(en)[:6]].astype(np.float64)
+    co, out_xyz, res = ctx.refine(lid, starts)
+    de, dr = [], []
+    for i, c in enumerate(starts):
+        eo, _, xo = orc.refine(g, lig, c, orc.refine_steps(lig))
+        if res[i].inside:
+            de.append(abs(res[i].e - eo))
+            dr.append(rmsd(out_xyz[i], xo))
+            ei, ea = orc.noncache_eval(g, lig, out_xyz[i].astype(np.float64)), orc.intra_fast(g, lig, out_xyz[i].astype(np.float64))
+            assert abs(res[i].e_inter - ei) <= 1e-3 + 1e-5 * abs(ei) and abs(res[i].e_intra - ea) <= 1e-3 + 1e-5 * abs(ea)
+        else:
+            assert eo > 1e300
+    say(f"synthetic {heavy}/{tors}: atoms {lig.n_atoms} |de| {np.round(de, 6)} rmsd {np.round(dr, 5)}")
+    assert len(de) >= 4
+    assert np.median(de) <= 1e-4 and np.median(dr) <= 1e-3 and max(de) <= 0.1
