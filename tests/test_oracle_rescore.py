@@ -80,6 +80,36 @@ def test_rescoring_catches_planted_mistakes(site):
     assert abs(e_smooth - e_ok) > 1e-4                                              # tables differ
 
 
+def test_rescoring_pins_receptor_typing(site):
+    """The printed scores depend on the XS type of every receptor atom within 8 A of the poses,
+    so they pin the oracle's receptor typing (model::assign_types and bonded_to_HD /
+    bonded_to_heteroatom, V11/lib/model.cpp:716-797; the XS classes of V11/lib/atom_constants.h:
+    60-80, 257-270) for the classes around the pocket: each plausible slip
+    -- the carbon polarity rule either way, a missed N-H donor, an O donor/acceptor mix-up --
+    moves several of the nine scores beyond the print's rounding.  Classes with a handful of atoms
+    far from the pocket (N_P, N_A, S_P here) are not reached by these poses."""
+    g, lig, X, remarks = site
+    xyz, xs = orc.parse_receptor(golden("1u1b_protein.pdbqt"))[:2]
+    ref = np.array([r[0] for r in remarks])
+    C_H, C_P, N_P, N_D, O_A, O_DA = 0, 1, 2, 3, 8, 9
+
+    def worst(xs2, min_off):
+        g2 = orc.Grid(g.lut, g.begin, g.end, g.n)
+        g2.set_receptor(xyz, xs2)
+        bi = orc.intra_fast(g2, lig, X[0])
+        s = np.array([lig.conf_independent(orc.noncache_eval(g2, lig, x) + orc.intra_fast(g2, lig, x) - bi)
+                      for x in X])
+        d = np.abs(s - ref)
+        return d.max(), int((d > 0.06).sum()) >= min_off
+
+    assert worst(xs, 1)[0] <= 0.051
+    for a, b, n in [(C_P, C_H, 3), (C_H, C_P, 2), (N_D, N_P, 9), (O_DA, O_A, 5), (O_A, O_DA, 5)]:
+        xs2 = xs.copy()
+        xs2[xs == a] = b
+        assert (xs == a).sum() > 0
+        assert worst(xs2, n)[1], (a, b)
+
+
 def kabsch(P, Q):
     pc, qc = P.mean(0), Q.mean(0)
     U, _, Vt = np.linalg.svd((P - pc).T @ (Q - qc))
