@@ -84,7 +84,7 @@ def test_rescoring_pins_receptor_typing(site):
     """The printed scores depend on the XS type of every receptor atom within 8 A of the poses,
     so they pin the oracle's receptor typing (model::assign_types and bonded_to_HD /
     bonded_to_heteroatom, V11/lib/model.cpp:716-797; the XS classes of V11/lib/atom_constants.h:
-    60-80, 257-270) for the classes around the pocket: each plausible slip
+    66-83, 249-270) for the classes around the pocket: each plausible slip
     -- the carbon polarity rule either way, a missed N-H donor, an O donor/acceptor mix-up --
     moves several of the nine scores beyond the print's rounding.  Classes with a handful of atoms
     far from the pocket (N_P, N_A, S_P here) are not reached by these poses."""
