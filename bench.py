@@ -12,7 +12,10 @@ Printed beside it on the same JSON line:
   grid              17-map precompute of the same receptor/box: kernel time and GB/s
   adjoint           forward + adjoint evaluations of 2^20 random conformations of a 40-atom,
                     12-torsion ligand (evals/s)
-  roofline          the dominant kernel (the MC kernel) against HBM peak, algorithmic bytes
+  roofline          the dominant kernel (the MC kernel) against HBM peak, algorithmic bytes; a
+                    step's MC launches (one per lanes-per-chain class) run at the same time on
+                    their own streams, so the unit is the step's set of MC launches: their
+                    algorithmic bytes over their span (first start to last end, HIP events)
   post              the step after the search for the docked batch: pose lists, FP64 refinement of
                     every pose against the receptor atoms in one launch, final scores; beside it
                     the oracle's FP64 refinement of a sample of the same poses on host threads
@@ -108,6 +111,22 @@ def large_box_leg(vdock, log, n_ligands=4096, chains=128):
             "mean_best_score": round(float(np.mean(ctx.batch_best())), 4),
             "grid": {"points": npts, "maps_MB": round(gbytes / 1e6, 1), "kernel_ms": round(g_ms, 3),
                      "GB_per_s": round(gbytes / (g_ms / 1000.0) / 1e9, 2)}}
+
+
+def mc_step_spans(step_launches):
+    """Per timed step: the span of its MC launches (first start to last end, from the HIP events
+    recorded around each launch on its own stream), their summed algorithmic bytes and
+    evaluations, and the share of the span in which both launches ran."""
+    out = []
+    for L in step_launches:
+        t0 = min(r.start_ms for r in L)
+        t1 = max(r.start_ms + r.kernel_ms for r in L)
+        ends = sorted(r.start_ms + r.kernel_ms for r in L)
+        starts = sorted(r.start_ms for r in L)
+        both = max(0.0, ends[0] - starts[-1]) if len(L) > 1 else 0.0
+        out.append(dict(span=t1 - t0, bytes=sum(r.eval_bytes for r in L), evals=sum(r.evals for r in L),
+                        overlap=both / (t1 - t0)))
+    return out
 
 
 def traffic_from_profile(kernel, algorithmic_bytes):
@@ -245,16 +264,18 @@ def main():
     for w in range(args.warmup):
         log(f"warmup {w}")
         ctx.dock_batch(ids, params)
-    times, kernel_ms, evals, launches = [], 0.0, 0, []
+    times, kernel_ms, evals, launches, per_step = [], 0.0, 0, [], []
     st = None
     for s in range(args.steps):
         log(f"step {s}")
         t0 = time.perf_counter()
         st = ctx.dock_batch(ids, params)
         times.append(time.perf_counter() - t0)
-        kernel_ms += st.kernel_ms
         evals += st.evals
-        launches += ctx.last_launches()
+        per_step.append(ctx.last_launches())
+        launches += per_step[-1]
+    spans = mc_step_spans(per_step)
+    kernel_ms = sum(x["span"] for x in spans)
     wall = sum(times)
     value = args.ligands * args.steps / wall
     best = ctx.batch_best()
@@ -283,16 +304,22 @@ def main():
         "mean_best_score": round(float(np.mean(best)), 4),
     }
     name, d, by_kernel = dominant_kernel(launches)
-    ms_by_kernel = {k: round(v["ms"], 2) for k, v in by_kernel.items()}
-    traffic = traffic_from_profile(name, d["bytes"] / d["n"])
-    avg_ms = d["ms"] / d["n"]
-    per_launch = d["bytes"] / d["n"]
-    achieved = per_launch / (avg_ms / 1000.0) / 1e9
-    out["roofline"] = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2),
+    ms_by_kernel = {k: round(v["ms"] / v["n"], 3) for k, v in by_kernel.items()}
+    per_step_bytes = sum(x["bytes"] for x in spans) / len(spans)
+    avg_span = sum(x["span"] for x in spans) / len(spans)
+    achieved = per_step_bytes / (avg_span / 1000.0) / 1e9
+    # traffic of the step: each instantiation's measured bytes per algorithmic byte times its bytes
+    tr = [traffic_from_profile(k, v["bytes"] / args.steps) for k, v in by_kernel.items()]
+    traffic = round(sum(tr)) if all(t is not None for t in tr) else None
+    out["roofline"] = {"kernel": "vd_mc_kernel (the step's %d concurrent launches: %s)" % (
+                           len(per_step[0]), ", ".join(r.kernel for r in per_step[0])),
+                       "bound": "hbm", "achieved": round(achieved, 2),
                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                       "traffic": traffic, "launches": d["n"], "avg_launch_ms": round(avg_ms, 3),
-                       "algorithmic_bytes_per_launch": round(per_launch), "evals_per_launch": d["evals"] // d["n"],
-                       "kernel_ms_by_instantiation": ms_by_kernel, "mc_launches_per_step": len(launches) // args.steps,
+                       "traffic": traffic, "steps": len(spans), "avg_span_ms": round(avg_span, 3),
+                       "both_running_share_of_span": round(sum(x["overlap"] for x in spans) / len(spans), 4),
+                       "algorithmic_bytes_per_step": round(per_step_bytes),
+                       "evals_per_step": sum(x["evals"] for x in spans) // len(spans),
+                       "avg_launch_ms_by_instantiation": ms_by_kernel, "largest_share_instantiation": name,
                        "algorithmic_bytes_by_instantiation": {k: round(v["bytes"] / v["n"]) for k, v in by_kernel.items()},
                        "launches_by_instantiation": {k: v["n"] for k, v in by_kernel.items()}}
 

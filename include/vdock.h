@@ -138,7 +138,8 @@ int vd_dock_params_default(vd_dock_params* p);
 int vd_dock_heuristics(int n_atoms, int n_torsions, int rilc, int* search_depth, int* bfgs_steps);
 
 typedef struct {
-  double kernel_ms;            /* device time of the docking kernel(s) */
+  double kernel_ms;            /* device time of the docking kernel(s); launches that run at the
+                                  same time count once (first start to last end) */
   unsigned long long evals;    /* energy/gradient evaluations performed */
   int chains_per_wave;         /* used by the last launch (plain mode) */
   int blocks;                  /* workgroups launched */
@@ -157,9 +158,19 @@ typedef struct {
   double kernel_ms;
   unsigned long long evals;
   double eval_bytes;
+  double start_ms;   /* start of the kernel after the first enqueue of the call (launches of one
+                        batch overlap when their classes run at the same time) */
 } vd_launch;
 /* Copies up to max records; returns the number of launches (>= 0) or an error. */
 int vd_last_launches(vd_ctx* ctx, int max, vd_launch* out);
+/* on = 1 (default): when a vd_dock_batch call has two lanes-per-chain classes, their launches run
+ * at the same time, each on its own stream: the narrower class takes half of its resident blocks
+ * per CU and the wider class fills the rest, so that a class whose waves mostly wait shares every
+ * CU with one that keeps the vector units busy, and the wider class's waiting workgroups take
+ * over the slots the other gives up when its queue runs dry.  Batches of one class, or of three
+ * or more, run one class after the other.  on = 0: always one after the other.  No counterpart
+ * in the reference (one kernel_monte launch per ligand). */
+int vd_set_class_overlap(vd_ctx* ctx, int on);
 
 /* Plain per-ligand launch (replaces kernel_monte, V11/inc/cuda/kernel2.cuh:65-79).
  * Per chain results: e[chains], conf[chains][7+T], heavy[chains][n_heavy][3] (host pointers,

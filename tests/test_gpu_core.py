@@ -189,3 +189,40 @@ def test_batch_matches_plain(ctx, synth_setup):
     say("best batched", best_b, "plain", best_p)
     for a, b in zip(eb, ep):
         assert np.allclose(a, b, atol=1e-2, rtol=1e-3) or np.mean(np.isclose(a, b, atol=1e-2)) > 0.9
+
+
+@pytest.mark.parametrize("policy", [0, 1, 2])
+def test_class_overlap_same_results(ctx, synth_setup, policy):
+    """The lanes-per-chain classes of a batch run at the same time on their own streams (the
+    default) or one after the other: the same chains either way.  Each chain's trajectory depends
+    only on its ligand, chain index and seed (histories are per chain), so the Vina and QuickVina2
+    policies give the same results bit for bit; QuickVina-W's gate reads a buffer that all chains
+    of a ligand fill in the order they run, so there only completion and finite results are
+    checked.  With overlap the two launches' device intervals intersect."""
+    import vdock
+    s = synth_setup
+    ctx.grid_upload(s["box"], s["maps"])
+    ids = [ctx.add_ligand(vdock.synth_ligand(14 + 3 * k if k % 2 == 0 else 40 + k, 2 if k % 2 == 0 else 12, 300 + k))
+           for k in range(8)]
+    assert sorted({min(ctx.ligand_info(i).n_torsions, 10) for i in ids}) == [2, 10]
+    p = vdock.default_params(chains=64, search_depth=3, bfgs_steps=8, seed=5, chains_per_wave=4, waves_per_simd=3,
+                             policy=policy)
+    out = {}
+    for on in (False, True):
+        ctx.set_class_overlap(on)
+        say("class overlap", on, "policy", policy)
+        st = ctx.dock_batch(ids, p)
+        out[on] = ([ctx.batch_results(k, p) for k in range(len(ids))], ctx.last_launches(), st)
+    ctx.set_class_overlap(True)
+    launches = sorted(out[True][1], key=lambda r: r.start_ms)
+    assert sorted(r.lanes_per_chain for r in launches) == [16, 32]
+    a, b = launches
+    assert b.start_ms < a.start_ms + a.kernel_ms
+    for k in range(len(ids)):
+        e, conf, heavy = out[True][0][k]
+        assert np.isfinite(e).all() and np.isfinite(conf).all() and np.isfinite(heavy).all()
+        if policy != 2:
+            for x, y in zip(out[False][0][k], out[True][0][k]):
+                assert np.array_equal(x, y)
+    if policy != 2:
+        assert out[True][2].evals == out[False][2].evals

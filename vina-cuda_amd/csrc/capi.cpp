@@ -58,6 +58,52 @@ struct BatchState {   // results of the last batch, kept on the device
   DevBuf<float> e, conf, heavy;
 };
 
+struct Packed {
+  std::vector<float> blob;
+  std::vector<vd_lig_header> headers;
+  int Amax = 0, Nmax = 0, Tmax = 0;
+};
+
+// Device buffers, stream and events of one MC launch, and the host side of a launch in flight
+// (the sources of its asynchronous copies stay alive until it is collected).
+struct McSlot {
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DevBuf<float4> blob;
+  DevBuf<vd_lig_header> headers;
+  DevBuf<uint32_t> streams;
+  DevBuf<int2> items;
+  DevBuf<longlong4> offs;
+  DevBuf<int> queue;
+  DevBuf<unsigned long long> evals;
+  DevBuf<float> hist;                        // QuickVina2 / QuickVina-W per-chain histories
+  DevBuf<float> glob;                        // QuickVina-W global buffers
+  DevBuf<long long> glob_off;
+  DevBuf<int> glob_axis, glob_T;
+  Packed pk;
+  std::vector<vd_lig_header> h_headers;
+  std::vector<longlong4> h_offs;
+  std::vector<uint32_t> h_streams;
+  std::vector<int2> h_items;
+  std::vector<long long> h_goff;
+  std::vector<int> h_gaxis, h_gT;
+  std::vector<unsigned long long> h_evals;
+  McArgs a{};
+  int G = 0, W = 0, blocks = 0;
+  bool persistent = false, timed = false, busy = false;
+  vd_launch rec{};
+  void release() {
+    blob.release(); headers.release(); streams.release(); items.release(); offs.release(); queue.release();
+    evals.release(); hist.release(); glob.release(); glob_off.release(); glob_axis.release(); glob_T.release();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+    ev0 = ev1 = nullptr; stream = nullptr; own_stream = false;
+  }
+};
+constexpr int kMcSlots = 4;   // at most four hardware queues per process on these devices
+
 struct PostLigand {   // modes of one ligand after refinement and selection
   int id = 0;
   std::vector<vd_mode> modes;
@@ -87,24 +133,18 @@ struct vd_ctx {
   double grid_ms = 0, eval_ms = 0;
   // ligands
   std::vector<vd::Ligand> ligs;
-  // per-launch buffers
-  DevBuf<float4> blob;
+  // MC launches: slot 0 runs on `stream`, slots 1.. on their own streams (created on first use)
+  McSlot mc[kMcSlots];
+  bool overlap_classes = true;               // batch classes run at the same time
+  hipEvent_t ev_base = nullptr;
+  DevBuf<float4> blob;                       // ligand record of vd_eval
   DevBuf<vd_lig_header> headers;
-  DevBuf<uint32_t> streams;
-  DevBuf<int2> items;
-  DevBuf<longlong4> offs;
-  DevBuf<int> queue;
-  DevBuf<unsigned long long> evals;
   DevBuf<float> io_a, io_b, io_c;
   BatchState batch;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int n_cu = 0;
   std::map<int, std::pair<int, int>> tuned;  // size bucket -> (lanes per chain, waves per SIMD)
   DevBuf<float> tune_e, tune_conf, tune_heavy;
-  DevBuf<float> hist;                        // QuickVina2 / QuickVina-W per-chain histories
-  DevBuf<float> glob;                        // QuickVina-W global buffers of the last launch
-  DevBuf<long long> glob_meta_off;
-  DevBuf<int> glob_meta_axis, glob_meta_T;
   std::vector<vd_launch> launches;           // MC launches of the last docking call
   // after the search: receptor cells of the direct evaluation (built on first use per box)
   bool nc_ready = false;
@@ -283,11 +323,6 @@ vdk::EvalK make_evalk(vd_ctx* c, float v_intra, float v_inter, float slope) {
   return k;
 }
 
-struct Packed {
-  std::vector<float> blob;
-  std::vector<vd_lig_header> headers;
-  int Amax = 0, Nmax = 0, Tmax = 0;
-};
 // Ligand records for chain groups of G lanes.
 Packed pack(vd_ctx* c, int G, const std::vector<int>& ids, const std::vector<int>* depth = nullptr,
             const std::vector<int>* bfgs = nullptr) {
@@ -314,53 +349,48 @@ double elapsed_ms(vd_ctx* c) {
 // atom (8 x 4 B) and one interpolation quad of the pair table per intramolecular pair (16 B).
 double eval_bytes(const vd_lig_header& h) { return 32.0 * h.n_heavy + 16.0 * h.n_pairs; }
 
-// Runs one MC launch over the given ligand slots (all with the same G) and work items.
-void run_mc(vd_ctx* c, int G, int W, const Packed& pk, const std::vector<uint32_t>& streams, const std::vector<int2>& items,
-            const std::vector<longlong4>& off, const vd_mc_consts& P, float* d_e, float* d_conf, float* d_heavy,
-            bool persistent, vd_stats* st, bool timed) {
-  c->blob.ensure(pk.blob.size() / 4 + 1);
-  c->headers.ensure(pk.headers.size());
-  c->streams.ensure(streams.size());
-  c->items.ensure(items.size());
-  c->offs.ensure(off.size());
-  c->queue.ensure(1);
-  c->evals.ensure(pk.headers.size());
-  std::vector<vd_lig_header> headers(pk.headers);
-  std::vector<longlong4> offs(off);
+// One MC launch in parts, so that the launches of different lanes-per-chain classes of a batch
+// can run at the same time on their own streams: mc_prepare does the host work and sizes the
+// slot's device buffers (allocation may synchronise the device, so every launch of a batch is
+// prepared before any is enqueued); mc_enqueue_copies and mc_enqueue_kernel issue the copies and
+// the kernel on the slot's stream; mc_collect waits for it and reads its evaluation counts.
+void mc_prepare(vd_ctx* c, McSlot& s, int G, int W, Packed&& pk, std::vector<uint32_t>&& streams,
+                std::vector<int2>&& items, std::vector<longlong4>&& off, const vd_mc_consts& P, float* d_e,
+                float* d_conf, float* d_heavy, bool persistent, bool timed, double share = 1.0) {
+  s.pk = std::move(pk); s.h_streams = std::move(streams); s.h_items = std::move(items);
+  s.h_headers = s.pk.headers; s.h_offs = std::move(off);
+  s.h_goff.clear(); s.h_gaxis.clear(); s.h_gT.clear();
+  s.timed = timed;
   long long glob_floats = 0;
-  std::vector<long long> g_off;
-  std::vector<int> g_axis, g_T;
   if (P.policy == VD_MC_QVINAW) {
     // one global buffer per ligand slot, 2^levels cells per axis (levels from chains x depth)
-    for (size_t k = 0; k < headers.size(); ++k) {
-      const int axis = 1 << qvw_levels(P.chains, headers[k].search_depth);
-      headers[k].reserved = axis;
-      offs[k].w = glob_floats;
-      g_off.push_back(glob_floats); g_axis.push_back(axis); g_T.push_back(headers[k].n_torsions);
-      glob_floats += (vdk::GlobalHist::floats(axis, headers[k].n_torsions) + 63) / 64 * 64;
+    for (size_t k = 0; k < s.h_headers.size(); ++k) {
+      const int axis = 1 << qvw_levels(P.chains, s.h_headers[k].search_depth);
+      s.h_headers[k].reserved = axis;
+      s.h_offs[k].w = glob_floats;
+      s.h_goff.push_back(glob_floats); s.h_gaxis.push_back(axis); s.h_gT.push_back(s.h_headers[k].n_torsions);
+      glob_floats += (vdk::GlobalHist::floats(axis, s.h_headers[k].n_torsions) + 63) / 64 * 64;
     }
-    c->glob.ensure(glob_floats);
-    c->glob_meta_off.ensure(g_off.size()); c->glob_meta_axis.ensure(g_axis.size()); c->glob_meta_T.ensure(g_T.size());
-    HIP_CHECK(hipMemcpyAsync(c->glob_meta_off.p, g_off.data(), g_off.size() * 8, hipMemcpyHostToDevice, c->stream));
-    HIP_CHECK(hipMemcpyAsync(c->glob_meta_axis.p, g_axis.data(), g_axis.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_CHECK(hipMemcpyAsync(c->glob_meta_T.p, g_T.data(), g_T.size() * 4, hipMemcpyHostToDevice, c->stream));
-    HIP_CHECK(vdl::glob_init(c->glob.p, c->glob_meta_off.p, c->glob_meta_axis.p, c->glob_meta_T.p, (int)g_off.size(), c->stream));
+    s.glob.ensure(glob_floats);
+    s.glob_off.ensure(s.h_goff.size()); s.glob_axis.ensure(s.h_gaxis.size()); s.glob_T.ensure(s.h_gT.size());
   }
-  HIP_CHECK(hipMemcpyAsync(c->blob.p, pk.blob.data(), pk.blob.size() * 4, hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->headers.p, headers.data(), headers.size() * sizeof(vd_lig_header), hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->streams.p, streams.data(), streams.size() * 4, hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->items.p, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemcpyAsync(c->offs.p, offs.data(), offs.size() * sizeof(longlong4), hipMemcpyHostToDevice, c->stream));
-  HIP_CHECK(hipMemsetAsync(c->evals.p, 0, pk.headers.size() * sizeof(unsigned long long), c->stream));
-  McArgs a{};
-  a.blob = c->blob.p; a.headers = c->headers.p; a.streams = c->streams.p; a.items = c->items.p;
-  a.n_items = (int)items.size();
+  s.blob.ensure(s.pk.blob.size() / 4 + 1);
+  s.headers.ensure(s.h_headers.size());
+  s.streams.ensure(s.h_streams.size());
+  s.items.ensure(s.h_items.size());
+  s.offs.ensure(s.h_offs.size());
+  s.queue.ensure(1);
+  s.evals.ensure(s.h_headers.size());
+  McArgs& a = s.a;
+  a = McArgs{};
+  a.blob = s.blob.p; a.headers = s.headers.p; a.streams = s.streams.p; a.items = s.items.p;
+  a.n_items = (int)s.h_items.size();
   a.K = make_evalk(c, P.v_intra, P.v_inter, P.slope);
   a.P = P;
-  a.out_e = d_e; a.out_conf = d_conf; a.out_heavy = d_heavy; a.out_off = c->offs.p;
-  a.slot_evals = c->evals.p;
-  a.Amax = pk.Amax; a.Nmax = pk.Nmax;
-  const int lds = vdl::mc_lds_bytes(G, pk.Amax, pk.Nmax, P.local_search == 0);
+  a.out_e = d_e; a.out_conf = d_conf; a.out_heavy = d_heavy; a.out_off = s.offs.p;
+  a.slot_evals = s.evals.p;
+  a.Amax = s.pk.Amax; a.Nmax = s.pk.Nmax;
+  const int lds = vdl::mc_lds_bytes(G, s.pk.Amax, s.pk.Nmax, P.local_search == 0);
   if (lds > 64 * 1024) throw ArgError(VD_ERR_CAPACITY, "ligand too large for the LDS layout");
   const int mode = P.local_search == 0 ? 0 : (P.policy == VD_MC_QVINA2 ? 2 : P.policy == VD_MC_QVINAW ? 3 : 1);
   // a tighter register budget only pays when the LDS of this launch lets the extra waves in:
@@ -370,44 +400,97 @@ void run_mc(vd_ctx* c, int G, int W, const Packed& pk, const std::vector<uint32_
     for (int w = 2; w < W; ++w)
       if (vdl::max_resident_mc_blocks(G, w, mode, lds) >= occ) { W = w; break; }
   }
-  int blocks = (int)items.size();
+  int blocks = (int)s.h_items.size();
   if (persistent) {
-    a.queue = c->queue.p;
-    HIP_CHECK(vdl::reset_counter(c->queue.p, c->stream));
-    const int per_cu = std::max(1, vdl::max_resident_mc_blocks(G, W, mode, lds));
-    blocks = std::min<long long>((long long)per_cu * c->n_cu, (long long)items.size());
+    a.queue = s.queue.p;
+    const int per_cu = std::max(1, (int)std::ceil(share * vdl::max_resident_mc_blocks(G, W, mode, lds)));
+    blocks = std::min<long long>((long long)per_cu * c->n_cu, (long long)s.h_items.size());
     blocks = std::max(blocks, 1);
   }
   if (P.policy == VD_MC_QVINA2 || P.policy == VD_MC_QVINAW) {
     // one visited-history region per chain slot of every launched block (blocks keep their
     // slot for all the items they take from the queue)
-    a.hist_stride = vdk::History::floats_per_chain(pk.Tmax, P.policy == VD_MC_QVINAW ? 5 : 10);
-    c->hist.ensure((size_t)blocks * (64 / G) * a.hist_stride);
-    a.hist = c->hist.p;
+    a.hist_stride = vdk::History::floats_per_chain(s.pk.Tmax, P.policy == VD_MC_QVINAW ? 5 : 10);
+    s.hist.ensure((size_t)blocks * (64 / G) * a.hist_stride);
+    a.hist = s.hist.p;
   }
-  if (P.policy == VD_MC_QVINAW) a.glob = c->glob.p;
-  if (timed) HIP_CHECK(hipEventRecord(c->ev0, c->stream));
-  HIP_CHECK(vdl::launch_mc(G, W, a, blocks, c->stream));
-  if (timed) HIP_CHECK(hipEventRecord(c->ev1, c->stream));
-  std::vector<unsigned long long> ev(pk.headers.size());
-  HIP_CHECK(hipMemcpyAsync(ev.data(), c->evals.p, ev.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
-  HIP_CHECK(hipStreamSynchronize(c->stream));
-  vd_launch rec{};
-  rec.lanes_per_chain = G; rec.waves_per_simd = W; rec.blocks = blocks; rec.items = (int)items.size();
-  rec.ligands = (int)pk.headers.size(); rec.persistent = persistent ? 1 : 0;
-  rec.mode = mode;
-  rec.kernel_ms = timed ? elapsed_ms(c) : 0.0;
-  for (size_t k = 0; k < ev.size(); ++k) {
-    rec.evals += ev[k];
-    rec.eval_bytes += (double)ev[k] * eval_bytes(pk.headers[k]);
+  if (P.policy == VD_MC_QVINAW) a.glob = s.glob.p;
+  s.G = G; s.W = W; s.blocks = blocks; s.persistent = persistent;
+  s.rec = vd_launch{};
+  s.rec.lanes_per_chain = G; s.rec.waves_per_simd = W; s.rec.blocks = blocks; s.rec.items = (int)s.h_items.size();
+  s.rec.ligands = (int)s.h_headers.size(); s.rec.persistent = persistent ? 1 : 0;
+  s.rec.mode = mode;
+  s.h_evals.assign(s.h_headers.size(), 0);
+}
+
+void mc_enqueue_copies(McSlot& s) {
+  hipStream_t st = s.stream;
+  if (!s.h_goff.empty()) {
+    HIP_CHECK(hipMemcpyAsync(s.glob_off.p, s.h_goff.data(), s.h_goff.size() * 8, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(s.glob_axis.p, s.h_gaxis.data(), s.h_gaxis.size() * 4, hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipMemcpyAsync(s.glob_T.p, s.h_gT.data(), s.h_gT.size() * 4, hipMemcpyHostToDevice, st));
+    HIP_CHECK(vdl::glob_init(s.glob.p, s.glob_off.p, s.glob_axis.p, s.glob_T.p, (int)s.h_goff.size(), st));
   }
-  if (timed) c->launches.push_back(rec);
+  HIP_CHECK(hipMemcpyAsync(s.blob.p, s.pk.blob.data(), s.pk.blob.size() * 4, hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(s.headers.p, s.h_headers.data(), s.h_headers.size() * sizeof(vd_lig_header), hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(s.streams.p, s.h_streams.data(), s.h_streams.size() * 4, hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(s.items.p, s.h_items.data(), s.h_items.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemcpyAsync(s.offs.p, s.h_offs.data(), s.h_offs.size() * sizeof(longlong4), hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipMemsetAsync(s.evals.p, 0, s.h_headers.size() * sizeof(unsigned long long), st));
+  if (s.persistent) HIP_CHECK(vdl::reset_counter(s.queue.p, st));
+}
+
+// The kernel.  Copies may run as kernels that need a
+// CU slot, so every slot's copies are enqueued before any slot's kernel: a persistent grid
+// already holding the CUs would otherwise hold back the next slot's copies until it ends.
+void mc_enqueue_kernel(McSlot& s) {
+  hipStream_t st = s.stream;
+  if (s.timed) HIP_CHECK(hipEventRecord(s.ev0, st));
+  HIP_CHECK(vdl::launch_mc(s.G, s.W, s.a, s.blocks, st));
+  if (s.timed) HIP_CHECK(hipEventRecord(s.ev1, st));
+  s.busy = true;
+}
+
+// base: an event recorded before every launch of the call (start_ms is measured from it), or null.
+void mc_collect(vd_ctx* c, McSlot& s, vd_stats* st, hipEvent_t base, bool add_ms = true) {
+  if (!s.busy) return;
+  s.busy = false;
+  // read back after the wait: a copy into pageable host memory blocks the calling thread until
+  // it is done, which would hold back the other slots' kernels
+  HIP_CHECK(hipStreamSynchronize(s.stream));
+  HIP_CHECK(hipMemcpyAsync(s.h_evals.data(), s.evals.p, s.h_evals.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s.stream));
+  HIP_CHECK(hipStreamSynchronize(s.stream));
+  vd_launch& rec = s.rec;
+  if (s.timed) {
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
+    rec.kernel_ms = ms;
+    if (base) { HIP_CHECK(hipEventElapsedTime(&ms, base, s.ev0)); rec.start_ms = ms; }
+  }
+  for (size_t k = 0; k < s.h_evals.size(); ++k) {
+    rec.evals += s.h_evals[k];
+    rec.eval_bytes += (double)s.h_evals[k] * eval_bytes(s.h_headers[k]);
+  }
+  if (s.timed) c->launches.push_back(rec);
   if (st) {
     st->evals += rec.evals;
-    st->blocks += blocks;
-    st->chains_per_wave = 64 / G;
-    st->kernel_ms += rec.kernel_ms;
+    st->blocks += s.blocks;
+    st->chains_per_wave = 64 / s.G;
+    if (add_ms) st->kernel_ms += rec.kernel_ms;
   }
+}
+
+// Runs one MC launch over the given ligand slots (all with the same G) and work items, on the
+// context's stream, and waits for it.
+void run_mc(vd_ctx* c, int G, int W, const Packed& pk, const std::vector<uint32_t>& streams, const std::vector<int2>& items,
+            const std::vector<longlong4>& off, const vd_mc_consts& P, float* d_e, float* d_conf, float* d_heavy,
+            bool persistent, vd_stats* st, bool timed) {
+  McSlot& s = c->mc[0];
+  mc_prepare(c, s, G, W, Packed(pk), std::vector<uint32_t>(streams), std::vector<int2>(items),
+             std::vector<longlong4>(off), P, d_e, d_conf, d_heavy, persistent, timed);
+  mc_enqueue_copies(s);
+  mc_enqueue_kernel(s);
+  mc_collect(c, s, st, nullptr);
 }
 
 struct LigPlan { int id, G, W, depth, bfgs, T, H, chains; };
@@ -801,6 +884,12 @@ int vd_create(int device, vd_ctx** out) {
     HIP_CHECK(hipStreamCreate(&c->stream));
     HIP_CHECK(hipEventCreate(&c->ev0));
     HIP_CHECK(hipEventCreate(&c->ev1));
+    HIP_CHECK(hipEventCreate(&c->ev_base));
+    for (int k = 0; k < kMcSlots; ++k) {
+      HIP_CHECK(hipEventCreate(&c->mc[k].ev0));
+      HIP_CHECK(hipEventCreate(&c->mc[k].ev1));
+    }
+    c->mc[0].stream = c->stream;
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, device));
     c->n_cu = prop.multiProcessorCount;
@@ -817,11 +906,14 @@ void vd_destroy(vd_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   c->lut_rows.release(); c->lutq.release(); c->maps.release(); c->grid_atoms.release(); c->grid_cells.release();
-  c->blob.release(); c->headers.release(); c->streams.release(); c->items.release(); c->offs.release();
-  c->queue.release(); c->evals.release(); c->io_a.release(); c->io_b.release(); c->io_c.release();
+  for (int k = 0; k < kMcSlots; ++k) {
+    if (c->mc[k].stream) (void)hipStreamSynchronize(c->mc[k].stream);
+    c->mc[k].release();
+  }
+  if (c->ev_base) (void)hipEventDestroy(c->ev_base);
+  c->blob.release(); c->headers.release(); c->io_a.release(); c->io_b.release(); c->io_c.release();
   c->batch.e.release(); c->batch.conf.release(); c->batch.heavy.release();
-  c->tune_e.release(); c->tune_conf.release(); c->tune_heavy.release(); c->hist.release();
-  c->glob.release(); c->glob_meta_off.release(); c->glob_meta_axis.release(); c->glob_meta_T.release();
+  c->tune_e.release(); c->tune_conf.release(); c->tune_heavy.release();
   c->nc_atoms.release(); c->nc_start.release(); c->ref_in.release(); c->ref_out.release(); c->ref_xyz.release();
   c->ref_res.release(); c->ref_lig.release(); c->ref_evals.release(); c->ref_coff.release(); c->ref_xoff.release();
   c->lutd.release(); c->ref_geo.release(); c->ref_pairs.release(); c->ref_hdr.release();
@@ -1149,9 +1241,34 @@ static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, 
       }
       return;
     }
-    // one persistent launch per group width; items ordered by decreasing estimated cost
+    // one persistent launch per (group width, register budget) class; items ordered by decreasing
+    // estimated cost.  With class overlap the launches are all prepared first (buffer growth may
+    // synchronise the device), then enqueued on one stream each, then collected.
     std::map<std::pair<int, int>, std::vector<int>> classes;
     for (int k = 0; k < n; ++k) classes[std::make_pair(plans[k].G, plans[k].W)].push_back(k);
+    // Overlap applies to batches of exactly two classes.  Measured on the screening batch (a
+    // 16-lane class whose waves mostly wait, a 64-lane class that keeps the vector units busy):
+    // 3 % shorter steps.  With more classes it lost: on a 4,096-ligand batch of five classes a
+    // fourth stream shared a hardware queue with the third (four queues per process), and capped
+    // classes that outlast the uncapped one keep half their CU slots empty to the end.
+    const bool overlap = c->overlap_classes && classes.size() == 2;
+    int used = 0;
+    auto flush = [&]() {
+      if (used == 0) return;
+      HIP_CHECK(hipEventRecord(c->ev_base, c->stream));
+      for (int k = 1; k < used; ++k) HIP_CHECK(hipStreamWaitEvent(c->mc[k].stream, c->ev_base, 0));
+      for (int k = 0; k < used; ++k) mc_enqueue_copies(c->mc[k]);
+      for (int k = 0; k < used; ++k) mc_enqueue_kernel(c->mc[k]);
+      // launches that ran at the same time count once: device time from first start to last end
+      double t0 = 1e300, t1 = -1e300;
+      for (int k = 0; k < used; ++k) {
+        mc_collect(c, c->mc[k], st, c->ev_base, false);
+        t0 = std::min(t0, c->mc[k].rec.start_ms);
+        t1 = std::max(t1, c->mc[k].rec.start_ms + c->mc[k].rec.kernel_ms);
+      }
+      if (st) st->kernel_ms += t1 - t0;
+      used = 0;
+    };
     for (auto& cl : classes) {
       const int G = cl.first.first, W = cl.first.second;
       std::vector<int>& ks = cl.second;
@@ -1170,10 +1287,29 @@ static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, 
       std::vector<int2> items;
       for (auto& o : order)
         for (int c0 = 0; c0 < p->chains; c0 += cpw) items.push_back(make_int2(o.second, c0));
-      run_mc(c, G, W, pk, streams, items, off, make_consts(c, p, 0, 0), c->batch.e.p, c->batch.conf.p, c->batch.heavy.p,
-             true, st, true);
+      if (used == (overlap ? kMcSlots : 1)) flush();
+      McSlot& s = c->mc[used];
+      if (!s.stream) { HIP_CHECK(hipStreamCreate(&s.stream)); s.own_stream = true; }
+      // with overlap, every class but the last (the widest groups) takes half of its resident
+      // blocks per CU, which leaves room for the last class's waves beside it; the last class
+      // launches its full grid, whose blocks that do not fit at first take over the CU slots the
+      // others give up when their queues run dry.  Measured on the screening batch (16- and
+      // 64-lane classes): 0.5 to 0.75 of the residency are equally good, below 0.4 the first
+      // class runs too long.
+      const bool last = std::next(classes.find(cl.first)) == classes.end();
+      const double share = (overlap && !last) ? 0.5 : 1.0;
+      mc_prepare(c, s, G, W, std::move(pk), std::move(streams), std::move(items), std::move(off),
+                 make_consts(c, p, 0, 0), c->batch.e.p, c->batch.conf.p, c->batch.heavy.p, true, true, share);
+      ++used;
     }
+    flush();
   });
+}
+
+int vd_set_class_overlap(vd_ctx* c, int on) {
+  if (!c || (on != 0 && on != 1)) return VD_ERR_ARG;
+  c->overlap_classes = on == 1;
+  return VD_OK;
 }
 
 int vd_dock_batch(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, vd_stats* st) {

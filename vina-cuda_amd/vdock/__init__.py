@@ -29,7 +29,7 @@ EXPORTS = (
     "vd_eval", "vd_eval_last_ms", "vd_dock_params_default", "vd_dock_heuristics", "vd_dock",
     "vd_dock_batch", "vd_dock_batch_plain", "vd_batch_results", "vd_batch_best", "vd_cluster",
     "vd_synth_receptor", "vd_synth_ligand", "vd_device_alloc", "vd_device_free", "vd_copy", "vd_random_confs",
-    "vd_last_launches", "vd_ligand_pair_chunks",
+    "vd_last_launches", "vd_set_class_overlap", "vd_ligand_pair_chunks",
     "vd_post_params_default", "vd_refine", "vd_select_modes", "vd_postprocess_batch", "vd_postprocess_docked",
     "vd_post_results", "vd_write_pdbqt",
 )
@@ -85,7 +85,7 @@ class Mode(C.Structure):
 class Launch(C.Structure):
     _fields_ = [("lanes_per_chain", C.c_int), ("mode", C.c_int), ("waves_per_simd", C.c_int), ("blocks", C.c_int),
                 ("items", C.c_int), ("ligands", C.c_int), ("persistent", C.c_int), ("kernel_ms", C.c_double),
-                ("evals", C.c_ulonglong), ("eval_bytes", C.c_double)]
+                ("evals", C.c_ulonglong), ("eval_bytes", C.c_double), ("start_ms", C.c_double)]
 
     @property
     def kernel(self):
@@ -134,6 +134,7 @@ def load():
         "vd_batch_results": (C.c_int, [vp, C.c_int, fp, fp, fp]),
         "vd_batch_best": (C.c_int, [vp, fp]),
         "vd_last_launches": (C.c_int, [vp, C.c_int, C.POINTER(Launch)]),
+        "vd_set_class_overlap": (C.c_int, [vp, C.c_int]),
         "vd_ligand_pair_chunks": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32)]),
         "vd_cluster": (C.c_int, [C.c_int, C.c_int, fp, fp, C.c_double, C.c_int, ip]),
         "vd_device_alloc": (C.c_int, [vp, C.c_size_t, C.POINTER(C.c_void_p)]),
@@ -395,6 +396,11 @@ class Context:
         out = np.zeros(max(n, 1), dtype=np.uint32)
         self._check(min(self.lib.vd_ligand_pair_chunks(self.h, lid, lanes, n, out.ctypes.data_as(C.POINTER(C.c_uint32))), 0))
         return out[:n].reshape(-1, lanes)
+
+    def set_class_overlap(self, on):
+        """Run the lanes-per-chain classes of a batch at the same time (True, the default) or one
+        after the other (False)."""
+        self._check(self.lib.vd_set_class_overlap(self.h, 1 if on else 0))
 
     def last_launches(self):
         """MC kernel launches of the last docking call (list of Launch)."""
