@@ -12,7 +12,8 @@ Printed beside it on the same JSON line:
   grid              17-map precompute of the same receptor/box: kernel time and GB/s
   adjoint           forward + adjoint evaluations of 2^20 random conformations of a 40-atom,
                     12-torsion ligand (evals/s)
-  roofline          the dominant kernel (the MC kernel) against HBM peak, algorithmic bytes; a
+  roofline          the dominant kernel (the MC kernel) against HBM peak (8 TB/s, and beside it
+                    the bandwidth a timed 1 GiB device-to-device copy reaches), algorithmic bytes; a
                     step's MC launches (one per lanes-per-chain class) run at the same time on
                     their own streams, so the unit is the step's set of MC launches: their
                     algorithmic bytes over their span (first start to last end, HIP events)
@@ -113,6 +114,35 @@ def large_box_leg(vdock, log, n_ligands=4096, chains=128):
                      "GB_per_s": round(gbytes / (g_ms / 1000.0) / 1e9, 2)}}
 
 
+def copy_bandwidth(ctx, nbytes=1 << 30, reps=5):
+    """HBM bandwidth a plain device-to-device copy reaches on this device: bytes read plus bytes
+    written over the best of `reps` timed copies (each call waits for its copy)."""
+    src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+    try:
+        ctx.copy_device(dst, src, nbytes)
+        best = min(_timed(lambda: ctx.copy_device(dst, src, nbytes)) for _ in range(reps))
+    finally:
+        ctx.free(src)
+        ctx.free(dst)
+    return 2 * nbytes / best / 1e9
+
+
+def _timed(f):
+    t0 = time.perf_counter()
+    f()
+    return time.perf_counter() - t0
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def mc_step_spans(step_launches):
     """Per timed step: the span of its MC launches (first start to last end, from the HIP events
     recorded around each launch on its own stream), their summed algorithmic bytes and
@@ -168,7 +198,7 @@ def run_cpu_baseline(vdock, texts, box, xyz, xs, chains, budget_s=15.0):
         done += 1
         if t_mc >= budget_s:
             break
-    return dict(value=done / t_mc, unit="ligands/s", cores=threads, kind="port",
+    return dict(value=done / t_mc, unit="ligands/s", cores=threads, kind="port", cpu_model=cpu_model(),
                 sample=f"first {done} of the screening ligands, {chains} chains each, FP64 oracle MC "
                        f"(same schedule and heuristics) on {threads} host threads; maps built once in {t_grid:.1f} s",
                 evals_per_s=evals / t_mc)
@@ -311,11 +341,14 @@ def main():
     # traffic of the step: each instantiation's measured bytes per algorithmic byte times its bytes
     tr = [traffic_from_profile(k, v["bytes"] / args.steps) for k, v in by_kernel.items()]
     traffic = round(sum(tr)) if all(t is not None for t in tr) else None
+    copy_gbs = copy_bandwidth(ctx)
     out["roofline"] = {"kernel": "vd_mc_kernel (the step's %d concurrent launches: %s)" % (
                            len(per_step[0]), ", ".join(r.kernel for r in per_step[0])),
                        "bound": "hbm", "achieved": round(achieved, 2),
                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                       "traffic": traffic, "steps": len(spans), "avg_span_ms": round(avg_span, 3),
+                       "traffic": traffic, "measured_copy_GBs": round(copy_gbs, 1),
+                       "frac_of_measured_copy": round(achieved / copy_gbs, 5),
+                       "steps": len(spans), "avg_span_ms": round(avg_span, 3),
                        "both_running_share_of_span": round(sum(x["overlap"] for x in spans) / len(spans), 4),
                        "algorithmic_bytes_per_step": round(per_step_bytes),
                        "evals_per_step": sum(x["evals"] for x in spans) // len(spans),
