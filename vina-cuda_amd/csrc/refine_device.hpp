@@ -20,6 +20,12 @@
 #include <stdint.h>
 #include "vd_common.h"
 
+// Every FP64 operation of this file is rounded on its own, as in the CPU code it restates: no
+// multiply and add are fused into an FMA, in the small helpers as in the kernel bodies (hipcc
+// fuses by default, and which products it fuses changes with the surrounding code).  The
+// compiler's default is restored at the end of the file.
+#pragma clang fp contract(off)
+
 namespace vdr {
 
 constexpr double kEps = 2.220446049250313e-16;      // epsilon_fl
@@ -111,7 +117,8 @@ __device__ __forceinline__ double lane_sum(double v) {
 }
 
 // LDS of one pose (doubles): atoms 4A, frames 20N (R row-major 9, origin 3, lab axis 3, q 4, pad),
-// xyz 4A, frc 4A, node sums 8N, torsions 64, gradient 64, three vector buffers 3 x 64, H n x n.
+// xyz 4A, frc 4A, node sums 8N, torsion half-angle cosines 64 and sines 64, gradient 64, three
+// vector buffers 3 x 64, H n x n.
 struct Pose {
   const double4* atom;      // LDS copy of the atom words
   const double4* node_o;    // global: rel_origin xyz + (parent + 1) | begin << 8 | end << 16
@@ -119,9 +126,9 @@ struct Pose {
   const uint32_t* pairs;    // global
   int A, N, T, C, n;
   double* frame; double4* xyz; double4* frc; double4* nsum;
-  double* tor; double* grad; double* vb; double* H;
+  double* tcos; double* tsin; double* grad; double* vb; double* H;
 };
-__host__ __device__ inline int pose_lds_doubles(int A, int N, int n) { return 12 * A + 28 * N + 5 * 64 + n * n; }
+__host__ __device__ inline int pose_lds_doubles(int A, int N, int n) { return 12 * A + 28 * N + 6 * 64 + n * n; }
 
 __device__ inline void carve(Pose& P, double* lds, int Amax, int Nmax) {
   double* p = lds;
@@ -130,7 +137,8 @@ __device__ inline void carve(Pose& P, double* lds, int Amax, int Nmax) {
   P.xyz = reinterpret_cast<double4*>(p); p += 4 * Amax;
   P.frc = reinterpret_cast<double4*>(p); p += 4 * Amax;
   P.nsum = reinterpret_cast<double4*>(p); p += 8 * Nmax;
-  P.tor = p; p += 64;
+  P.tcos = p; p += 64;
+  P.tsin = p; p += 64;
   P.grad = p; p += 64;
   P.vb = p; p += 3 * 64;
   P.H = p;
@@ -142,10 +150,15 @@ struct Conf { D3 pos; DQ q; double tor; };
 // model::set for one pose (tree.h:205-209, 282-292 as the oracle's set_conf restates it):
 // lane 0 walks the nodes in preorder, q_k = angle_to_quaternion(R_parent rel_axis, theta_k) *
 // q_parent (normalize_approx), o_k = o_parent + R_parent rel_origin; then every lane places its
-// atoms, x_i = o_node + R_node local_i.  Leaves frames and absolute coordinates in LDS.
+// atoms, x_i = o_node + R_node local_i.  The half-angle cosine and sine of angle_to_quaternion
+// are computed beforehand by each torsion's own lane, so the serial walk only reads them.
+// Leaves frames and absolute coordinates in LDS.
 __device__ inline void set_coords(const Pose& P, const Conf& c, int l) {
-#pragma clang fp contract(off)
-  if (l >= 6 && l < 6 + P.T) P.tor[l - 6] = c.tor;
+  if (l >= 6 && l < 6 + P.T) {
+    const double a = normalize_angle(c.tor);
+    P.tcos[l - 6] = cos(a / 2);
+    P.tsin[l - 6] = sin(a / 2);
+  }
   __syncthreads();
   if (l == 0) {
     double* f = P.frame;
@@ -159,7 +172,8 @@ __device__ inline void set_coords(const Pose& P, const Conf& c, int l) {
       double* fk = P.frame + 20 * k;
       const D3 axis = mat_vec(fp, D3{na.x, na.y, na.z});
       const D3 org = add(D3{fp[9], fp[10], fp[11]}, mat_vec(fp, D3{no.x, no.y, no.z}));
-      const DQ q = normalize_approx(qmul(axis_angle(axis, P.tor[k - 1]), DQ{fp[15], fp[16], fp[17], fp[18]}));
+      const double hc = P.tcos[k - 1], hs = P.tsin[k - 1];
+      const DQ q = normalize_approx(qmul(DQ{hc, hs * axis.x, hs * axis.y, hs * axis.z}, DQ{fp[15], fp[16], fp[17], fp[18]}));
       qmat(q, fk);
       fk[9] = org.x; fk[10] = org.y; fk[11] = org.z; fk[12] = axis.x; fk[13] = axis.y; fk[14] = axis.z;
       fk[15] = q.a; fk[16] = q.b; fk[17] = q.c; fk[18] = q.d;
@@ -183,7 +197,6 @@ __device__ inline void set_coords(const Pose& P, const Conf& c, int l) {
 // Returns this lane's share of the energy.
 template <bool FAST>
 __device__ inline double nc_inter(const Pose& P, const Consts& K, double slope, int l) {
-#pragma clang fp contract(off)
   const int row = l >> 4, rl = l & 15;
   double e_lane = 0;
   for (int i0 = 0; i0 < P.A; i0 += 4) {
@@ -204,23 +217,40 @@ __device__ inline double nc_inter(const Pose& P, const Consts& K, double slope, 
       }
       const int cell = (ci[2] * K.nc.n[1] + ci[1]) * K.nc.n[0] + ci[0];
       const int end = K.nc.start[cell + 1];
-      for (int k = K.nc.start[cell] + rl; k < end; k += 16) {
-        const double4 a = K.nc.atoms[k];
-        const double rx = p[0] - a.x, ry = p[1] - a.y, rz = p[2] - a.z;
-        const double r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 < VD_CUTOFF_SQR) {
-          const int t2 = (int)a.w;
+      // four candidates per pass: their atom words, then their table quads, are all loaded
+      // before the first is used; each lane still adds its candidates in list order
+      constexpr int U = 4;
+      for (int k0 = K.nc.start[cell] + rl; k0 < end; k0 += 16 * U) {
+        double4 a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) a[u] = K.nc.atoms[k0 + 16 * u < end ? k0 + 16 * u : k0];
+        double rx[U], ry[U], rz[U], rem[U];
+        int idx[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          rx[u] = p[0] - a[u].x; ry[u] = p[1] - a[u].y; rz[u] = p[2] - a[u].z;
+          const double r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
+          ok[u] = k0 + 16 * u < end && r2 < VD_CUTOFF_SQR;
+          const int t2 = (int)a[u].w;
           const int tp = t1 <= t2 ? t1 + t2 * (t2 + 1) / 2 : t2 + t1 * (t1 + 1) / 2;
           const double rf = 32.0 * r2;
-          const int b = (int)rf;
-          const double4 q = K.lut[tp * VD_LUT_BINS + b];
+          const int b = ok[u] ? (int)rf : 0;
+          rem[u] = rf - b;
+          idx[u] = ok[u] ? tp * VD_LUT_BINS + b : 0;
+        }
+        double4 q[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) q[u] = K.lut[idx[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (!ok[u]) continue;
           if constexpr (FAST) {
-            e += (q.z + q.x) / 2;
+            e += (q[u].z + q[u].x) / 2;
           } else {
-            const double rem = rf - b;
-            e += q.x + rem * (q.z - q.x);
-            const double dor = q.y + rem * (q.w - q.y);
-            dx += dor * rx; dy += dor * ry; dz += dor * rz;
+            e += q[u].x + rem[u] * (q[u].z - q[u].x);
+            const double dor = q[u].y + rem[u] * (q[u].w - q[u].y);
+            dx += dor * rx[u]; dy += dor * ry[u]; dz += dor * rz[u];
           }
         }
       }
@@ -247,10 +277,11 @@ __device__ inline double nc_inter(const Pose& P, const Consts& K, double slope, 
 // plain read-modify-write of LDS in chunk order.  Returns this lane's share of the energy.
 template <bool FAST>
 __device__ inline double pair_terms(const Pose& P, const Consts& K, int l) {
-#pragma clang fp contract(off)
   double e_lane = 0;
+  uint32_t w_next = P.C > 0 ? P.pairs[l] : VD_NO_PAIR;
   for (int ch = 0; ch < P.C; ++ch) {
-    const uint32_t w = P.pairs[ch * 64 + l];
+    const uint32_t w = w_next;
+    if (ch + 1 < P.C) w_next = P.pairs[(ch + 1) * 64 + l];
     const bool ok = w != VD_NO_PAIR;
     const int a = ok ? (int)(w & 255u) : 0, b = ok ? (int)((w >> 8) & 255u) : 0;
     const double4 pa = P.xyz[a], pb = P.xyz[b];
@@ -292,7 +323,6 @@ __device__ inline double pair_terms(const Pose& P, const Consts& K, int l) {
 // The adjoint follows the oracle's order: per-node force and torque sums over the node's atoms
 // in index order, then lane 0 folds children into parents in reverse preorder.
 __device__ inline double eval_deriv(const Pose& P, const Consts& K, double slope, const Conf& c, int l, double& g) {
-#pragma clang fp contract(off)
   set_coords(P, c, l);
   double e = nc_inter<false>(P, K, slope, l);
   __syncthreads();
@@ -339,7 +369,6 @@ __device__ inline double eval_deriv(const Pose& P, const Consts& K, double slope
 
 // x (+) factor * d (conf.h:60-66, 118-124; quaternion_increment, quaternion.cpp:235-240).
 __device__ inline Conf increment(const Conf& x, double d, double factor, int l, int T) {
-#pragma clang fp contract(off)
   Conf y = x;
   const double d0 = __shfl(d, 0), d1 = __shfl(d, 1), d2 = __shfl(d, 2);
   const double d3 = __shfl(d, 3), d4 = __shfl(d, 4), d5 = __shfl(d, 5);
@@ -355,7 +384,6 @@ __device__ inline Conf increment(const Conf& x, double d, double factor, int l, 
 
 // Dot product of two lane vectors summed in index order (every lane gets the same value).
 __device__ inline double ordered_dot(const Pose& P, double a, double b, int l) {
-#pragma clang fp contract(off)
   __syncthreads();
   P.vb[l] = a; P.vb[64 + l] = b;
   __syncthreads();
@@ -366,7 +394,6 @@ __device__ inline double ordered_dot(const Pose& P, double a, double b, int l) {
 }
 // -(H v) for this lane's row, summed in index order.
 __device__ inline double minus_hv(const Pose& P, double v, int l) {
-#pragma clang fp contract(off)
   __syncthreads();
   P.vb[128 + l] = v;
   __syncthreads();
@@ -382,7 +409,6 @@ __device__ inline double minus_hv(const Pose& P, double v, int l) {
 // diagonal, the rank-two update skips steps with alpha y.p < epsilon; a result above the start
 // reverts to the start.  No early stop beyond the gradient-norm test.  Returns f; x is updated.
 __device__ inline double bfgs(const Pose& P, const Consts& K, double slope, Conf& x, int max_steps, int l, int& evals) {
-#pragma clang fp contract(off)
   const int n = P.n;
   if (l < n)
     for (int j = 0; j < n; ++j) P.H[l * n + j] = (j == l) ? 1.0 : 0.0;
@@ -454,3 +480,5 @@ __device__ inline bool within(const Pose& P, const Consts& K, int l) {
 }
 
 }  // namespace vdr
+
+#pragma clang fp contract(fast)
