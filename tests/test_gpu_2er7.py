@@ -1,0 +1,93 @@
+"""The largest ligand of the reference's examples on its own scene (V11/input_file_example/
+2er7_*: 89 atoms, 32 torsions, 2,761 intramolecular pairs, a 2,906-atom receptor, a 50 x 26.25
+x 22.5 A box).  It takes the 64-lane group with more atoms than lanes, so the pair forces go
+through the scatter path, and the refinement holds more than one atom per lane.  Checked against
+the FP64 oracle: evaluation on the same maps, a short docking run re-evaluated, and the
+refinement of the docked poses."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+CENTER = [2.865, 30.924, 17.484]   # V11/input_file_example/2er7_config.txt
+SIZE = [50.0, 26.25, 22.5]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def scene(ctx):
+    import vdock
+    ctx.set_receptor_pdbqt(golden("2er7_protein.pdbqt"))
+    box = vdock.box_from_center(CENTER, SIZE, 0.375)
+    say("2er7 populate", box.dims)
+    ctx.grid_populate(box)
+    maps = ctx.grid_download()
+    g_maps = orc.Grid(orc.Lut(), *box.as_tuple())
+    g_maps.set_maps(maps.astype(np.float64))          # the device's maps, evaluated in FP64
+    xyz, xs = orc.parse_receptor(golden("2er7_protein.pdbqt"))[:2]
+    g_nc = orc.Grid(orc.Lut(), *box.as_tuple())        # receptor atoms for the refinement
+    g_nc.set_receptor(xyz, xs)
+    text = golden("2er7_ligand.pdbqt")
+    lig = orc.Ligand(text)
+    lid = ctx.add_ligand(text)
+    inf = ctx.ligand_info(lid)
+    assert (inf.n_atoms, inf.n_torsions, inf.n_pairs) == (89, 32, 2761)
+    return dict(box=box, g_maps=g_maps, g_nc=g_nc, lig=lig, lid=lid)
+
+
+def test_eval_matches_oracle_2er7(ctx, scene):
+    """Energy within max(1e-4, 1e-3 |e|), gradient within 1e-3 max(1, max|g|) + 1e-4 (FP32
+    against FP64 on the same maps), 64 poses within 4 A of the box centre."""
+    lig = scene["lig"]
+    rng = np.random.default_rng(17)
+    conf = np.zeros((64, 7 + lig.T))
+    conf[:, :3] = np.asarray(CENTER) + rng.uniform(-4, 4, (64, 3))
+    q = rng.normal(size=(64, 4))
+    conf[:, 3:7] = q / np.linalg.norm(q, axis=1, keepdims=True)
+    conf[:, 7:] = rng.uniform(-np.pi, np.pi, (64, lig.T))
+    conf = conf.astype(np.float32)
+    say("2er7 eval")
+    e, g = ctx.eval(scene["lid"], conf)
+    worst = 0.0
+    for i in range(len(conf)):
+        e_ref, g_ref, _, _ = orc.eval_deriv(scene["g_maps"], lig, conf[i].astype(np.float64))
+        assert abs(e[i] - e_ref) <= max(1e-4, 1e-3 * abs(e_ref)), (i, e[i], e_ref)
+        tol_g = 1e-3 * max(1.0, np.abs(g_ref).max()) + 1e-4
+        assert np.all(np.abs(g[i] - g_ref) <= tol_g), (i, np.abs(g[i] - g_ref).max(), tol_g)
+        worst = max(worst, abs(e[i] - e_ref) / max(1.0, abs(e_ref)))
+    say("2er7 worst relative energy difference", worst)
+
+
+def test_dock_and_refine_2er7(ctx, scene):
+    """A short batched docking run (256 chains, 3 MC steps): finite results whose energies the
+    oracle reproduces at the returned conformations on the same maps; then the refinement of
+    the 20 best poses against the oracle's refine_structure (FP64 both sides; energies to 1e-4
+    kcal/mol in the median, poses to 1e-3 A in the median, every pose within 0.1 kcal/mol)."""
+    import vdock
+    lig, lid = scene["lig"], scene["lid"]
+    p = vdock.default_params(chains=256, search_depth=3, bfgs_steps=10, seed=3)
+    say("2er7 dock")
+    st = ctx.dock_batch([lid], p)
+    e, conf, heavy = ctx.batch_results(0, p)
+    assert np.isfinite(e).all() and np.isfinite(conf).all() and st.evals > 256
+    for i in np.argsort(e)[:8]:
+        e_ref = orc.eval_deriv(scene["g_maps"], lig, conf[i].astype(np.float64))[0]
+        assert abs(e[i] - e_ref) <= max(1e-3, 1e-3 * abs(e_ref)), (i, e[i], e_ref)
+    best = conf[np.argsort(e)[:20]].astype(np.float32).astype(np.float64)
+    say("2er7 refine", len(best), "best docked", float(e.min()))
+    co, xyz, res = ctx.refine(lid, best)
+    de, dr = [], []
+    steps = orc.refine_steps(lig)
+    for i, c in enumerate(best):
+        eo, _, xo = orc.refine(scene["g_nc"], lig, c, steps)
+        de.append(abs(res[i].e - eo))
+        dr.append(float(np.sqrt(((xyz[i] - xo) ** 2).sum(-1).mean())))
+    de, dr = np.array(de), np.array(dr)
+    say("2er7 refine |de| median", np.median(de), "max", de.max(), "rmsd median", np.median(dr), "max", dr.max())
+    assert np.median(de) <= 1e-4 and np.median(dr) <= 1e-3 and de.max() <= 0.1
