@@ -207,10 +207,16 @@ def run_cpu_baseline(vdock, texts, box, xyz, xs, chains, budget_s=15.0):
 def post_leg(vdock, ctx, log, texts, box, xyz, xs, max_poses=20, cpu_budget_s=10.0):
     """The step after the search for every ligand of the last docking batch: each ligand's chains
     reduced to at most max_poses poses (add_to_output_container), every pose refined against the
-    receptor atoms in one FP64 launch, final scores and mode selection.  Beside it, the oracle's
+    receptor atoms in one FP64 launch, final scores and mode selection; timed on its second call
+    (the first, reported as first_call_wall_ms, also builds the per-box set-up).  Beside it, the oracle's
     FP64 refinement (the reference runs this step on the CPU) of a bounded sample of the same
     poses on host threads."""
     log("post: refinement + final scores of the docked batch")
+    # the first call per receptor and box also builds the receptor cell lists of the direct
+    # evaluation and the pinned staging buffers (once, like the maps); the timed call follows it
+    t0 = time.perf_counter()
+    ctx.postprocess_docked(max_poses)
+    first = time.perf_counter() - t0
     t0 = time.perf_counter()
     st = ctx.postprocess_docked(max_poses)
     wall = time.perf_counter() - t0
@@ -218,6 +224,7 @@ def post_leg(vdock, ctx, log, texts, box, xyz, xs, max_poses=20, cpu_budget_s=10
     modes = [ctx.post_results(k)[0] for k in range(n_lig)]
     best = [m[0].e for m in modes if m]
     out = {"ligands": n_lig, "poses": st.blocks, "kernel_ms": round(st.kernel_ms, 2), "wall_ms": round(1000 * wall, 2),
+           "first_call_wall_ms": round(1000 * first, 2),
            "poses_per_s_kernel": round(st.blocks / (st.kernel_ms / 1000.0), 1),
            "ligands_per_s_wall": round(n_lig / wall, 2), "evals": int(st.evals),
            "evals_per_s_kernel": round(st.evals / (st.kernel_ms / 1000.0), 1),

@@ -50,6 +50,22 @@ struct DevBuf {
   void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
 };
 
+// Page-locked host staging for large device-to-host copies (hipHostMalloc), grown on demand and
+// kept: copies into it run at the link's rate and leave the calling thread free until the wait.
+template <class T>
+struct PinnedBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void ensure(size_t count) {
+    if (count <= n && p) return;
+    if (p) HIP_CHECK(hipHostFree(p));
+    p = nullptr; n = 0;
+    HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault));
+    n = std::max<size_t>(count, 1);
+  }
+  void release() { if (p) (void)hipHostFree(p); p = nullptr; n = 0; }
+};
+
 struct BatchState {   // results of the last batch, kept on the device
   std::vector<int> ids;
   std::vector<longlong4> off;      // per slot: e, conf, heavy offsets
@@ -160,6 +176,9 @@ struct vd_ctx {
   DevBuf<int> ref_lig, ref_evals;
   DevBuf<long long> ref_coff, ref_xoff;
   PostState post;
+  PinnedBuf<float> pin_e, pin_conf, pin_heavy;   // per-chain results of a batch, for pose lists
+  PinnedBuf<double> pin_cout, pin_xout;          // refinement outputs
+  PinnedBuf<double4> pin_res;
 };
 
 namespace {
@@ -798,14 +817,16 @@ RefineOut run_refine(vd_ctx* c, const std::vector<int>& ids, const std::vector<i
   HIP_CHECK(hipEventRecord(c->ev0, c->stream));
   HIP_CHECK(vdl::launch_refine(a, n, c->stream));
   HIP_CHECK(hipEventRecord(c->ev1, c->stream));
-  std::vector<double4> res(n);
   std::vector<int> ev(n);
-  std::vector<double> cout_(nc), xout(nx);
-  HIP_CHECK(hipMemcpyAsync(cout_.data(), c->ref_out.p, nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_CHECK(hipMemcpyAsync(xout.data(), c->ref_xyz.p, nx * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_CHECK(hipMemcpyAsync(res.data(), c->ref_res.p, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
-  HIP_CHECK(hipMemcpyAsync(ev.data(), c->ref_evals.p, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  c->pin_cout.ensure(nc); c->pin_xout.ensure(nx); c->pin_res.ensure(n);
+  const double* cout_ = c->pin_cout.p;
+  const double* xout = c->pin_xout.p;
+  const double4* res = c->pin_res.p;
+  HIP_CHECK(hipMemcpyAsync(c->pin_cout.p, c->ref_out.p, nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->pin_xout.p, c->ref_xyz.p, nx * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->pin_res.p, c->ref_res.p, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
+  HIP_CHECK(hipMemcpy(ev.data(), c->ref_evals.p, n * sizeof(int), hipMemcpyDeviceToHost));
   r.ms = elapsed_ms(c);
   for (long long i = 0; i < nc; ++i) r.conf[i] = (float)cout_[i];
   for (long long i = 0; i < nx; ++i) r.coords[i] = (float)xout[i];
@@ -917,6 +938,8 @@ void vd_destroy(vd_ctx* c) {
   c->nc_atoms.release(); c->nc_start.release(); c->ref_in.release(); c->ref_out.release(); c->ref_xyz.release();
   c->ref_res.release(); c->ref_lig.release(); c->ref_evals.release(); c->ref_coff.release(); c->ref_xoff.release();
   c->lutd.release(); c->ref_geo.release(); c->ref_pairs.release(); c->ref_hdr.release();
+  c->pin_e.release(); c->pin_conf.release(); c->pin_heavy.release();
+  c->pin_cout.release(); c->pin_xout.release(); c->pin_res.release();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1425,21 +1448,25 @@ int vd_postprocess_docked(vd_ctx* c, int max_poses, const vd_post_params* p, vd_
     need_device(c);
     BatchState& b = c->batch;
     const int nl = (int)b.ids.size();
-    std::vector<float> e(b.total.x), cf(b.total.y), hv(b.total.z);
-    if (b.total.x) HIP_CHECK(hipMemcpy(e.data(), b.e.p, e.size() * 4, hipMemcpyDeviceToHost));
-    if (b.total.y) HIP_CHECK(hipMemcpy(cf.data(), b.conf.p, cf.size() * 4, hipMemcpyDeviceToHost));
-    if (b.total.z) HIP_CHECK(hipMemcpy(hv.data(), b.heavy.p, hv.size() * 4, hipMemcpyDeviceToHost));
+    c->pin_e.ensure(b.total.x); c->pin_conf.ensure(b.total.y); c->pin_heavy.ensure(b.total.z);
+    const float* e = c->pin_e.p;
+    const float* cf = c->pin_conf.p;
+    const float* hv = c->pin_heavy.p;
+    if (b.total.x) HIP_CHECK(hipMemcpyAsync(c->pin_e.p, b.e.p, b.total.x * 4, hipMemcpyDeviceToHost, c->stream));
+    if (b.total.y) HIP_CHECK(hipMemcpyAsync(c->pin_conf.p, b.conf.p, b.total.y * 4, hipMemcpyDeviceToHost, c->stream));
+    if (b.total.z) HIP_CHECK(hipMemcpyAsync(c->pin_heavy.p, b.heavy.p, b.total.z * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
     std::vector<std::vector<int>> kept(nl);
     parallel_for(nl, [&](int k) {
       const longlong4 o = b.off[k];
-      kept[k] = vd::cluster_poses(b.chains[k], b.H[k], e.data() + o.x, hv.data() + o.z, 1.0, max_poses);
+      kept[k] = vd::cluster_poses(b.chains[k], b.H[k], e + o.x, hv + o.z, 1.0, max_poses);
     });
     std::vector<int> ids(b.ids), poses(nl);
     std::vector<float> conf;
     for (int k = 0; k < nl; ++k) {
       const int T = b.T[k];
       poses[k] = (int)kept[k].size();
-      const float* base = cf.data() + b.off[k].y;
+      const float* base = cf + b.off[k].y;
       for (int i : kept[k]) conf.insert(conf.end(), base + (size_t)i * (7 + T), base + (size_t)(i + 1) * (7 + T));
     }
     postprocess(c, ids, poses, conf.data(), p, st);
