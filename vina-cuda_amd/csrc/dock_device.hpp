@@ -293,13 +293,20 @@ __device__ float gyration_radius(const LigView& L, const Scratch& S, int l) {
   return cnt > 0.0f ? sqrtf(acc / cnt) : 0.0f;
 }
 
+// Curl constants of the pair terms: the cap factor is vs / (vd + e), which is v / (v + e), or 0
+// when v < epsilon (V11/lib/curl.h:4-13); computed once per evaluation.
+struct CurlK {
+  float vs, vd;
+  __device__ explicit CurlK(float v) : vs(v < kEpsFl ? 0.0f : v), vd(v < kEpsFl ? 1.0f : v) {}
+};
+
 // One pair term: value and derivative/r from the pair table quad, with the curl cap
 // (V11/lib/curl.h:4-13) above zero.  The cap's quotient uses the hardware reciprocal (about one
 // ulp, far inside the parity tolerance).
-__device__ __forceinline__ void pair_term(const float4 q, float rem, float v_intra, float& e, float& dor) {
+__device__ __forceinline__ void pair_term(const float4 q, float rem, const CurlK& ck, float& e, float& dor) {
   e = q.x + rem * (q.z - q.x);
   dor = q.y + rem * (q.w - q.y);
-  const float t = (v_intra < kEpsFl) ? 0.0f : v_intra * __builtin_amdgcn_rcpf(v_intra + e);
+  const float t = ck.vs * __builtin_amdgcn_rcpf(ck.vd + e);
   const bool cap = e > 0.0f;
   e = cap ? e * t : e;
   dor = cap ? dor * (t * t) : dor;
@@ -309,16 +316,19 @@ __device__ __forceinline__ void pair_term(const float4 q, float rem, float v_int
 // Lane l keeps the force on atom l in registers: its own pairs give the first-atom side, and the
 // second-atom side comes from the one slot per chunk whose pair has second atom l, through a
 // per-group LDS exchange row (every lane writes its pair force to its entry, then reads the
-// source slot's entry; entry G stays zero for lanes without a source).  Empty slots read the
-// all-zero table quad and contribute nothing, so no lane needs a select.  Four chunks per pass
-// (one 16-byte read of the lane's pair words, prefetched a pass ahead), all four table reads
-// issued before the first is used.
+// source slot's entry; entry G stays zero for lanes without a source).  The words need no
+// decoding: a slot without a pair names a written coordinate row as its partner and the all-zero
+// row of the pair table as its type pair, so it reads zero quads and contributes exact zeros,
+// and a lane without a source names entry G.  Four chunks per pass (one 16-byte read of the
+// lane's pair words, prefetched a pass ahead); the four partner reads, then the four table reads,
+// are issued before the first is used.
 template <int G>
 __device__ float pair_terms_rows(const LigView& L, const Scratch& S, const EvalK& K, int l) {
   constexpr int U = 4;
   const int groups = (L.C + U - 1) / U;
   const uint4* pw = L.pairs + l;
   const float4 own = l < L.A ? S.xyz[l] : make_float4(0, 0, 0, 0);
+  const CurlK ck(K.v_intra);
   if (l == 0) S.fx[G] = make_float4(0, 0, 0, 0);
   float e_intra = 0.0f, ax = 0.0f, ay = 0.0f, az = 0.0f;
   uint4 w_next = groups > 0 ? pw[0] : make_uint4(0, 0, 0, 0);
@@ -326,23 +336,18 @@ __device__ float pair_terms_rows(const LigView& L, const Scratch& S, const EvalK
     const uint4 w4 = w_next;
     if (cg + 1 < groups) w_next = pw[(cg + 1) * G];
     const uint32_t w[U] = {w4.x, w4.y, w4.z, w4.w};
-    int src[U], idx[U];
+    float4 pb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) pb[u] = S.xyz[(w[u] >> 8) & 255u];
     float rx[U], ry[U], rz[U], rem[U];
+    uint32_t idx[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint32_t bb = (w[u] >> 8) & 255u, ss = w[u] & 255u;
-      const bool has = w[u] != VD_NO_PAIR && bb != 255u;
-      src[u] = (w[u] != VD_NO_PAIR && ss != 255u) ? (int)ss : G;
-      // a slot without a pair reads a written coordinate (its own atom, or atom 0 for lanes
-      // beyond the ligand) so that the unused distance stays finite
-      const float4 pb = S.xyz[has ? (int)bb : (l < L.A ? l : 0)];
-      rx[u] = pb.x - own.x; ry[u] = pb.y - own.y; rz[u] = pb.z - own.z;
+      rx[u] = pb[u].x - own.x; ry[u] = pb[u].y - own.y; rz[u] = pb[u].z - own.z;
       const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
-      const bool ok = has && r2 < VD_CUTOFF_SQR;
       const float rf = VD_LUT_FACTOR * r2;
-      const int i1 = ok ? (int)rf : 0;
-      rem[u] = rf - (float)i1;
-      idx[u] = ok ? (int)(w[u] >> 16) * VD_LUT_BINS + i1 : VD_ZERO_QUAD;
+      rem[u] = __builtin_amdgcn_fractf(rf);   // rf - trunc(rf), exact for rf >= 0
+      idx[u] = r2 < VD_CUTOFF_SQR ? (w[u] >> 16) * (uint32_t)VD_LUT_BINS + (uint32_t)rf : (uint32_t)VD_ZERO_QUAD;
     }
     float4 q[U];
 #pragma unroll
@@ -350,12 +355,12 @@ __device__ float pair_terms_rows(const LigView& L, const Scratch& S, const EvalK
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       float e, dor;
-      pair_term(q[u], rem[u], K.v_intra, e, dor);
+      pair_term(q[u], rem[u], ck, e, dor);
       e_intra += e;
       const float fx = dor * rx[u], fy = dor * ry[u], fz = dor * rz[u];
       ax -= fx; ay -= fy; az -= fz;
       S.fx[l] = make_float4(fx, fy, fz, 0.0f);
-      const float4 in = S.fx[src[u]];
+      const float4 in = S.fx[w[u] & 255u];
       ax += in.x; ay += in.y; az += in.z;
     }
   }
@@ -376,6 +381,7 @@ __device__ float pair_terms_scatter(const LigView& L, const Scratch& S, const Ev
   constexpr int U = 4;
   const int groups = (L.C + U - 1) / U;
   const uint4* pw = L.pairs + l;
+  const CurlK ck(K.v_intra);
   float e_intra = 0.0f;
   uint4 w_next = groups > 0 ? pw[0] : make_uint4(0, 0, 0, 0);
   for (int cg = 0; cg < groups; ++cg) {
@@ -383,7 +389,8 @@ __device__ float pair_terms_scatter(const LigView& L, const Scratch& S, const Ev
     if (cg + 1 < groups) w_next = pw[(cg + 1) * G];
     const uint32_t w[U] = {w4.x, w4.y, w4.z, w4.w};
     bool ok[U];
-    int a[U], b[U], idx[U];
+    int a[U], b[U];
+    uint32_t idx[U];
     float rx[U], ry[U], rz[U], rem[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -401,9 +408,8 @@ __device__ float pair_terms_scatter(const LigView& L, const Scratch& S, const Ev
       const float r2 = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
       ok[u] = ok[u] && r2 < VD_CUTOFF_SQR;
       const float rf = VD_LUT_FACTOR * r2;
-      const int i1 = ok[u] ? (int)rf : 0;
-      rem[u] = rf - (float)i1;
-      idx[u] = ok[u] ? (int)(w[u] >> 16) * VD_LUT_BINS + i1 : VD_ZERO_QUAD;
+      rem[u] = __builtin_amdgcn_fractf(rf);   // rf - trunc(rf), exact for rf >= 0
+      idx[u] = ok[u] ? (w[u] >> 16) * (uint32_t)VD_LUT_BINS + (uint32_t)rf : (uint32_t)VD_ZERO_QUAD;
     }
     float4 q[U];
 #pragma unroll
@@ -411,7 +417,7 @@ __device__ float pair_terms_scatter(const LigView& L, const Scratch& S, const Ev
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       float e, dor;
-      pair_term(q[u], rem[u], K.v_intra, e, dor);
+      pair_term(q[u], rem[u], ck, e, dor);
       if (!ok[u]) continue;
       e_intra += e;
       const float fx = dor * rx[u], fy = dor * ry[u], fz = dor * rz[u];

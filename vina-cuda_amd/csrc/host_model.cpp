@@ -148,8 +148,9 @@ void Precalculated::grid_rows(std::vector<float>& out) const {
 }
 
 void Precalculated::pair_quads(std::vector<float>& out) const {
-  // one extra all-zero quad at VD_ZERO_QUAD: empty pair slots read it and contribute nothing
-  out.assign(((size_t)VD_NUM_TYPE_PAIRS * VD_LUT_BINS + 1) * 4, 0.0f);
+  // one extra all-zero row after the 153 type pairs, whose first quad is VD_ZERO_QUAD: empty pair
+  // slots and pairs beyond the cutoff read it and contribute nothing
+  out.assign(((size_t)VD_NUM_TYPE_PAIRS + 1) * VD_LUT_BINS * 4, 0.0f);
   for (int tp = 0; tp < VD_NUM_TYPE_PAIRS; ++tp)
     for (int b = 0; b < VD_LUT_BINS; ++b) {
       const size_t s = (size_t)tp * VD_LUT_BINS;
@@ -634,20 +635,25 @@ vd_lig_header pack_ligand(const Ligand& L, std::vector<float>& blob, int G) {
   h.n_chunks = (int)(pairs.size() / G);
   h.pair_rows = (int)L.atoms.size() <= G ? 1 : 0;
   // rows: slot l holds pair (l, b); its word carries, in place of the first atom (= l), the slot
-  // whose pair has second atom l in the same chunk (255: none), from which lane l pulls that
-  // pair's force on atom l
+  // whose pair has second atom l in the same chunk (G: none), from which lane l pulls that
+  // pair's force on atom l.  A slot without a pair names a written coordinate row as partner
+  // (atom l, or atom 0 for lanes beyond the ligand) and the all-zero row of the pair table
+  // (VD_NUM_TYPE_PAIRS) as type pair, so the device reads every word without decoding it.
+  const int A = (int)L.atoms.size();
+  auto empty_row = [&](int l, uint32_t src) {
+    return src | ((uint32_t)(l < A ? l : 0) << 8) | ((uint32_t)VD_NUM_TYPE_PAIRS << 16);
+  };
   std::vector<uint32_t> words(pairs);
   if (h.pair_rows) {
     for (int ch = 0; ch < h.n_chunks; ++ch) {
-      std::vector<uint32_t> inv(G, 255u);
+      std::vector<uint32_t> inv(G, (uint32_t)G);
       for (int l = 0; l < G; ++l) {
         const uint32_t w = pairs[(size_t)ch * G + l];
         if (w != VD_NO_PAIR) inv[(w >> 8) & 255] = (uint32_t)l;
       }
       for (int l = 0; l < G; ++l) {
         const uint32_t w = pairs[(size_t)ch * G + l];
-        const uint32_t b = w != VD_NO_PAIR ? (w >> 8) & 255 : 255u, tp = w != VD_NO_PAIR ? w >> 16 : 0u;
-        words[(size_t)ch * G + l] = (w == VD_NO_PAIR && inv[l] == 255u) ? VD_NO_PAIR : (inv[l] | (b << 8) | (tp << 16));
+        words[(size_t)ch * G + l] = w == VD_NO_PAIR ? empty_row(l, inv[l]) : (inv[l] | (w & 0xFFFFFF00u));
       }
     }
   }
@@ -657,7 +663,8 @@ vd_lig_header pack_ligand(const Ligand& L, std::vector<float>& blob, int G) {
     for (int l = 0; l < G; ++l)
       for (int u = 0; u < 4; ++u) {
         const int ch = cg * 4 + u;
-        blob.push_back(as_f(ch < h.n_chunks ? words[(size_t)ch * G + l] : VD_NO_PAIR));
+        const uint32_t pad = h.pair_rows ? empty_row(l, (uint32_t)G) : VD_NO_PAIR;
+        blob.push_back(as_f(ch < h.n_chunks ? words[(size_t)ch * G + l] : pad));
       }
   pad16();
   return h;

@@ -10,7 +10,7 @@
 #define VD_LUT_BINS 2051             // int(32 * 64) + 3
 #define VD_LUT_FACTOR 32.0f
 #define VD_CUTOFF_SQR 64.0f
-#define VD_ZERO_QUAD (VD_NUM_TYPE_PAIRS * VD_LUT_BINS)   // index of the all-zero quad after the pair table
+#define VD_ZERO_QUAD (VD_NUM_TYPE_PAIRS * VD_LUT_BINS)   // first quad of the all-zero row after the pair table
 #define VD_GRID_ROW 20               // per (t1, bin) row of the map LUT: 17 t2 values padded to 20
 #define VD_MAX_ATOMS 128             // per ligand (the reference cap MAX_NUM_OF_ATOMS, V11/inc/kernel2.h:35)
 #define VD_MAX_NODES 64              // rigid nodes (torsions + 1)
@@ -26,9 +26,10 @@
 //             and the `b` of at most one pair, so the force scatter of a chunk needs no atomics.
 //             Unused slots hold VD_NO_PAIR.  When the ligand has at most chunk_lanes atoms
 //             (pair_rows = 1), the pair in slot l has first atom l and its word is
-//             src | b << 8 | tp << 16 instead: b = 255 for no pair, src = the slot of the same
-//             chunk whose pair has second atom l (255: none).  The area is laid out in groups of
-//             four chunks, [group][lane][4].
+//             src | b << 8 | tp << 16 instead: src = the slot of the same chunk whose pair has
+//             second atom l (chunk_lanes: none); a slot without a pair has b = l (0 for lanes
+//             beyond the ligand) and tp = VD_NUM_TYPE_PAIRS, the all-zero row of the pair table.
+//             The area is laid out in groups of four chunks, [group][lane][4].
 #define VD_NO_PAIR 0xFFFFFFFFu
 struct vd_lig_header {
   int n_atoms, n_nodes, n_pairs, n_heavy;
