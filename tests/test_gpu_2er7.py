@@ -84,10 +84,30 @@ def test_dock_and_refine_2er7(ctx, scene):
     co, xyz, res = ctx.refine(lid, best)
     de, dr = [], []
     steps = orc.refine_steps(lig)
+    lo, hi, _ = (np.asarray(v, dtype=np.float64) for v in scene["box"].as_tuple())
+    heavy = lig.atoms()["xs"] < orc.N_XS
+    flips = both_out = 0
     for i, c in enumerate(best):
         eo, _, xo = orc.refine(scene["g_nc"], lig, c, steps)
+        dev_out, orc_out = res[i].e >= 3e38, eo >= 1e300   # max_fl: FLT_MAX on the ABI, DBL_MAX in the oracle
+        if dev_out and orc_out:
+            both_out += 1   # both report that the pose did not end inside the box
+            assert not res[i].inside
+            continue
+        if dev_out != orc_out:
+            # one side ended inside the box and the other not: refine_structure's within() test
+            # (heavy atoms inside the box grown by 1e-4 A) decided on a pose that sits on that
+            # margin, where the last bits of the two FP64 runs (device and host sine/cosine)
+            # decide.  Accepted only for such a pose, and rarely.
+            out = np.maximum(lo - 1e-4 - xyz[i][heavy], xyz[i][heavy] - (hi + 1e-4)).max()
+            say("2er7 refine: inside/outside differs on pose", i, "distance to the margin", out)
+            assert abs(out) < 1e-3
+            flips += 1
+            continue
         de.append(abs(res[i].e - eo))
         dr.append(float(np.sqrt(((xyz[i] - xo) ** 2).sum(-1).mean())))
     de, dr = np.array(de), np.array(dr)
-    say("2er7 refine |de| median", np.median(de), "max", de.max(), "rmsd median", np.median(dr), "max", dr.max())
+    say("2er7 refine |de| median", np.median(de), "max", de.max(), "rmsd median", np.median(dr), "max", dr.max(),
+        "outside on both sides", both_out, "inside/outside differs", flips)
+    assert flips <= 2 and len(de) >= 10
     assert np.median(de) <= 1e-4 and np.median(dr) <= 1e-3 and de.max() <= 0.1
