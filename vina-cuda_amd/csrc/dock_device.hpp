@@ -177,6 +177,12 @@ struct Conf {                  // register view of one conformation
 
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
 
+// A product rounded on its own, never fused with a following add into an FMA.
+__device__ __forceinline__ float mul_rn(float a, float b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+
 // Trilinear map value and derivative at p (grid::evaluate_aux, V11/lib/grid.cpp:50-167).
 // prel = position relative to the map origin.  NaN/inf inputs stay in bounds.
 __device__ __forceinline__ float map_eval(const float* __restrict__ m, const vd_grid_desc& gd, float3 prel,
@@ -205,9 +211,10 @@ __device__ __forceinline__ float map_eval(const float* __restrict__ m, const vd_
   float gx = (f100 - f000) * my * mz + (f110 - f010) * y * mz + (f101 - f001) * my * z + (f111 - f011) * y * z;
   float gy = (f010 - f000) * mx * mz + (f110 - f100) * x * mz + (f011 - f001) * mx * z + (f111 - f101) * x * z;
   float gz = (f001 - f000) * mx * my + (f101 - f100) * x * my + (f011 - f010) * mx * y + (f111 - f110) * x * y;
-  if (f > 0.0f) {  // curl (V11/lib/curl.h:31-38); hardware reciprocal for the quotient
-    const float t = (v < kEpsFl) ? 0.0f : v * __builtin_amdgcn_rcpf(v + f);
-    f *= t; const float t2 = t * t; gx *= t2; gy *= t2; gz *= t2;
+  {  // curl (V11/lib/curl.h:31-38); hardware reciprocal for the quotient; below zero the factor is
+     // 1, which leaves f and its gradient exactly as they are
+    const float t = f > 0.0f ? ((v < kEpsFl) ? 0.0f : v * __builtin_amdgcn_rcpf(v + f)) : 1.0f;
+    f = mul_rn(f, t); const float t2 = t * t; gx *= t2; gy *= t2; gz *= t2;
   }
   deriv.x = gd.factor[0] * (region[0] == 0 ? gx : 0.0f) + slope * region[0];
   deriv.y = gd.factor[1] * (region[1] == 0 ? gy : 0.0f) + slope * region[1];
@@ -306,10 +313,10 @@ struct CurlK {
 __device__ __forceinline__ void pair_term(const float4 q, float rem, const CurlK& ck, float& e, float& dor) {
   e = q.x + rem * (q.z - q.x);
   dor = q.y + rem * (q.w - q.y);
-  const float t = ck.vs * __builtin_amdgcn_rcpf(ck.vd + e);
-  const bool cap = e > 0.0f;
-  e = cap ? e * t : e;
-  dor = cap ? dor * (t * t) : dor;
+  // the factor, not the results, is selected: multiplying by 1 leaves e and dor exactly as they are
+  const float t = e > 0.0f ? ck.vs * __builtin_amdgcn_rcpf(ck.vd + e) : 1.0f;
+  e = mul_rn(e, t);
+  dor *= t * t;
 }
 
 // Pair terms when every atom has its own lane (L.rows): slot l of each chunk holds pair (l, b).
