@@ -53,8 +53,12 @@ __global__ __launch_bounds__(256) void vd_grid_kernel(const float4* __restrict__
     for (int cy = cy0; cy <= cy1; ++cy) {
       const int row = (cz * gc.n[1] + cy) * gc.n[0];
       const int begin = cells[row + cx0].x, end = cells[row + cx1].y;   // cells of one x-row are contiguous
+      // the next atom's load is issued before this one is tested (the index is uniform, so these
+      // are scalar loads, and the wait for each would otherwise stall the whole workgroup)
+      float4 a_next = begin < end ? atoms[begin] : make_float4(0, 0, 0, 0);
       for (int i = begin; i < end; ++i) {
-        const float4 a = atoms[i];
+        const float4 a = a_next;
+        if (i + 1 < end) a_next = atoms[i + 1];
         // skip atoms farther than the cutoff from the whole tile box (wave-uniform test)
         const float ddx = fmaxf(fmaxf(lo_x - a.x, a.x - hi_x), 0.0f), ddy = fmaxf(fmaxf(lo_y - a.y, a.y - hi_y), 0.0f),
                     ddz = fmaxf(fmaxf(lo_z - a.z, a.z - hi_z), 0.0f);
