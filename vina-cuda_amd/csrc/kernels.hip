@@ -25,6 +25,7 @@ namespace {
 // box grown by the 8 A cutoff).  For every atom within the cutoff of a point, one 20-float row
 // of the map LUT gives the contribution to all 17 maps (rows indexed [t1][bin]).
 constexpr int TX = 16, TY = 4, TZ = 4;
+constexpr int GU = 2;   // receptor atoms per pass of the grid kernel's loop (4 measured slower)
 
 __global__ __launch_bounds__(256) void vd_grid_kernel(const float4* __restrict__ atoms, const int2* __restrict__ cells,
                                                       GridCells gc, const float4* __restrict__ lut_rows,
@@ -53,28 +54,35 @@ __global__ __launch_bounds__(256) void vd_grid_kernel(const float4* __restrict__
     for (int cy = cy0; cy <= cy1; ++cy) {
       const int row = (cz * gc.n[1] + cy) * gc.n[0];
       const int begin = cells[row + cx0].x, end = cells[row + cx1].y;   // cells of one x-row are contiguous
-      // the next atom's load is issued before this one is tested (the index is uniform, so these
-      // are scalar loads, and the wait for each would otherwise stall the whole workgroup)
-      float4 a_next = begin < end ? atoms[begin] : make_float4(0, 0, 0, 0);
-      for (int i = begin; i < end; ++i) {
-        const float4 a = a_next;
-        if (i + 1 < end) a_next = atoms[i + 1];
-        // skip atoms farther than the cutoff from the whole tile box (wave-uniform test)
-        const float ddx = fmaxf(fmaxf(lo_x - a.x, a.x - hi_x), 0.0f), ddy = fmaxf(fmaxf(lo_y - a.y, a.y - hi_y), 0.0f),
-                    ddz = fmaxf(fmaxf(lo_z - a.z, a.z - hi_z), 0.0f);
-        if (ddx * ddx + ddy * ddy + ddz * ddz > VD_CUTOFF_SQR) continue;
-        const float dx = a.x - px, dy = a.y - py, dz = a.z - pz;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (inside && r2 <= VD_CUTOFF_SQR) {
+      // GU atoms per pass: their table rows are loaded before any is added (each lane still adds
+      // its atoms in list order, so the sums are those of a one-by-one loop)
+      for (int i = begin; i < end; i += GU) {
+        float4 rv[GU][5];
+        bool use[GU];
+#pragma unroll
+        for (int u = 0; u < GU; ++u) {
+          const int k = i + u < end ? i + u : i;
+          const float4 a = atoms[k];
+          // skip atoms farther than the cutoff from the whole tile box (uniform test)
+          const float ddx = fmaxf(fmaxf(lo_x - a.x, a.x - hi_x), 0.0f), ddy = fmaxf(fmaxf(lo_y - a.y, a.y - hi_y), 0.0f),
+                      ddz = fmaxf(fmaxf(lo_z - a.z, a.z - hi_z), 0.0f);
+          const float dx = a.x - px, dy = a.y - py, dz = a.z - pz;
+          const float r2 = dx * dx + dy * dy + dz * dz;
+          use[u] = i + u < end && ddx * ddx + ddy * ddy + ddz * ddz <= VD_CUTOFF_SQR && inside && r2 <= VD_CUTOFF_SQR;
           const int t1 = (int)__float_as_uint(a.w);
-          const int bin = (int)(VD_LUT_FACTOR * r2);
-          const float4* rw = lut_rows + ((size_t)t1 * VD_LUT_BINS + bin) * (VD_GRID_ROW / 4);
-          const float4 v0 = rw[0], v1 = rw[1], v2 = rw[2], v3 = rw[3], v4 = rw[4];
-          acc[0] += v0.x; acc[1] += v0.y; acc[2] += v0.z; acc[3] += v0.w;
-          acc[4] += v1.x; acc[5] += v1.y; acc[6] += v1.z; acc[7] += v1.w;
-          acc[8] += v2.x; acc[9] += v2.y; acc[10] += v2.z; acc[11] += v2.w;
-          acc[12] += v3.x; acc[13] += v3.y; acc[14] += v3.z; acc[15] += v3.w;
-          acc[16] += v4.x;
+          const int bin = use[u] ? (int)(VD_LUT_FACTOR * r2) : 0;
+          const float4* rw = lut_rows + ((size_t)(use[u] ? t1 : 0) * VD_LUT_BINS + bin) * (VD_GRID_ROW / 4);
+#pragma unroll
+          for (int c = 0; c < 5; ++c) rv[u][c] = use[u] ? rw[c] : make_float4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < GU; ++u) {
+          if (!use[u]) continue;
+          acc[0] += rv[u][0].x; acc[1] += rv[u][0].y; acc[2] += rv[u][0].z; acc[3] += rv[u][0].w;
+          acc[4] += rv[u][1].x; acc[5] += rv[u][1].y; acc[6] += rv[u][1].z; acc[7] += rv[u][1].w;
+          acc[8] += rv[u][2].x; acc[9] += rv[u][2].y; acc[10] += rv[u][2].z; acc[11] += rv[u][2].w;
+          acc[12] += rv[u][3].x; acc[13] += rv[u][3].y; acc[14] += rv[u][3].z; acc[15] += rv[u][3].w;
+          acc[16] += rv[u][4].x;
         }
       }
     }
