@@ -376,6 +376,8 @@ double eval_bytes(const vd_lig_header& h) { return 32.0 * h.n_heavy + 16.0 * h.n
 void mc_prepare(vd_ctx* c, McSlot& s, int G, int W, Packed&& pk, std::vector<uint32_t>&& streams,
                 std::vector<int2>&& items, std::vector<longlong4>&& off, const vd_mc_consts& P, float* d_e,
                 float* d_conf, float* d_heavy, bool persistent, bool timed, double share = 1.0) {
+  // a launch left in flight by a call that failed part-way still reads this slot's host copies
+  if (s.busy) { (void)hipStreamSynchronize(s.stream); s.busy = false; }
   s.pk = std::move(pk); s.h_streams = std::move(streams); s.h_items = std::move(items);
   s.h_headers = s.pk.headers; s.h_offs = std::move(off);
   s.h_goff.clear(); s.h_gaxis.clear(); s.h_gT.clear();
