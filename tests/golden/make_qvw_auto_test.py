@@ -7,12 +7,18 @@ checkout that has the reference beside it at /root/reference):
   qvw_auto_test.npz         for each of the 31 docked outputs (<dir>/<ligand>_out.pdbqt): the
                             atom serials, the coordinates of the nine modes in milli-Angstrom
                             (exactly as printed, %8.3f) and the REMARK VINA RESULT triples
+  qvw_auto_confs.npz        the conformations (position, orientation, torsions; rounded to FP32)
+                            that put each ligand's atoms on those modes, fitted with
+                            tests/test_oracle_rescore.py's conf_from_coords (Kabsch fit of the
+                            root, then a scan per torsion), for the device post-processing test
 
 The receptors (auto_test/{1u1b,2bm2,2er7}_protein.pdbqt) are byte-identical to the golden
 1u1b/2bm2/2er7_protein.pdbqt.  Nothing here runs reference code: the files are read as text.
+`--confs` writes only qvw_auto_confs.npz, from the committed fixtures.
 """
 import os
 import shutil
+import sys
 
 import numpy as np
 
@@ -66,5 +72,24 @@ def main():
     print(len(recs), "outputs")
 
 
+def fit_confs():
+    sys.path[:0] = [os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")]
+    import oracle as orc
+    from test_oracle_rescore import conf_from_coords
+    from test_oracle_rescore_qvw_auto import CASES, ligand_text, modes
+    out, worst = {}, 0.0
+    for _, name, k in CASES:
+        lig = orc.Ligand(ligand_text(name))
+        X = modes(k, lig)
+        c = np.array([conf_from_coords(lig, x) for x in X]).astype(np.float32)
+        worst = max(worst, max(np.abs(lig.set_conf(ci.astype(np.float64)) - x).max() for ci, x in zip(c, X)))
+        out[f"conf_{k}"] = c
+    np.savez_compressed(os.path.join(GOLDEN, "qvw_auto_confs.npz"), **out)
+    print(len(out), "outputs fitted, worst atom distance", worst)
+
+
 if __name__ == "__main__":
-    main()
+    if "--confs" in sys.argv:
+        fit_confs()
+    else:
+        main()
