@@ -266,6 +266,8 @@ def main():
     ap.add_argument("--ligands", type=int, default=1024)
     ap.add_argument("--chains", type=int, default=256)
     ap.add_argument("--plain-subset", type=int, default=0, help="0 = all ligands")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time each size bucket's launch shape (vd_set_autotune) instead of the fixed table")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-large-box", action="store_true", help="skip the large-box QuickVina2 leg")
@@ -282,6 +284,7 @@ def main():
 
     import vdock
     ctx = vdock.Context(0)
+    ctx.set_autotune(args.autotune)
     if args.grid_only:
         xyz, xs, box, _ = build_scene(vdock, ctx)
         ms = [ctx.grid_populate(box) for _ in range(5)]
@@ -335,6 +338,7 @@ def main():
                         "(81^3 x 17 FP32 maps), %d ligands (10-60 heavy atoms, 0..min(16,atoms/3) torsions), "
                         "%d MC chains each, RILC-BFGS, persistent work-queue launch" % (args.ligands, args.chains),
             "ligands": args.ligands, "chains": args.chains, "maps": "81x81x81x17 fp32",
+            "launch_shapes": "timed per size bucket" if args.autotune else "fixed per size bucket",
         },
         "kernel_ms_per_step": round(kernel_ms / args.steps, 2),
         "evals_per_s": round(evals / (kernel_ms / 1000.0), 1),

@@ -125,9 +125,11 @@ typedef struct {
   int local_search;      /* VD_LS_RILC_BFGS (reference --rilc_bfgs 1) or VD_LS_BFGS */
   int policy;            /* VD_MC_VINA, VD_MC_QVINA2 or VD_MC_QVINAW (the last two with RILC-BFGS) */
   int mutation_quirk;    /* 1: mutate one of 1+T entities like the reference GPU host tables */
-  int chains_per_wave;   /* 1, 2, 4; 0 = autotuned per ligand size bucket */
-  int waves_per_simd;    /* register budget of the MC kernel: 2, 3 or 4 resident waves per SIMD; 0 = autotuned */
-  uint32_t seed;
+  int chains_per_wave;   /* 1, 2, 4; 0 = per ligand size bucket (vd_set_autotune) */
+  int waves_per_simd;    /* register budget of the MC kernel: 2, 3 or 4 resident waves per SIMD; 0 = per size bucket */
+  uint32_t seed;         /* chain c of ligand id draws from the Philox stream keyed (seed, id) at
+                            counter c: ids count from 0 in the order of vd_add_ligand, as the
+                            reference's ligands count in the order of its batch */
   float temperature;     /* 1.2 */
   float amplitude;       /* mutation amplitude 2 */
   float v_intra, v_inter;/* curl caps during search: hunt_cap (10, 10) */
@@ -171,6 +173,14 @@ int vd_last_launches(vd_ctx* ctx, int max, vd_launch* out);
  * or more, run one class after the other.  on = 0: always one after the other.  No counterpart
  * in the reference (one kernel_monte launch per ligand). */
 int vd_set_class_overlap(vd_ctx* ctx, int on);
+/* on = 0 (default): with chains_per_wave or waves_per_simd left at 0, each ligand's
+ * (lanes per chain, waves per SIMD) pair comes from a fixed table per size bucket (atoms x
+ * torsions), so that the same inputs and seed give the same results in every process.  on = 1:
+ * the pair is timed per bucket on first use (trial launches of a representative ligand) and the
+ * fastest kept for the context; timings that flip between processes then change a run's FP32
+ * rounding and with it its trajectories.  Turning it off forgets the timed pairs.  No counterpart
+ * in the reference (one fixed launch shape). */
+int vd_set_autotune(vd_ctx* ctx, int on);
 
 /* Plain per-ligand launch (replaces kernel_monte, V11/inc/cuda/kernel2.cuh:65-79).
  * Per chain results: e[chains], conf[chains][7+T], heavy[chains][n_heavy][3] (host pointers,

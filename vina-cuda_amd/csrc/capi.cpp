@@ -160,6 +160,7 @@ struct vd_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int n_cu = 0;
   std::map<int, std::pair<int, int>> tuned;  // size bucket -> (lanes per chain, waves per SIMD)
+  bool autotune_on = false;                   // timing-based choice of the pair above (vd_set_autotune)
   DevBuf<float> tune_e, tune_conf, tune_heavy;
   std::vector<vd_launch> launches;           // MC launches of the last docking call
   // after the search: receptor cells of the direct evaluation (built on first use per box)
@@ -305,6 +306,24 @@ int size_bucket(const vd::Ligand& L) {
   const int tb = t + 7 <= 16 ? 0 : t + 7 <= 32 ? 1 : 2;
   return ab * 4 + tb;
 }
+// (lanes per chain, waves per SIMD) per size bucket when timing-based tuning is off: what the
+// autotuner picks on MI355X (gfx950, 256 CUs) in batched screening (1,024 ligands of 10-60 heavy
+// atoms, 256 chains each): 16 lanes for ligands under 80 atoms with at most 9 torsions, 64 lanes
+// for the rest, 3 waves per SIMD.  Two widths keep any batch at two classes, which run at the
+// same time.  A fixed choice makes a run depend only on its inputs and seed: a chain's FP32 sums
+// take another order under another width or register budget, and its trajectory drifts from
+// there, so choices taken from timings that flip between processes would give another result
+// for the same seed.
+const std::pair<int, int> kDefaultPick[16] = {
+    {16, 3}, {64, 3}, {64, 3}, {64, 3},    // fewer than 24 atoms; torsions <= 9, <= 25, more
+    {16, 3}, {64, 3}, {64, 3}, {64, 3},    // 24 to 47
+    {16, 3}, {64, 3}, {64, 3}, {64, 3},    // 48 to 79
+    {64, 3}, {64, 3}, {64, 3}, {64, 3}};   // 80 and more
+std::pair<int, int> bucket_pick(vd_ctx* c, const vd::Ligand& L) {
+  const int b = size_bucket(L);
+  auto it = c->tuned.find(b);
+  return it != c->tuned.end() ? it->second : kDefaultPick[b];
+}
 int pick_group(vd_ctx* c, const vd::Ligand& L, int cpw) {
   const int gmin = min_group(L);
   if (gmin == 0) throw ArgError(VD_ERR_CAPACITY, "ligand has too many torsions");
@@ -312,17 +331,14 @@ int pick_group(vd_ctx* c, const vd::Ligand& L, int cpw) {
   if (cpw == 1) G = 64;
   else if (cpw == 2) G = 32;
   else if (cpw == 4) G = 16;
-  else {
-    auto it = c->tuned.find(size_bucket(L));
-    if (it != c->tuned.end()) G = it->second.first;
-  }
+  else G = bucket_pick(c, L).first;
   if (G < gmin) G = gmin;
   return G;
 }
 
 int pick_waves(vd_ctx* c, const vd::Ligand& L, int G) {
-  auto it = c->tuned.find(size_bucket(L));
-  return (it != c->tuned.end() && it->second.first == G) ? it->second.second : 2;
+  const std::pair<int, int> pick = bucket_pick(c, L);
+  return pick.first == G ? pick.second : 2;
 }
 
 vd_mc_consts make_consts(vd_ctx* c, const vd_dock_params* p, int depth, int bfgs) {
@@ -1221,7 +1237,7 @@ int vd_dock(vd_ctx* c, int id, const vd_dock_params* p, float* e, float* conf, f
     need_device(c);
     if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no maps");
     if (st) std::memset(st, 0, sizeof(*st));
-    if (p->chains_per_wave == 0 || p->waves_per_simd == 0) autotune(c, {id}, p);
+    if (c->autotune_on && (p->chains_per_wave == 0 || p->waves_per_simd == 0)) autotune(c, {id}, p);
     c->launches.clear();
     LigPlan pl = plan_for(c, id, p, true);
     prepare_batch(c, {pl});
@@ -1245,7 +1261,8 @@ static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, 
     need_device(c);
     if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no maps");
     if (st) std::memset(st, 0, sizeof(*st));
-    if (p->chains_per_wave == 0 || p->waves_per_simd == 0) autotune(c, std::vector<int>(ids, ids + n), p);
+    if (c->autotune_on && (p->chains_per_wave == 0 || p->waves_per_simd == 0))
+      autotune(c, std::vector<int>(ids, ids + n), p);
     c->launches.clear();
     std::vector<LigPlan> plans;
     for (int k = 0; k < n; ++k) plans.push_back(plan_for(c, ids[k], p, !persistent));
@@ -1329,6 +1346,13 @@ static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, 
     }
     flush();
   });
+}
+
+int vd_set_autotune(vd_ctx* c, int on) {
+  if (!c || (on != 0 && on != 1)) return VD_ERR_ARG;
+  c->autotune_on = on == 1;
+  if (!c->autotune_on) c->tuned.clear();
+  return VD_OK;
 }
 
 int vd_set_class_overlap(vd_ctx* c, int on) {
