@@ -226,3 +226,30 @@ def test_class_overlap_same_results(ctx, synth_setup, policy):
                 assert np.array_equal(x, y)
     if policy != 2:
         assert out[True][2].evals == out[False][2].evals
+
+
+def test_fixed_launch_shapes_and_autotune(ctx, synth_setup):
+    """Left at 0, lanes per chain and waves per SIMD come from the fixed table per size bucket:
+    16 lanes for a small ligand with few torsions, 64 for one with 12, and a second identical call
+    gives the same chains bit for bit.  With timing-based tuning on, the
+    batch still docks (trial launches, then the timed shapes) to finite results; turning it off
+    again restores the table's shapes and results."""
+    import vdock
+    s = synth_setup
+    ctx.grid_upload(s["box"], s["maps"])
+    ids = [ctx.add_ligand(vdock.synth_ligand(14, 2, 401)), ctx.add_ligand(vdock.synth_ligand(40, 12, 402))]
+    p = vdock.default_params(chains=64, search_depth=3, bfgs_steps=8, seed=9)
+    runs = []
+    for tune in (False, False, True, False):
+        ctx.set_autotune(tune)
+        say("autotune", tune)
+        ctx.dock_batch(ids, p)
+        runs.append(([ctx.batch_results(k, p) for k in range(len(ids))],
+                     sorted((r.lanes_per_chain, r.waves_per_simd) for r in ctx.last_launches())))
+    assert [g for g, _ in runs[0][1]] == [16, 64]
+    for k in range(len(ids)):
+        assert all(np.isfinite(x).all() for x in runs[2][0][k])
+        for r in (runs[1], runs[3]):
+            for x, y in zip(runs[0][0][k], r[0][k]):
+                assert np.array_equal(x, y)
+    assert runs[3][1] == runs[0][1]
