@@ -8,7 +8,7 @@
  *
  * The reference path this replaces (paths under the reference root, V11 = Vina-CUDA_v1.1):
  *   kernel_grid   V11/inc/cuda/kernel1.cuh:30-41   -> vd_grid_populate
- *   kernel_monte  V11/inc/cuda/kernel2.cuh:65-79   -> vd_dock / vd_dock_batch
+ *   kernel_monte  V11/inc/cuda/kernel2.cuh:25-38, defined at kernel2.cu:567   -> vd_dock / vd_dock_batch
  *   main_procedure_cl V11/lib/main_procedure_cu.h:2-4 (host driver)  -> the calls below in sequence
  *   cache::populate   V11/lib/cache.h:73            -> vd_grid_populate
  *   model::eval_deriv V11/lib/model.cpp:1324-1334   -> vd_eval
@@ -182,7 +182,7 @@ int vd_set_class_overlap(vd_ctx* ctx, int on);
  * in the reference (one fixed launch shape). */
 int vd_set_autotune(vd_ctx* ctx, int on);
 
-/* Plain per-ligand launch (replaces kernel_monte, V11/inc/cuda/kernel2.cuh:65-79).
+/* Plain per-ligand launch (replaces kernel_monte, V11/inc/cuda/kernel2.cuh:25-38, defined at kernel2.cu:567).
  * Per chain results: e[chains], conf[chains][7+T], heavy[chains][n_heavy][3] (host pointers,
  * any may be NULL).  The box is the one of the resident maps. */
 int vd_dock(vd_ctx* ctx, int id, const vd_dock_params* p, float* e, float* conf, float* heavy, vd_stats* st);
@@ -192,8 +192,11 @@ int vd_dock(vd_ctx* ctx, int id, const vd_dock_params* p, float* e, float* conf,
 int vd_dock_batch(vd_ctx* ctx, int n, const int* ids, const vd_dock_params* p, vd_stats* st);
 /* Same work as vd_dock_batch issued as one plain launch per ligand (the per-ligand baseline). */
 int vd_dock_batch_plain(vd_ctx* ctx, int n, const int* ids, const vd_dock_params* p, vd_stats* st);
-/* Results of ligand k (0..n-1) of the last batch, host pointers (any may be NULL). */
-int vd_batch_results(vd_ctx* ctx, int k, float* e, float* conf, float* heavy);
+/* Results of ligand k (0..n-1) of the last batch (the per-chain output of kernel_monte that
+ * cl_to_vina reads, V11/lib/main_procedure_cu.cpp:84-124): the first min(max_chains, chains) chains
+ * into e[], conf[][7+T], heavy[][n_heavy][3] (host pointers, any may be NULL).  Returns the number
+ * of chains the ligand was docked with (>= 0; max_chains = 0 asks for it) or an error. */
+int vd_batch_results(vd_ctx* ctx, int k, int max_chains, float* e, float* conf, float* heavy);
 /* Best energy per ligand of the last batch, best_e[n]. */
 int vd_batch_best(vd_ctx* ctx, float* best_e);
 
@@ -257,9 +260,11 @@ int vd_postprocess_batch(vd_ctx* ctx, int n, const int* ids, const int* poses, c
  * are first reduced to its pose list as the reference's host does (add_to_output_container with
  * min_rmsd 1 and max_poses, V11/lib/main_procedure_cu.cpp:908-926; max_poses 20 = num_saved_mins). */
 int vd_postprocess_docked(vd_ctx* ctx, int max_poses, const vd_post_params* p, vd_stats* st);
-/* Modes of ligand k of the last post-processing call: modes[num_modes], their conformations
- * conf[m][7+T] and all-atom coordinates coords[m][n_atoms][3] (NULL allowed).  Returns the count. */
-int vd_post_results(vd_ctx* ctx, int k, vd_mode* modes, float* conf, float* coords);
+/* Modes of ligand k of the last post-processing call: the first min(max_modes, count) modes into
+ * modes[], their conformations conf[][7+T] and all-atom coordinates coords[][n_atoms][3] (NULL
+ * allowed).  Returns the number of modes the ligand has (>= 0; max_modes = 0 asks for it) or an
+ * error. */
+int vd_post_results(vd_ctx* ctx, int k, int max_modes, vd_mode* modes, float* conf, float* coords);
 
 /* PDBQT text of n modes of ligand id (write_all_output, V11/main/main.cpp:89-102; vina_remark,
  * main.cpp:158-170; model::write_context, V11/lib/model.cpp:1063-1110): per mode "MODEL i", the

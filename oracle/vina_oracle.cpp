@@ -18,6 +18,7 @@
  */
 #include "oracle.h"
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <cstdio>
 #include <string>
@@ -41,6 +42,17 @@ const fl MAXFL = std::numeric_limits<fl>::max();        // V11/lib/common.h:271
 thread_local std::string g_err;
 int g_use_inter = 1, g_use_intra = 1;   // term switches for the finite-difference pins
 int fail(const std::string& m) { g_err = m; return -1; }
+
+// Worker threads: the hardware threads, capped by OMP_NUM_THREADS when it is set (the CPU share a
+// batch host grants the process).
+unsigned host_threads() {
+  unsigned n = std::max(1u, std::thread::hardware_concurrency());
+  if (const char* s = std::getenv("OMP_NUM_THREADS")) {
+    const long v = std::strtol(s, nullptr, 10);
+    if (v > 0) n = std::min<unsigned>(n, (unsigned)v);
+  }
+  return n;
+}
 
 struct V3 {
   fl v[3];
@@ -526,7 +538,7 @@ struct Grid {
     std::vector<int> all;
     for (size_t i = 0; i < rxyz.size(); ++i) all.push_back((int)i);
     const size_t plane = (size_t)dim[0] * dim[1] * dim[2];
-    unsigned nth = std::max(1u, std::thread::hardware_concurrency());
+    unsigned nth = host_threads();
     std::vector<std::thread> pool;
     for (unsigned t = 0; t < nth; ++t) pool.emplace_back([&, t]() {
       fl aff[XS_SIZE], slk[XS_SIZE];
@@ -1461,7 +1473,7 @@ int orc_mc_run(void* gp, void* lp, const orc_mc_params* p, double* e, double* co
   Ligand* L = (Ligand*)lp; Grid* G = (Grid*)gp;
   const int T = L->T(), H = L->heavy;
   std::atomic<int> next(0); std::atomic<long long> total(0);
-  int nth = p->threads > 0 ? p->threads : (int)std::max(1u, std::thread::hardware_concurrency());
+  int nth = p->threads > 0 ? p->threads : (int)host_threads();
   std::vector<McState> st(p->chains);
   auto output = [&](int c) {
     e[c] = st[c].best.e;

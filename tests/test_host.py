@@ -144,9 +144,12 @@ def test_parse_errors_are_reported(host_ctx):
 
 def test_capacity_errors_are_reported(host_ctx):
     import vdock
-    big = vdock.synth_ligand(140, 10, 5)     # more atoms than a ligand record holds
+    big = vdock.synth_ligand(260, 10, 5)     # more atoms than the 8-bit atom indices of a ligand record address
     with pytest.raises(vdock.VdockError):
         host_ctx.add_ligand(big)
+    # beyond the reference's 128-atom cap (MAX_NUM_OF_ATOMS) but within the record's
+    lid = host_ctx.add_ligand(vdock.synth_ligand(200, 20, 5))
+    assert host_ctx.ligand_info(lid).n_atoms >= 200
     n_before = host_ctx.lib.vd_ligand_count(host_ctx.h)
     ok = host_ctx.add_ligand(vdock.synth_ligand(12, 2, 5))
     assert ok == n_before   # a rejected ligand does not consume an id
@@ -190,6 +193,7 @@ def test_pair_chunks_cover_every_pair_once_without_conflicts(host_ctx, lanes):
     from conftest import golden
     texts = [golden(f"{n}_ligand.pdbqt") for n in ("1u1b", "2bm2", "2er7")]
     texts += [vdock.synth_ligand(h, min(16, h // 3), 300 + h) for h in (10, 25, 40, 60)]
+    texts += [vdock.synth_ligand(h, 24, 300 + h) for h in (140, 225)]      # beyond 128 atoms
     for text in texts:
         lid = host_ctx.add_ligand(text)
         pairs = host_ctx.ligand_export(lid)["pairs"]          # (tp, a, b)

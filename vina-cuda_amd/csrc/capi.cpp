@@ -159,6 +159,7 @@ struct vd_ctx {
   BatchState batch;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int n_cu = 0;
+  int lds_max = 64 * 1024;                   // LDS one workgroup may take (queried from the device)
   std::map<int, std::pair<int, int>> tuned;  // size bucket -> (lanes per chain, waves per SIMD)
   bool autotune_on = false;                   // timing-based choice of the pair above (vd_set_autotune)
   DevBuf<float> tune_e, tune_conf, tune_heavy;
@@ -428,7 +429,9 @@ void mc_prepare(vd_ctx* c, McSlot& s, int G, int W, Packed&& pk, std::vector<uin
   a.slot_evals = s.evals.p;
   a.Amax = s.pk.Amax; a.Nmax = s.pk.Nmax;
   const int lds = vdl::mc_lds_bytes(G, s.pk.Amax, s.pk.Nmax, P.local_search == 0);
-  if (lds > 64 * 1024) throw ArgError(VD_ERR_CAPACITY, "ligand too large for the LDS layout");
+  if (lds > c->lds_max)
+    throw ArgError(VD_ERR_CAPACITY, "ligand too large for the LDS layout (" + std::to_string(lds) + " bytes of " +
+                                        std::to_string(c->lds_max) + " per workgroup)");
   const int mode = P.local_search == 0 ? 0 : (P.policy == VD_MC_QVINA2 ? 2 : P.policy == VD_MC_QVINAW ? 3 : 1);
   // a tighter register budget only pays when the LDS of this launch lets the extra waves in:
   // take the loosest budget that reaches the same residency
@@ -809,7 +812,9 @@ RefineOut run_refine(vd_ctx* c, const std::vector<int>& ids, const std::vector<i
     cin[i] = conf[i];
   }
   const int lds = vdl::refine_lds_bytes(Amax, Nmax, nmax);
-  if (lds > 64 * 1024) throw ArgError(VD_ERR_CAPACITY, "ligand too large for the refinement's LDS layout");
+  if (lds > c->lds_max)
+    throw ArgError(VD_ERR_CAPACITY, "ligand too large for the refinement's LDS layout (" + std::to_string(lds) +
+                                        " bytes of " + std::to_string(c->lds_max) + " per workgroup)");
   if (pw.empty()) pw.push_back(VD_NO_PAIR);
   c->ref_geo.ensure(geo.size()); c->ref_pairs.ensure(pw.size()); c->ref_hdr.ensure(hdr.size());
   c->ref_lig.ensure(n); c->ref_evals.ensure(n); c->ref_res.ensure(n);
@@ -932,6 +937,7 @@ int vd_create(int device, vd_ctx** out) {
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, device));
     c->n_cu = prop.multiProcessorCount;
+    c->lds_max = (int)prop.sharedMemPerBlock;
     upload_lut(c);
   });
   if (rc != VD_OK) { delete c; return rc; }
@@ -1387,18 +1393,22 @@ int vd_last_launches(vd_ctx* c, int max, vd_launch* out) {
   return n;
 }
 
-int vd_batch_results(vd_ctx* c, int k, float* e, float* conf, float* heavy) {
-  if (!c) return VD_ERR_ARG;
-  return guard(c, [&]() {
+int vd_batch_results(vd_ctx* c, int k, int max_chains, float* e, float* conf, float* heavy) {
+  if (!c || max_chains < 0) return VD_ERR_ARG;
+  int count = 0;
+  const int rc = guard(c, [&]() {
     need_device(c);
     BatchState& b = c->batch;
     if (k < 0 || k >= (int)b.ids.size()) throw ArgError(VD_ERR_ARG, "batch index out of range");
     const longlong4 o = b.off[k];
-    const size_t ch = b.chains[k];
+    count = b.chains[k];
+    const size_t ch = (size_t)std::min(count, max_chains);
+    if (ch == 0) return;
     if (e) HIP_CHECK(hipMemcpy(e, b.e.p + o.x, ch * 4, hipMemcpyDeviceToHost));
     if (conf) HIP_CHECK(hipMemcpy(conf, b.conf.p + o.y, ch * (7 + b.T[k]) * 4, hipMemcpyDeviceToHost));
     if (heavy) HIP_CHECK(hipMemcpy(heavy, b.heavy.p + o.z, ch * b.H[k] * 12, hipMemcpyDeviceToHost));
   });
+  return rc == VD_OK ? count : rc;
 }
 
 int vd_batch_best(vd_ctx* c, float* best_e) {
@@ -1499,16 +1509,19 @@ int vd_postprocess_docked(vd_ctx* c, int max_poses, const vd_post_params* p, vd_
   });
 }
 
-int vd_post_results(vd_ctx* c, int k, vd_mode* modes, float* conf, float* coords) {
-  if (!c) return VD_ERR_ARG;
+int vd_post_results(vd_ctx* c, int k, int max_modes, vd_mode* modes, float* conf, float* coords) {
+  if (!c || max_modes < 0) return VD_ERR_ARG;
   int count = 0;
   const int rc = guard(c, [&]() {
     if (k < 0 || k >= (int)c->post.ligs.size()) throw ArgError(VD_ERR_ARG, "post-processing index out of range");
     const PostLigand& pl = c->post.ligs[k];
-    if (modes) std::copy(pl.modes.begin(), pl.modes.end(), modes);
-    if (conf) std::copy(pl.conf.begin(), pl.conf.end(), conf);
-    if (coords) std::copy(pl.coords.begin(), pl.coords.end(), coords);
     count = (int)pl.modes.size();
+    const size_t m = (size_t)std::min(count, max_modes);
+    if (m == 0) return;
+    const size_t per_conf = pl.conf.size() / pl.modes.size(), per_xyz = pl.coords.size() / pl.modes.size();
+    if (modes) std::copy(pl.modes.begin(), pl.modes.begin() + m, modes);
+    if (conf) std::copy(pl.conf.begin(), pl.conf.begin() + m * per_conf, conf);
+    if (coords) std::copy(pl.coords.begin(), pl.coords.begin() + m * per_xyz, coords);
   });
   return rc == VD_OK ? count : rc;
 }

@@ -786,77 +786,6 @@ __device__ bool qvw_interesting(const History& h, const GlobalHist& gh, const Co
   return glob || !full || ind;
 }
 
-// ----------------------------------------------------------------------------------------
-// RILC-BFGS local search with the Lewis-Overton line search
-// (V11/inc/cuda/quasi_newton.cuh:1710-1806, 2172-2389), run by every group of the wave in
-// lockstep.  Groups with enabled == false evaluate harmlessly and keep x unchanged.
-// Dot products are exact group sums over all 6+T components.
-// QV: QuickVina2 gate on the start point with the chain's history (see History).
-template <int G, bool QV>
-__device__ float rilc_bfgs(const LigView& L, const Scratch& S, const EvalK& K, Conf& x, bool enabled,
-                           float x_e, int max_steps, int l, int& evals, History* hist) {
-  float g;
-  float fx = eval_deriv<G>(L, S, K, x, l, g);
-  evals += enabled ? 1 : 0;
-  const float fx_orig = fx;
-  stash_put<G>(S.stash, kSlotOrig, x);
-  bool search = enabled;
-  if constexpr (QV) {
-    const bool interesting = hist_interesting<G>(*hist, x, fx, g, l, L.T, enabled);
-    hist_add<G>(*hist, x, fx, g, l, L.T, enabled && interesting);
-    search = enabled && interesting;
-  }
-  float d = -g;
-  bool active = search && (sqrtf(gsum<G>(g * g)) >= 1e-5f);
-  float step = 1.0f / sqrtf(gsum<G>(d * d));
-  int k = 1;
-  while (__any(active)) {
-    stash_put<G>(S.stash, kSlotPrev, x);
-    const float gp = g; const float fxp = fx;
-    // line search
-    const float dginit = gsum<G>(gp * d);
-    bool ls = active && !(0.0f < dginit);
-    int result = -1;
-    const float finit = fx, dgtest = 1e-4f * dginit, dstest = 0.1f * dginit;
-    float mu = 0.0f, nu = 1.0e20f; bool brackt = false; int count = 0;
-    while (__any(ls)) {
-      const Conf xt = increment<G>(stash_get<G>(S.stash, kSlotPrev), d, step, l, L.T);
-      float gt;
-      const float ft = eval_deriv<G>(L, S, K, csel(ls, xt, x), l, gt);
-      const float gtd = gsum<G>(gt * d);
-      if (ls) {
-        x = xt; fx = ft; g = gt; ++count; ++evals;
-        bool done = false;
-        if (fx > finit + step * dgtest) { nu = step; brackt = true; }
-        else if (gtd < dstest) { mu = step; }
-        else { result = count; done = true; }
-        if (!done && count >= 10) { result = (fx > finit) ? -1 : count; done = true; }
-        if (!done) step = brackt ? 0.5f * (mu + nu) : 2.0f * step;
-        if (done) ls = false;
-      }
-    }
-    const float gnorm = sqrtf(gsum<G>(g * g));
-    bool update = false;
-    if (active) {
-      if (result < 0) { x = stash_get<G>(S.stash, kSlotPrev); g = gp; fx = fxp; active = false; }
-      else if (!(gnorm >= 1e-5f)) active = false;
-      else if ((max_steps != 0 && max_steps <= k) || k >= 1000) active = false;   // 1000: hard bound
-      else { ++k; update = true; }
-    }
-    // one-pair update of the direction (sign of the y term as in quasi_newton.cuh:2342-2350)
-    const float s = step * d, y = g - gp;
-    const float ys = gsum<G>(y * s), yy = gsum<G>(y * y), ss = gsum<G>(s * s), gpgp = gsum<G>(gp * gp);
-    const float nd = -g;
-    const float alpha = gsum<G>(s * nd) / ys;
-    const float nd2 = (nd + alpha * y) * (ys / yy);
-    const float beta = gsum<G>(y * nd2) / ys;
-    const float nd3 = nd2 + (alpha - beta) * s;
-    if (update) { d = (ys > ss * sqrtf(gpgp) * 1.0e-6f) ? nd3 : nd; step = 1.0f; }
-  }
-  if (enabled && !(fx <= fx_orig)) { fx = fx_orig; x = stash_get<G>(S.stash, kSlotOrig); }
-  return enabled ? fx : x_e;
-}
-
 // Dense BFGS with backtracking line search (V11/lib/bfgs.h:30-140; device form
 // quasi_newton.cuh:1859-1916, 2391-2499).  H (n x n, n = 6+T <= G) lives in LDS, row i in the
 // lane that owns change component i.

@@ -549,7 +549,7 @@ std::vector<std::vector<uint32_t>> chunks_by_colouring(const std::vector<uint32_
 
 // Greedy first fit over the pairs in list order.
 std::vector<std::vector<uint32_t>> chunks_first_fit(const std::vector<uint32_t>& pairs, int G) {
-  struct Chunk { uint64_t as[2] = {0, 0}, bs[2] = {0, 0}; int n = 0; };
+  struct Chunk { uint64_t as[4] = {0, 0, 0, 0}, bs[4] = {0, 0, 0, 0}; int n = 0; };   // one bit per atom (< 256)
   auto has = [](const uint64_t* m, int i) { return (m[i >> 6] >> (i & 63)) & 1u; };
   auto set = [](uint64_t* m, int i) { m[i >> 6] |= 1ull << (i & 63); };
   std::vector<Chunk> chunks;

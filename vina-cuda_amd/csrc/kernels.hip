@@ -131,12 +131,13 @@ __device__ __forceinline__ void make_views(const float4* __restrict__ blob, cons
   H = reinterpret_cast<float*>(S.fx + G + 1);
 }
 
-template <int G, int MODE>
+// Dense-BFGS local search of the groups with enabled == true (the RILC-BFGS modes run inside
+// mc_chain_flat's state machine instead).
+template <int G>
 __device__ float local_search(const LigView& L, const Scratch& S, float* H, const EvalK& K, Conf& x,
-                              bool enabled, float x_e, int max_steps, int l, int& evals, History* hist) {
+                              bool enabled, float x_e, int max_steps, int l, int& evals) {
   if (!__any(enabled)) return x_e;
-  if constexpr (MODE >= 1) return rilc_bfgs<G, MODE == 2>(L, S, K, x, enabled, x_e, max_steps, l, evals, hist);
-  else return dense_bfgs<G>(L, S, H, H + G * G, K, x, enabled, x_e, max_steps, l, evals);
+  return dense_bfgs<G>(L, S, H, H + G * G, K, x, enabled, x_e, max_steps, l, evals);
 }
 
 // Initial conformation of a chain (conf::randomize, V11/lib/conf.h:127-130, 214-217;
@@ -169,16 +170,15 @@ __device__ Conf initial_conf(const vd_mc_consts& P, uint32_t stream, uint32_t ch
   return c;
 }
 
-// One MC chain per group with the reference GPU schedule (V11/inc/cuda/kernel2.cu:290-426):
+// One MC chain per group with the reference GPU schedule and dense BFGS (--rilc_bfgs 0,
+// V11/inc/cuda/kernel2.cu:290-426 with bfgs_optimized, quasi_newton.cuh:2391-2499):
 // mutate -> local search -> Metropolis(T) -> on accept, if better than the best so far: second
 // local search, keep the best.  Mutation as V11/lib/mutate.cpp:62-127 (see initial_conf for the
 // random stream); the orientation step uses the gyration radius of the current conformation.
-// hist != nullptr: QuickVina2 policy (QV2/inc/cuda/kernel2.cu, same schedule with the visited
-// history of the chain gating both local searches).
-template <int G, int MODE>
+template <int G>
 __device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const EvalK& K, const vd_mc_consts& P,
                          int depth, int bfgs_steps, uint32_t stream, uint32_t chain, bool valid, int l, float* out_e,
-                         float* out_conf, float* out_heavy, int& evals, History* hist) {
+                         float* out_conf, float* out_heavy, int& evals) {
   const uint32_t k0 = P.seed, k1 = stream;
   Conf tmp = initial_conf<G>(P, stream, chain, l, L.T);
   float tmp_e = 0.0f;
@@ -203,11 +203,11 @@ __device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const Eva
     if (which == 0) { cand.px += P.amplitude * sx; cand.py += P.amplitude * sy; cand.pz += P.amplitude * sz; }
     else if (which == 1) { if (rg > kEpsFl) { const float f = P.amplitude / rg; cand.q = qincrement(cand.q, f * sx, f * sy, f * sz); } }
     else if (l == which - 1) cand.tor = tor;
-    const float ce = local_search<G, MODE>(L, S, H, K, cand, valid, 0.0f, bfgs_steps, l, evals, hist);
+    const float ce = local_search<G>(L, S, H, K, cand, valid, 0.0f, bfgs_steps, l, evals);
     const bool accept = valid && (step == 0 || ce < tmp_e || u_metro < expf((tmp_e - ce) / P.temperature));
     if (accept) { tmp = cand; tmp_e = ce; } else tmp = stash_get<G>(S.stash, kSlotTmp);
     const bool second = accept && tmp_e < best_e;
-    const float e2 = local_search<G, MODE>(L, S, H, K, tmp, second, tmp_e, bfgs_steps, l, evals, hist);
+    const float e2 = local_search<G>(L, S, H, K, tmp, second, tmp_e, bfgs_steps, l, evals);
     if (second) { tmp_e = e2; if (tmp_e < best_e) { stash_put<G>(S.stash, kSlotBest, tmp); best_e = tmp_e; } }
     if (__any(accept)) {
       set_coords<G>(L, S, tmp, l);
@@ -258,8 +258,8 @@ __device__ void write_chain(const LigView& L, const Scratch& S, const Conf& best
   }
 }
 
-// The same MC schedule and RILC-BFGS arithmetic as mc_chain + rilc_bfgs, run as a flat state
-// machine: every pass evaluates one conformation per chain group and each group then advances
+// The MC schedule of mc_chain with RILC-BFGS and the Lewis-Overton line search
+// (V11/inc/cuda/quasi_newton.cuh:1710-1806, 2172-2389), run as a flat state machine: every pass evaluates one conformation per chain group and each group then advances
 // its own state (start of a local search, line-search trial, end of a BFGS iteration, end of a
 // local search, Metropolis step, next mutation) as far as it can without another evaluation.
 // Chains of a wavefront thus never wait for each other's line searches or local searches; they
@@ -459,9 +459,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
       for (int k = 0; k < 3; ++k) { gh.lo[k] = a.P.box_lo[k]; gh.hi[k] = a.P.box_hi[k]; }
     }
     if constexpr (MODE == 0)
-      mc_chain<G, MODE>(L, S, H, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l,
-                        a.out_e + a.out_off[wi.x].x, a.out_conf + a.out_off[wi.x].y, a.out_heavy + a.out_off[wi.x].z,
-                        evals, &hist);
+      mc_chain<G>(L, S, H, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l,
+                  a.out_e + a.out_off[wi.x].x, a.out_conf + a.out_off[wi.x].y, a.out_heavy + a.out_off[wi.x].z, evals);
     else
       mc_chain_flat<G, MODE - 1>(L, S, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l,
                                  a.out_e + a.out_off[wi.x].x, a.out_conf + a.out_off[wi.x].y,

@@ -12,9 +12,10 @@
 #define VD_CUTOFF_SQR 64.0f
 #define VD_ZERO_QUAD (VD_NUM_TYPE_PAIRS * VD_LUT_BINS)   // first quad of the all-zero row after the pair table
 #define VD_GRID_ROW 20               // per (t1, bin) row of the map LUT: 17 t2 values padded to 20
-#define VD_MAX_ATOMS 128             // per ligand (the reference cap MAX_NUM_OF_ATOMS, V11/inc/kernel2.h:35)
+#define VD_MAX_ATOMS 255             // per ligand: atom indices are 8-bit fields of the pair words and node
+                                     // atom ranges (the reference caps at 128, MAX_NUM_OF_ATOMS, V11/inc/kernel2.h:35)
 #define VD_MAX_NODES 64              // rigid nodes (torsions + 1)
-#define VD_MAX_PAIRS 16384
+#define VD_MAX_PAIRS (VD_MAX_ATOMS * (VD_MAX_ATOMS - 1) / 2)   // every pair of atoms (reference: 2048)
 
 // Device ligand record: all ligands of a job live in one blob of 16-byte words.
 // Atom word:  x,y,z = coordinates relative to the owning node's origin; w = meta bits

@@ -131,7 +131,7 @@ def load():
         "vd_dock": (C.c_int, [vp, C.c_int, C.POINTER(DockParams), fp, fp, fp, C.POINTER(Stats)]),
         "vd_dock_batch": (C.c_int, [vp, C.c_int, ip, C.POINTER(DockParams), C.POINTER(Stats)]),
         "vd_dock_batch_plain": (C.c_int, [vp, C.c_int, ip, C.POINTER(DockParams), C.POINTER(Stats)]),
-        "vd_batch_results": (C.c_int, [vp, C.c_int, fp, fp, fp]),
+        "vd_batch_results": (C.c_int, [vp, C.c_int, C.c_int, fp, fp, fp]),
         "vd_batch_best": (C.c_int, [vp, fp]),
         "vd_last_launches": (C.c_int, [vp, C.c_int, C.POINTER(Launch)]),
         "vd_set_class_overlap": (C.c_int, [vp, C.c_int]),
@@ -150,7 +150,7 @@ def load():
                                       C.POINTER(Mode)]),
         "vd_postprocess_batch": (C.c_int, [vp, C.c_int, ip, ip, fp, C.POINTER(PostParams), C.POINTER(Stats)]),
         "vd_postprocess_docked": (C.c_int, [vp, C.c_int, C.POINTER(PostParams), C.POINTER(Stats)]),
-        "vd_post_results": (C.c_int, [vp, C.c_int, C.POINTER(Mode), fp, fp]),
+        "vd_post_results": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(Mode), fp, fp]),
         "vd_write_pdbqt": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(Mode), fp, C.c_char_p, C.c_int]),
     }
     for name, (res, args) in sig.items():
@@ -375,14 +375,17 @@ class Context:
         self._batch_ids = list(a)
         return st
 
-    def batch_results(self, k, params):
-        lid = self._batch_ids[k]
-        inf = self.ligand_info(lid)
-        n = params.chains
+    def batch_results(self, k, params=None):
+        """Per-chain results of ligand k of the last batch -> (e, conf, heavy), sized by the chain
+        count the ligand was docked with (params is accepted for older callers and ignored)."""
+        n = self.lib.vd_batch_results(self.h, k, 0, None, None, None)
+        self._check(min(n, 0))
+        inf = self.ligand_info(self._batch_ids[k])
         e = np.zeros(n, dtype=np.float32)
         conf = np.zeros((n, 7 + inf.n_torsions), dtype=np.float32)
         heavy = np.zeros((n, inf.n_heavy, 3), dtype=np.float32)
-        self._check(self.lib.vd_batch_results(self.h, k, _f(e), _f(conf), _f(heavy)))
+        m = self.lib.vd_batch_results(self.h, k, n, _f(e), _f(conf), _f(heavy))
+        self._check(min(m, 0))
         return e, conf, heavy
 
     def batch_best(self):
@@ -460,16 +463,20 @@ class Context:
         self._post_ids = list(self._batch_ids)
         return st
 
-    def post_results(self, k, num_modes=9):
-        """Modes of ligand k of the last post-processing call -> (modes, conf, coords)."""
-        self._check(min(self.lib.vd_post_results(self.h, k, None, None, None), 0))
+    def post_results(self, k, num_modes=None):
+        """Modes of ligand k of the last post-processing call -> (modes, conf, coords): all of them,
+        or at most num_modes."""
+        n = self.lib.vd_post_results(self.h, k, 0, None, None, None)
+        self._check(min(n, 0))
+        if num_modes is not None:
+            n = min(n, num_modes)
         inf = self.ligand_info(self._post_ids[k])
-        modes = (Mode * num_modes)()
-        conf = np.zeros((num_modes, 7 + inf.n_torsions), dtype=np.float32)
-        xyz = np.zeros((num_modes, inf.n_atoms, 3), dtype=np.float32)
-        m = self.lib.vd_post_results(self.h, k, modes, _f(conf), _f(xyz))
+        modes = (Mode * max(n, 1))()
+        conf = np.zeros((n, 7 + inf.n_torsions), dtype=np.float32)
+        xyz = np.zeros((n, inf.n_atoms, 3), dtype=np.float32)
+        m = self.lib.vd_post_results(self.h, k, n, modes, _f(conf), _f(xyz))
         self._check(min(m, 0))
-        return list(modes[:m]), conf[:m], xyz[:m]
+        return list(modes[:n]), conf, xyz
 
     def write_pdbqt(self, lid, modes, coords):
         """PDBQT text of the given modes (coords[m][n_atoms][3])."""
