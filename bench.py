@@ -7,16 +7,23 @@ points, all 17 FP32 maps resident in HBM), 1,024 synthetic ligands (10-60 heavy 
 search.  One step = docking all 1,024 ligands.
 
 Printed beside it on the same JSON line:
-  per_ligand_plain  the plain per-ligand launch loop and the batched launch on the same ligand
-                    subset, with their best scores (equal-accuracy check)
-  grid              17-map precompute of the same receptor/box: kernel time and GB/s
+  roofline          the dominant kernel (the MC kernel): algorithmic bytes (map corners + pair
+                    quads) over the span of the step's concurrent MC launches, against 8 TB/s as
+                    the contract asks; beside it the HBM traffic those launches cause (derived from
+                    the committed counter profile's bytes per algorithmic byte), its fraction of
+                    8 TB/s, and the VALU busy share the counter profile measured -- the kernel is
+                    bound by instruction issue and latency, not by bandwidth
+  per_ligand_plain  the plain per-ligand launch loop and the batched launch on the same ligands;
+                    equal accuracy over several seeds on a subset (best-score differences and the
+                    RMSD between the two paths' top poses)
+  grid              17-map precompute of the same receptor/box: kernel time, GB/s, and the
+                    (grid point, receptor atom) terms per second it evaluates
   adjoint           forward + adjoint evaluations of 2^20 random conformations of a 40-atom,
-                    12-torsion ligand (evals/s)
-  roofline          the dominant kernel (the MC kernel) against HBM peak (8 TB/s, and beside it
-                    the bandwidth a timed 1 GiB device-to-device copy reaches), algorithmic bytes; a
-                    step's MC launches (one per lanes-per-chain class) run at the same time on
-                    their own streams, so the unit is the step's set of MC launches: their
-                    algorithmic bytes over their span (first start to last end, HIP events)
+                    12-torsion ligand (evals/s) and their error against the FP64 oracle
+  single_ligand     one 32-atom, 8-torsion ligand with 8,192 chains through the plain launch
+  small_scene       the CPU-runnable case: 2,000-atom receptor, 20 A box, one 20-atom 5-torsion
+                    ligand, 512 chains -- on the device, and the oracle's Monte Carlo on host cores
+  dense_bfgs        the screening step with dense BFGS (the reference's --rilc_bfgs 0)
   post              the step after the search for the docked batch: pose lists, FP64 refinement of
                     every pose against the receptor atoms in one launch, final scores; beside it
                     the oracle's FP64 refinement of a sample of the same poses on host threads
@@ -24,8 +31,8 @@ Printed beside it on the same JSON line:
                     gate, throughput, mean best score)
   large_box         large-box batched screening with the QuickVina2 policy (20,000-atom receptor,
                     188^3 maps, 4,096 ligands of 10-80 heavy atoms, 128 chains)
-  cpu_baseline      the FP64 oracle's Monte Carlo (same schedule, same random stream) on host
-                    cores, over a bounded sample of the same ligands
+  cpu_baseline      the FP64 oracle's Monte Carlo (same schedule, same random stream) on the host
+                    cores this process may use, over a bounded sample of the same ligands
 """
 import argparse
 import json
@@ -46,21 +53,35 @@ def log(*a):
 
 
 def build_scene(vdock, ctx):
-    xyz, xs = vdock.synth_receptor(8000, 60.0, 10.0, 42)
-    ctx.set_receptor(xyz, xs)
-    box = vdock.box_from_center([0, 0, 0], [30.0, 30.0, 30.0], 0.375)
-    ms = ctx.grid_populate(box)
-    return xyz, xs, box, ms
+    from vdock import workloads as W
+    xyz, xs, box = W.single_ligand_scene(vdock, ctx)
+    return xyz, xs, box, vdock.load().vd_grid_last_ms(ctx.h)
 
 
 def screening_ligands(vdock, n=1024):
-    rng = np.random.default_rng(2024)
-    out = []
-    for i in range(n):
-        heavy = int(rng.integers(10, 61))
-        tors = int(rng.integers(0, min(16, heavy // 3) + 1))
-        out.append(vdock.synth_ligand(heavy, tors, i))
-    return out
+    from vdock import workloads as W
+    return W.texts(vdock, W.screening_specs(n))
+
+
+def host_threads():
+    """Worker threads for the CPU legs: the cores this process may run on (its affinity mask),
+    capped by the cgroup's CPU quota and by OMP_NUM_THREADS when set (the share a batch host
+    grants), with the counts behind the choice."""
+    info = {"nproc": os.cpu_count()}
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    info["affinity"] = n
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            info["cgroup_quota_cpus"] = round(int(quota) / int(period), 2)
+            n = min(n, max(1, int(int(quota) / int(period))))
+    except (OSError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        info["OMP_NUM_THREADS"] = int(omp)
+        n = min(n, int(omp))
+    return max(1, n), info
 
 
 def dominant_kernel(launches):
@@ -82,18 +103,11 @@ def large_box_leg(vdock, log, n_ligands=4096, chains=128):
     (90 A cube, 25 A cavity), 70 A box at 0.375 A (188^3 points, 17 FP32 maps = 452 MB), 4,096
     synthetic ligands (10-80 heavy atoms, 0-24 rotatable bonds drawn, capped by the bonds the tree
     has), 128 chains each, RILC-BFGS with the visited-history gate."""
+    from vdock import workloads as W
     ctx = vdock.Context(0)
-    xyz, xs = vdock.synth_receptor(20000, 90.0, 25.0, 4242)
-    ctx.set_receptor(xyz, xs)
-    box = vdock.box_from_center([0, 0, 0], [70.0, 70.0, 70.0], 0.375)
-    ctx.grid_populate(box)
+    _, _, box = W.large_box_scene(vdock, ctx)
     g_ms = float(np.median([ctx.grid_populate(box) for _ in range(3)]))
-    rng = np.random.default_rng(4096)
-    ids = []
-    for i in range(n_ligands):
-        heavy = int(rng.integers(10, 81))
-        tors = int(rng.integers(0, 25))
-        ids.append(ctx.add_ligand(vdock.synth_ligand(heavy, tors, 100000 + i)))
+    ids = [ctx.add_ligand(t) for t in W.texts(vdock, W.large_box_specs(n_ligands))]
     p = vdock.default_params(chains=chains, seed=11, policy=1)
     log("large box: warmup")
     ctx.dock_batch(ids, p)
@@ -175,10 +189,139 @@ def traffic_from_profile(kernel, algorithmic_bytes):
     return None
 
 
-def run_cpu_baseline(vdock, texts, box, xyz, xs, chains, budget_s=15.0):
+def profile_figures(kernel):
+    """VALU busy share and resident waves per CU that the committed counter profile measured for
+    `kernel` on this workload (None when it has no figure for it)."""
+    path = os.path.join(ROOT, "profiles", "vd_mc_kernel_batched_screening.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("<!-- valu:"):
+            fields = dict(f.split("=", 1) for f in line.strip("<!-> \n").split()[1:] if "=" in f)
+            if fields.get("kernel") == kernel.replace(" ", ""):
+                return {"valu_busy": float(fields["valu_busy"]), "waves_per_cu": float(fields["waves_per_cu"])}
+    return None
+
+
+def grid_terms(xyz, xs, box):
+    """(grid point, typed receptor atom) pairs within the 8 A cutoff: each is one table row of the
+    17 map values, the unit of work of the precompute."""
+    from scipy.spatial import cKDTree
+    b, e, n = box.as_tuple()
+    axes = [np.linspace(b[k], e[k], n[k] + 1) for k in range(3)]
+    tree = cKDTree(np.asarray(xyz, dtype=np.float64)[np.asarray(xs) < 17])
+    X, Y = np.meshgrid(axes[0], axes[1], indexing="ij")
+    total = 0
+    for z in axes[2]:
+        pts = np.stack([X.ravel(), Y.ravel(), np.full(X.size, z)], 1)
+        total += int(tree.query_ball_point(pts, 8.0, return_length=True).sum())
+    return total
+
+
+def post_leg(ctx, log, max_poses=20):
+    """The step after the search for every ligand of the last docking batch: each ligand's chains
+    reduced to at most max_poses poses (add_to_output_container), every pose refined against the
+    receptor atoms in one FP64 launch, final scores and mode selection; timed on its second call
+    (the first, reported as first_call_wall_ms, also builds the per-box set-up)."""
+    log("post: refinement + final scores of the docked batch")
+    t0 = time.perf_counter()
+    ctx.postprocess_docked(max_poses)
+    first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    st = ctx.postprocess_docked(max_poses)
+    wall = time.perf_counter() - t0
+    n_lig = len(ctx._post_ids)
+    results = [ctx.post_results(k) for k in range(n_lig)]
+    best = [m[0].e for m, _, _ in results if m]
+    out = {"ligands": n_lig, "poses": st.blocks, "kernel_ms": round(st.kernel_ms, 2), "wall_ms": round(1000 * wall, 2),
+           "first_call_wall_ms": round(1000 * first, 2),
+           "poses_per_s_kernel": round(st.blocks / (st.kernel_ms / 1000.0), 1),
+           "ligands_per_s_wall": round(n_lig / wall, 2), "evals": int(st.evals),
+           "evals_per_s_kernel": round(st.evals / (st.kernel_ms / 1000.0), 1),
+           "mean_best_final_score": round(float(np.mean(best)), 4), "dtype": "f64"}
+    return out, [conf for _, conf, _ in results[:64]]
+
+
+def plain_accuracy(vdock, ctx, log, ids, chains, seeds=(7, 8, 9, 10), every=8):
+    """Equal accuracy of the batched launch and the plain per-ligand launches over several seeds,
+    on every `every`-th ligand: per ligand and seed, the two paths' best scores and the RMSD
+    (heavy atoms) between their top poses.  Both run the same chains; they differ only in the
+    lanes per chain the plain launch takes (a launch of one ligand's chains fills the GPU with one
+    chain per wavefront), so FP32 rounding, and with it the trajectories, part after a while."""
+    sub = ids[::every]
+    d_best, rms = [], []
+    for seed in seeds:
+        p = vdock.default_params(chains=chains, seed=seed)
+        log(f"equal accuracy, seed {seed}: batched and plain over {len(sub)} ligands")
+        runs = []
+        for plain in (False, True):
+            ctx.dock_batch(sub, p, plain=plain)
+            runs.append([ctx.batch_results(k) for k in range(len(sub))])
+        for (eb, _, hb), (ep, _, hp) in zip(*runs):
+            ib, ip = int(np.argmin(eb)), int(np.argmin(ep))
+            d_best.append(float(eb[ib] - ep[ip]))
+            rms.append(float(np.sqrt(((hb[ib] - hp[ip]) ** 2).sum(-1).mean())))
+    d_best, rms = np.array(d_best), np.array(rms)
+    return {"ligands": len(sub), "seeds": list(seeds),
+            "mean_best_score_diff_batched_minus_plain": round(float(d_best.mean()), 4),
+            "mean_abs_best_score_diff": round(float(np.abs(d_best).mean()), 4),
+            "top_pose_rmsd_median_A": round(float(np.median(rms)), 3),
+            "top_pose_within_2A_fraction": round(float((rms <= 2.0).mean()), 4)}
+
+
+def single_ligand_leg(vdock, ctx, log):
+    """One 32-heavy-atom, 8-torsion ligand with 8,192 chains through the plain per-ligand launch
+    (the drop-in for kernel_monte) on the screening scene's maps."""
+    log("single ligand: 8192 chains")
+    lid = ctx.add_ligand(vdock.synth_ligand(32, 8, 1))
+    p = vdock.default_params(chains=8192, seed=11)
+    ctx.dock(lid, p)
+    t0 = time.perf_counter()
+    e, _, _, st = ctx.dock(lid, p)
+    wall = time.perf_counter() - t0
+    inf = ctx.ligand_info(lid)
+    return {"workload": "one ligand of 32 heavy atoms and 8 torsions, 8192 chains, plain launch, RILC-BFGS",
+            "atoms": inf.n_atoms, "torsions": inf.n_torsions, "chains": 8192, "kernel_ms": round(st.kernel_ms, 2),
+            "wall_ms": round(1000 * wall, 2), "ligands_per_s": round(1.0 / wall, 3),
+            "evals_per_s": round(st.evals / (st.kernel_ms / 1000.0), 1), "best_score": round(float(e.min()), 4),
+            "chains_per_wave": st.chains_per_wave}
+
+
+def small_scene_leg(vdock, log):
+    """The CPU-runnable case on the device: 2,000-atom receptor (36 A cube, 8 A cavity), 20 A box
+    (55^3 points), one 20-heavy-atom ligand with 5 torsions, 512 chains, the heuristics' depth and
+    steps, seed 42; maps and docking timed separately."""
+    from vdock import workloads as W
+    log("small scene: device")
+    ctx = vdock.Context(0)
+    try:
+        xyz, xs, box = W.small_scene(vdock, ctx)
+        g_ms = float(np.median([ctx.grid_populate(box) for _ in range(3)]))
+        text = vdock.synth_ligand(20, 5, 1)
+        lid = ctx.add_ligand(text)
+        p = vdock.default_params(chains=512, seed=42)
+        ctx.dock(lid, p)
+        t0 = time.perf_counter()
+        e, _, _, st = ctx.dock(lid, p)
+        wall = time.perf_counter() - t0
+        out = {"workload": "2000-atom receptor (36 A cube, 8 A cavity), 20 A box (55^3 points), one ligand of 20 "
+                           "heavy atoms and 5 torsions, 512 chains, RILC-BFGS, seed 42",
+               "grid_kernel_ms": round(g_ms, 4), "dock_kernel_ms": round(st.kernel_ms, 3),
+               "dock_wall_ms": round(1000 * wall, 3), "best_score": round(float(e.min()), 4), "evals": int(st.evals)}
+    finally:
+        ctx.close()
+    return out, dict(xyz=xyz, xs=xs, box=box, text=text, lid=lid)
+
+
+def oracle_leg(vdock, texts, box, xyz, xs, chains, adjoint_sample, post_confs, small, threads, host_info,
+               budget_s=15.0):
+    """Everything that runs the FP64 oracle: the CPU baselines (the oracle's Monte Carlo, same
+    schedule and random stream, on the screening ligands and on the CPU-runnable case; its
+    refinement of the docked poses) and, as the checker, the adjoint workload's error against it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
-    threads = max(1, min(16, os.cpu_count() or 1))
+    from concurrent.futures import ThreadPoolExecutor
+    out = {}
     lut = orc.Lut()
     b, e, n = box.as_tuple()
     t0 = time.time()
@@ -186,6 +329,7 @@ def run_cpu_baseline(vdock, texts, box, xyz, xs, chains, budget_s=15.0):
     grid.set_receptor(xyz.astype(np.float64), xs)
     grid.populate(True)
     t_grid = time.time() - t0
+    # screening ligands: the headline's CPU baseline
     done, t_mc, evals = 0, 0.0, 0
     for k, text in enumerate(texts):
         lig = orc.Ligand(text)
@@ -198,63 +342,72 @@ def run_cpu_baseline(vdock, texts, box, xyz, xs, chains, budget_s=15.0):
         done += 1
         if t_mc >= budget_s:
             break
-    return dict(value=done / t_mc, unit="ligands/s", cores=threads, kind="port", cpu_model=cpu_model(),
-                sample=f"first {done} of the screening ligands, {chains} chains each, FP64 oracle MC "
-                       f"(same schedule and heuristics) on {threads} host threads; maps built once in {t_grid:.1f} s",
-                evals_per_s=evals / t_mc)
-
-
-def post_leg(vdock, ctx, log, texts, box, xyz, xs, max_poses=20, cpu_budget_s=10.0):
-    """The step after the search for every ligand of the last docking batch: each ligand's chains
-    reduced to at most max_poses poses (add_to_output_container), every pose refined against the
-    receptor atoms in one FP64 launch, final scores and mode selection; timed on its second call
-    (the first, reported as first_call_wall_ms, also builds the per-box set-up).  Beside it, the oracle's
-    FP64 refinement (the reference runs this step on the CPU) of a bounded sample of the same
-    poses on host threads."""
-    log("post: refinement + final scores of the docked batch")
-    # the first call per receptor and box also builds the receptor cell lists of the direct
-    # evaluation and the pinned staging buffers (once, like the maps); the timed call follows it
-    t0 = time.perf_counter()
-    ctx.postprocess_docked(max_poses)
-    first = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    st = ctx.postprocess_docked(max_poses)
-    wall = time.perf_counter() - t0
-    n_lig = len(ctx._post_ids)
-    modes = [ctx.post_results(k)[0] for k in range(n_lig)]
-    best = [m[0].e for m in modes if m]
-    out = {"ligands": n_lig, "poses": st.blocks, "kernel_ms": round(st.kernel_ms, 2), "wall_ms": round(1000 * wall, 2),
-           "first_call_wall_ms": round(1000 * first, 2),
-           "poses_per_s_kernel": round(st.blocks / (st.kernel_ms / 1000.0), 1),
-           "ligands_per_s_wall": round(n_lig / wall, 2), "evals": int(st.evals),
-           "evals_per_s_kernel": round(st.evals / (st.kernel_ms / 1000.0), 1),
-           "mean_best_final_score": round(float(np.mean(best)), 4), "dtype": "f64"}
-    # CPU: the oracle's refine_structure on the same poses, ctypes calls on host threads
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as orc
-    from concurrent.futures import ThreadPoolExecutor
-    b, e, n = box.as_tuple()
-    grid = orc.Grid(orc.Lut(), b, e, n)
-    grid.set_receptor(xyz.astype(np.float64), xs)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    out["cpu_baseline"] = dict(
+        value=done / t_mc, unit="ligands/s", cores=threads, kind="port", cpu_model=cpu_model(), host=host_info,
+        sample=f"first {done} of the screening ligands, {chains} chains each, FP64 oracle MC (same schedule and "
+               f"heuristics) on {threads} host threads (the cores this process may use); maps built once in {t_grid:.1f} s",
+        evals_per_s=evals / t_mc)
+    # the CPU-runnable case, as specified: 512 chains, the heuristics' depth and steps, seed 42
+    sb, se, sn = small["box"].as_tuple()
+    t0 = time.time()
+    sg = orc.Grid(lut, sb, se, sn)
+    sg.set_receptor(small["xyz"].astype(np.float64), small["xs"])
+    sg.populate(True)
+    t_sgrid = time.time() - t0
+    lig = orc.Ligand(small["text"])
+    d, bf = vdock.heuristics(lig.n_atoms, lig.T, True)
+    t0 = time.time()
+    eo, _, _, ev = orc.mc_run(sg, lig, orc.mc_params(512, d, bf, sb, se, seed=42, stream=small["lid"], threads=threads))
+    t_small = time.time() - t0
+    # the CPU code's own defaults (dense BFGS, monte_carlo.h:71 and ssd.h:56: 2,500 MC steps, 300 BFGS
+    # steps) on a sample: one chain per thread for a tenth of the steps, time scaled to 512 x 2,500
+    k = threads
+    t0 = time.time()
+    orc.mc_run(sg, lig, orc.mc_params(k, 250, 300, sb, se, seed=42, stream=small["lid"], threads=threads, mode=0))
+    t_def = (time.time() - t0) * (512 / k) * 10
+    out["small_scene_cpu"] = {"grid_s": round(t_sgrid, 3), "matched_s": round(t_small, 3),
+                              "matched_steps": [d, bf], "matched_best_score": round(float(eo.min()), 4),
+                              "matched_evals": int(ev), "defaults_s_for_512_chains": round(t_def, 2),
+                              "defaults_sample": f"{k} chains x 250 MC steps, dense BFGS of 300 steps, time scaled "
+                                                 f"to 512 chains x 2500 steps",
+                              "cores": threads, "kind": "port"}
+    # the refinement after the search: the oracle's refine_structure on the docked poses
     jobs = []
-    for k, lid in enumerate(ctx._post_ids[:64]):
-        lig = orc.Ligand(texts[k])
-        _, conf, _ = ctx.post_results(k)
-        for c in conf:
-            jobs.append((lig, c.astype(np.float64)))
-    done, t_cpu = 0, 0.0
+    for kk, confs in enumerate(post_confs):
+        lg = orc.Ligand(texts[kk])
+        for c in confs:
+            jobs.append((lg, c.astype(np.float64)))
+    full = orc.Grid(lut, b, e, n)
+    full.set_receptor(xyz.astype(np.float64), xs)
+    done_r, t_cpu = 0, 0.0
     with ThreadPoolExecutor(threads) as pool:
         t1 = time.perf_counter()
         for chunk in range(0, len(jobs), threads):
-            list(pool.map(lambda j: orc.refine(grid, j[0], j[1], orc.refine_steps(j[0])), jobs[chunk:chunk + threads]))
-            done = min(len(jobs), chunk + threads)
-            if time.perf_counter() - t1 >= cpu_budget_s:
+            list(pool.map(lambda j: orc.refine(full, j[0], j[1], orc.refine_steps(j[0])), jobs[chunk:chunk + threads]))
+            done_r = min(len(jobs), chunk + threads)
+            if time.perf_counter() - t1 >= 10.0:
                 break
         t_cpu = time.perf_counter() - t1
-    out["cpu_refine"] = {"poses_per_s": round(done / t_cpu, 2), "cores": threads, "kind": "port",
-                         "sample": f"{done} refined poses of the first ligands, FP64 oracle refine_structure"}
-    out["speedup_vs_cpu_poses_per_s"] = round(out["poses_per_s_kernel"] / out["cpu_refine"]["poses_per_s"], 1)
+    out["cpu_refine"] = {"poses_per_s": round(done_r / t_cpu, 2), "cores": threads, "kind": "port",
+                         "sample": f"{done_r} refined poses of the first ligands, FP64 oracle refine_structure"}
+    # the adjoint workload's error: the oracle on the device's maps (the evaluation alone) and on
+    # its own FP64 maps (end to end)
+    shared = orc.Grid(lut, b, e, n)
+    shared.set_maps(adjoint_sample["maps"].astype(np.float64))
+    alig = orc.Ligand(adjoint_sample["text"])
+    worst = {"shared_maps": [0.0, 0.0], "fp64_maps": [0.0, 0.0]}
+    for c, ed, gd in zip(adjoint_sample["conf"], adjoint_sample["e"], adjoint_sample["g"]):
+        for key, gr in (("shared_maps", shared), ("fp64_maps", grid)):
+            er, gref, _, _ = orc.eval_deriv(gr, alig, c.astype(np.float64))
+            worst[key][0] = max(worst[key][0], abs(float(ed) - er) / max(1.0, abs(er)))
+            worst[key][1] = max(worst[key][1], float(np.abs(gd - gref).max()) / max(1.0, float(np.abs(gref).max())))
+    out["adjoint_error"] = {"sample": len(adjoint_sample["conf"]),
+                            "max_rel_energy_error_shared_maps": worst["shared_maps"][0],
+                            "max_rel_gradient_error_shared_maps": worst["shared_maps"][1],
+                            "max_rel_energy_error_fp64_oracle": worst["fp64_maps"][0],
+                            "max_rel_gradient_error_fp64_oracle": worst["fp64_maps"][1],
+                            "note": "relative to max(1, |e|) and max(1, max |g|); most conformations lie partly "
+                                    "outside the box, where the penalty slope 1e6 dominates"}
     return out
 
 
@@ -265,10 +418,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--ligands", type=int, default=1024)
     ap.add_argument("--chains", type=int, default=256)
-    ap.add_argument("--plain-subset", type=int, default=0, help="0 = all ligands")
     ap.add_argument("--autotune", action="store_true",
                     help="time each size bucket's launch shape (vd_set_autotune) instead of the fixed table")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip every leg that runs the oracle")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-large-box", action="store_true", help="skip the large-box QuickVina2 leg")
     ap.add_argument("--no-post", action="store_true", help="skip the refinement + final-score leg")
@@ -286,11 +438,10 @@ def main():
     ctx = vdock.Context(0)
     ctx.set_autotune(args.autotune)
     if args.grid_only:
+        from vdock import workloads as W
         xyz, xs, box, _ = build_scene(vdock, ctx)
         ms = [ctx.grid_populate(box) for _ in range(5)]
-        x2, s2 = vdock.synth_receptor(20000, 90.0, 25.0, 4242)
-        ctx.set_receptor(x2, s2)
-        b2 = vdock.box_from_center([0, 0, 0], [70.0, 70.0, 70.0], 0.375)
+        _, _, b2 = W.large_box_scene(vdock, ctx)
         ms2 = [ctx.grid_populate(b2) for _ in range(3)]
         print(json.dumps({"grid_only": True, "screening_ms": ms, "large_box_ms": ms2}), flush=True)
         return
@@ -304,8 +455,7 @@ def main():
     for w in range(args.warmup):
         log(f"warmup {w}")
         ctx.dock_batch(ids, params)
-    times, kernel_ms, evals, launches, per_step = [], 0.0, 0, [], []
-    st = None
+    times, evals, launches, per_step = [], 0, [], []
     for s in range(args.steps):
         log(f"step {s}")
         t0 = time.perf_counter()
@@ -319,7 +469,7 @@ def main():
     wall = sum(times)
     value = args.ligands * args.steps / wall
     best = ctx.batch_best()
-    post = None if args.no_post else post_leg(vdock, ctx, log, texts, box, xyz, xs)
+    post, post_confs = (None, []) if args.no_post else post_leg(ctx, log)
     out = {
         "metric": "ligands/sec on 1 MI355X, batched vs per-ligand at equal score acc; grid GB/s",
         "value": round(value, 3),
@@ -349,15 +499,27 @@ def main():
     per_step_bytes = sum(x["bytes"] for x in spans) / len(spans)
     avg_span = sum(x["span"] for x in spans) / len(spans)
     achieved = per_step_bytes / (avg_span / 1000.0) / 1e9
-    # traffic of the step: each instantiation's measured bytes per algorithmic byte times its bytes
+    # HBM traffic of the step: each instantiation's bytes per algorithmic byte (committed counter
+    # profile of this workload) times its algorithmic bytes in this run -- derived, not counted here
     tr = [traffic_from_profile(k, v["bytes"] / args.steps) for k, v in by_kernel.items()]
     traffic = round(sum(tr)) if all(t is not None for t in tr) else None
     copy_gbs = copy_bandwidth(ctx)
+    valu = {k: profile_figures(k) for k in by_kernel}
     out["roofline"] = {"kernel": "vd_mc_kernel (the step's %d concurrent launches: %s)" % (
                            len(per_step[0]), ", ".join(r.kernel for r in per_step[0])),
                        "bound": "hbm", "achieved": round(achieved, 2),
                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                       "traffic": traffic, "measured_copy_GBs": round(copy_gbs, 1),
+                       "traffic": traffic,
+                       "achieved_is": "algorithmic bytes (8 map corners per heavy atom + 1 pair-table quad per "
+                                      "pair, per evaluation) per second of the step's MC span; most are served "
+                                      "from L2, so this is not HBM traffic",
+                       "traffic_source": "derived: this run's algorithmic bytes x each instantiation's bytes per "
+                                         "algorithmic byte from the committed FETCH_SIZE/WRITE_SIZE profile",
+                       "hbm_traffic_GBs": round(traffic / (avg_span / 1000.0) / 1e9, 1) if traffic else None,
+                       "hbm_traffic_frac": round(traffic / (avg_span / 1000.0) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+                       "binding_resource": "VALU issue and memory latency (no dense contraction, so no MFMA bound)",
+                       "valu_busy_by_instantiation_profiled": valu,
+                       "measured_copy_GBs": round(copy_gbs, 1),
                        "frac_of_measured_copy": round(achieved / copy_gbs, 5),
                        "steps": len(spans), "avg_span_ms": round(avg_span, 3),
                        "both_running_share_of_span": round(sum(x["overlap"] for x in spans) / len(spans), 4),
@@ -366,45 +528,44 @@ def main():
                        "avg_launch_ms_by_instantiation": ms_by_kernel, "largest_share_instantiation": name,
                        "algorithmic_bytes_by_instantiation": {k: round(v["bytes"] / v["n"]) for k, v in by_kernel.items()},
                        "launches_by_instantiation": {k: v["n"] for k, v in by_kernel.items()}}
-
     if post is not None:
         out["post"] = post
+
+    adjoint_sample = None
+    small = None
     if not args.no_extras:
         # per-ligand plain launch loop (one kernel_monte-style launch per ligand) on the same
-        # ligands and parameters as the headline step; equal accuracy = same best scores
-        if args.plain_subset and args.plain_subset < len(ids):
-            sub = ids[:: max(1, len(ids) // args.plain_subset)][: args.plain_subset]
-            t0 = time.perf_counter()
-            st_b = ctx.dock_batch(sub, params)
-            t_batch = time.perf_counter() - t0
-            best_batch = ctx.batch_best()
-            batch_kernel = st_b.kernel_ms
-        else:
-            sub, t_batch, best_batch, batch_kernel = ids, wall / args.steps, best, kernel_ms / args.steps
-        log(f"plain loop over {len(sub)} ligands")
+        # ligands and parameters as the headline step
+        log(f"plain loop over {len(ids)} ligands")
         t0 = time.perf_counter()
-        st_p = ctx.dock_batch(sub, params, plain=True)
+        st_p = ctx.dock_batch(ids, params, plain=True)
         t_plain = time.perf_counter() - t0
         best_plain = ctx.batch_best()
         out["per_ligand_plain"] = {
-            "ligands": len(sub), "plain_ligands_per_s": round(len(sub) / t_plain, 3),
-            "batched_ligands_per_s": round(len(sub) / t_batch, 3), "speedup": round(t_plain / t_batch, 2),
+            "ligands": len(ids), "plain_ligands_per_s": round(len(ids) / t_plain, 3),
+            "batched_ligands_per_s": round(value, 3), "speedup": round(t_plain / (wall / args.steps), 2),
             "mean_best_score_plain": round(float(np.mean(best_plain)), 4),
-            "mean_best_score_batched": round(float(np.mean(best_batch)), 4),
-            "mean_abs_best_score_diff": round(float(np.mean(np.abs(best_plain - best_batch))), 4),
-            "plain_kernel_ms": round(st_p.kernel_ms, 2), "batched_kernel_ms": round(batch_kernel, 2)}
+            "mean_best_score_batched": round(float(np.mean(best)), 4),
+            "mean_abs_best_score_diff": round(float(np.mean(np.abs(best_plain - best))), 4),
+            "plain_kernel_ms": round(st_p.kernel_ms, 2), "batched_kernel_ms": round(kernel_ms / args.steps, 2),
+            "accuracy_over_seeds": plain_accuracy(vdock, ctx, log, ids, args.chains)}
         # grid precompute
         ctx.set_receptor(xyz, xs)
         gms = [ctx.grid_populate(box) for _ in range(5)]
         g_ms = float(np.median(gms))
         npts = np.prod(box.dims)
         gbytes = 17 * npts * 4 + 16 * 8000 + 153 * 2051 * 4
+        terms = grid_terms(xyz, xs, box)
         out["grid"] = {"points": int(npts), "kernel_ms": round(g_ms, 4), "algorithmic_bytes": int(gbytes),
                        "GB_per_s": round(gbytes / (g_ms / 1000.0) / 1e9, 2),
-                       "frac_hbm_peak": round(gbytes / (g_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 5)}
+                       "frac_hbm_peak": round(gbytes / (g_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 5),
+                       "terms": terms, "terms_per_s": round(terms / (g_ms / 1000.0), 1),
+                       "terms_note": "(grid point, typed receptor atom) pairs within 8 A; each reads one 17-value "
+                                     "table row and adds it to the point's 17 sums"}
         # adjoint microbenchmark: 2^20 random conformations of a 40-atom, 12-torsion ligand,
         # generated and kept in HBM; forward energy + adjoint gradient per conformation
-        lid = ctx.add_ligand(vdock.synth_ligand(40, 12, 777))
+        atext = vdock.synth_ligand(40, 12, 777)
+        lid = ctx.add_ligand(atext)
         inf = ctx.ligand_info(lid)
         T = inf.n_torsions
         n = 1 << 20
@@ -420,16 +581,30 @@ def main():
             ms = [ctx.eval_device(lid, d_conf, d_e, d_g, n, chains_per_wave=cpw) for _ in range(3)]
             rates[cpw] = float(np.median(ms))
         best_cpw = min(rates, key=rates.get)
+        ctx.eval_device(lid, d_conf, d_e, d_g, n, chains_per_wave=best_cpw)
         e_host = ctx.to_host(d_e, (n,))
+        sample = np.random.default_rng(5).choice(n, 256, replace=False)
+        conf_host = ctx.to_host(d_conf, (n, 7 + T))[sample]
+        g_host = ctx.to_host(d_g, (n, 6 + T))[sample]
         for ptr in (d_conf, d_e, d_g):
             ctx.free(ptr)
+        adjoint_sample = dict(text=atext, maps=ctx.grid_download(), conf=conf_host, e=e_host[sample], g=g_host)
         out["adjoint"] = {"conformations": n, "atoms": inf.n_atoms, "torsions": T, "pairs": inf.n_pairs,
                           "chains_per_wave": best_cpw, "kernel_ms": round(rates[best_cpw], 3),
                           "evals_per_s": round(n / (rates[best_cpw] / 1000.0), 1),
                           "ms_by_chains_per_wave": {str(k): round(v, 3) for k, v in rates.items()},
                           "finite_fraction": float(np.isfinite(e_host).mean())}
-
-    if not args.no_extras:
+        out["single_ligand"] = single_ligand_leg(vdock, ctx, log)
+        # the screening step with dense BFGS (--rilc_bfgs 0) and its own heuristics
+        log("dense BFGS")
+        pdn = vdock.default_params(chains=args.chains, seed=7, local_search=vdock.VD_LS_BFGS)
+        ctx.dock_batch(ids, pdn)
+        t0 = time.perf_counter()
+        st_d = ctx.dock_batch(ids, pdn)
+        t_d = time.perf_counter() - t0
+        out["dense_bfgs"] = {"ligands_per_s": round(len(ids) / t_d, 3), "kernel_ms": round(st_d.kernel_ms, 2),
+                             "evals": int(st_d.evals), "mean_best_score": round(float(np.mean(ctx.batch_best())), 4),
+                             "launches": [r.kernel for r in ctx.last_launches()]}
         # the QuickVina-W policy on the same ligands and schedule (global + per-chain buffers)
         log("QuickVina-W policy")
         pq = vdock.default_params(chains=args.chains, seed=7, policy=vdock.VD_MC_QVINAW)
@@ -441,9 +616,9 @@ def main():
         out["quickvina_w"] = {"ligands_per_s": round(len(ids) / t_q, 3), "kernel_ms": round(st_q.kernel_ms, 2),
                               "evals": int(st_q.evals), "evals_vs_vina_policy": round(st_q.evals / (evals / args.steps), 4),
                               "mean_best_score": round(float(np.mean(best_q)), 4)}
-
-    if not args.no_extras and not args.no_large_box:
-        out["large_box"] = large_box_leg(vdock, log)
+        out["small_scene"], small = small_scene_leg(vdock, log)
+        if not args.no_large_box:
+            out["large_box"] = large_box_leg(vdock, log)
 
     if args.copy_calibration:
         n = 256 << 20
@@ -452,8 +627,25 @@ def main():
             ctx.copy_device(dst, src, n)
         ctx.free(src); ctx.free(dst)
     if not args.no_cpu:
-        log("cpu baseline")
-        out["cpu_baseline"] = run_cpu_baseline(vdock, texts, box, xyz, xs, args.chains)
+        log("cpu baseline and oracle checks")
+        threads, host_info = host_threads()
+        if small is None:
+            _, small = small_scene_leg(vdock, log)
+        if adjoint_sample is None:
+            adjoint_sample = dict(text=vdock.synth_ligand(40, 12, 777), maps=ctx.grid_download(),
+                                  conf=np.zeros((0, 19)), e=np.zeros(0), g=np.zeros((0, 18)))
+        orc_out = oracle_leg(vdock, texts, box, xyz, xs, args.chains, adjoint_sample, post_confs, small, threads, host_info)
+        out["cpu_baseline"] = orc_out["cpu_baseline"]
+        if "small_scene" in out:
+            out["small_scene"]["cpu"] = orc_out["small_scene_cpu"]
+            out["small_scene"]["speedup_vs_cpu_matched"] = round(
+                orc_out["small_scene_cpu"]["matched_s"] * 1000.0 / out["small_scene"]["dock_wall_ms"], 1)
+        if "post" in out:
+            out["post"]["cpu_refine"] = orc_out["cpu_refine"]
+            out["post"]["speedup_vs_cpu_poses_per_s"] = round(out["post"]["poses_per_s_kernel"] /
+                                                              orc_out["cpu_refine"]["poses_per_s"], 1)
+        if "adjoint" in out:
+            out["adjoint"]["error_vs_oracle"] = orc_out["adjoint_error"]
     print(json.dumps(out), flush=True)
 
 

@@ -183,9 +183,23 @@ int vd_set_class_overlap(vd_ctx* ctx, int on);
 int vd_set_autotune(vd_ctx* ctx, int on);
 
 /* Plain per-ligand launch (replaces kernel_monte, V11/inc/cuda/kernel2.cuh:25-38, defined at kernel2.cu:567).
+ * init_conf: each chain's starting conformation [chains][7+T] (position, quaternion w,x,y,z,
+ * torsions; host pointer), the role of kernel_monte's ric (drawn by the reference host at
+ * V11/lib/main_procedure_cu.cpp:584-635), or NULL to draw them from the chain's random stream.
  * Per chain results: e[chains], conf[chains][7+T], heavy[chains][n_heavy][3] (host pointers,
  * any may be NULL).  The box is the one of the resident maps. */
-int vd_dock(vd_ctx* ctx, int id, const vd_dock_params* p, float* e, float* conf, float* heavy, vd_stats* st);
+int vd_dock(vd_ctx* ctx, int id, const vd_dock_params* p, const float* init_conf, float* e, float* conf,
+            float* heavy, vd_stats* st);
+
+/* One local search from each of n given conformations conf_in[n][7+T] of ligand id, with no
+ * Monte Carlo steps: the search the MC kernel runs after every mutation (p->local_search:
+ * RILC-BFGS with the Lewis-Overton line search, V11/inc/cuda/quasi_newton.cuh:1710-1806,
+ * 2172-2389, or dense BFGS, quasi_newton.cuh:2391-2499 / V11/lib/bfgs.h:30-140), p->bfgs_steps
+ * iterations (0 = heuristic), p's curl caps and slope; the better of the start and the search's
+ * end point, as the reference keeps it.  Outputs conf_out[n][7+T] and e[n] (host pointers, either
+ * may be NULL).  Overwrites the results of the last batch. */
+int vd_local_search(vd_ctx* ctx, int id, int n, const float* conf_in, const vd_dock_params* p, float* conf_out,
+                    float* e, vd_stats* st);
 
 /* Many ligands of uneven size in one persistent launch fed by a device work queue.
  * Results stay on the device; fetch them per ligand with vd_batch_results. */

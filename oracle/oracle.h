@@ -88,6 +88,9 @@ double orc_local_search(void* grid, void* lig, double* conf, int max_steps, int 
 /* the same optimisers on closed-form test functions over R^n (kind 0 = quadratic with
  * diagonal curvature coef[0..n), 1 = Rosenbrock); x updated in place, returns f */
 double orc_opt_test(int kind, int n, double* x, const double* coef, int max_steps, int mode, int* n_evals);
+/* the same, recording the first max_pts evaluated points pts[max_pts][n] and their values fs[max_pts] */
+double orc_opt_trace(int kind, int n, double* x, const double* coef, int max_steps, int mode, int max_pts,
+                     double* pts, double* fs, int* n_evals);
 
 /* ---- QuickVina2 visited history (QV2/inc/cuda/visited.cuh:7-226, intended semantics) ----
  * Feeds m points of a ligand with T torsions into one fresh history in order: x[m][7+T]
@@ -115,6 +118,16 @@ typedef struct {
 /* per chain: e[c], conf[c*(7+T)], heavy coords[c*3*H]; returns total evals through *evals */
 int orc_mc_run(void* grid, void* lig, const orc_mc_params* p, double* e, double* conf,
                double* heavy_coords, long long* evals);
+/* the same from given starting conformations starts[chains][7+T] (quaternions normalised) */
+int orc_mc_run_from(void* grid, void* lig, const orc_mc_params* p, const double* starts, double* e, double* conf,
+                    double* heavy_coords, long long* evals);
+/* One chain of orc_mc_run (policy 0 or 1), run alone, recording per MC step s < search_depth: the
+ * entity mutated which[s] (0 position, 1 orientation, 2+k torsion k), the Metropolis draw u[s], the
+ * current energy before the step e_prev[s], the candidate's energy after its local search
+ * e_cand[s], the decision accept[s], the gyration radius the orientation step used rg[s], and the
+ * conformations before[s][7+T] (current) and mutated[s][7+T] (candidate before its search). */
+int orc_mc_trace(void* grid, void* lig, const orc_mc_params* p, int chain, int* which, double* u, double* e_prev,
+                 double* e_cand, int* accept, double* rg, double* before, double* mutated);
 /* initial conformation of chain c from the shared random stream */
 int orc_initial_conf(void* lig, const orc_mc_params* p, int chain, double* conf);
 

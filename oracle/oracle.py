@@ -10,7 +10,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# ORACLE_LIBRARY selects another build of the oracle (its sanitizer build)
+LIB_PATH = os.environ.get("ORACLE_LIBRARY") or os.path.join(HERE, "_build", "liboracle.so")
 
 # Weights of the five Vina terms (V11/main/main.cpp:649-654)
 VINA_WEIGHTS = (-0.035579, -0.005156, 0.840245, -0.035069, -0.587439)
@@ -61,7 +62,10 @@ def _load():
         "orc_conf_increment": (C.c_int, [vp, dp, dp, C.c_double]),
         "orc_local_search": (C.c_double, [vp, vp, dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, ip]),
         "orc_opt_test": (C.c_double, [C.c_int, C.c_int, dp, dp, C.c_int, C.c_int, ip]),
+        "orc_opt_trace": (C.c_double, [C.c_int, C.c_int, dp, dp, C.c_int, C.c_int, C.c_int, dp, dp, ip]),
+        "orc_mc_trace": (C.c_int, [vp, vp, vp, C.c_int, ip, dp, dp, dp, ip, dp, dp, dp]),
         "orc_mc_run": (C.c_int, [vp, vp, vp, dp, dp, dp, C.POINTER(C.c_longlong)]),
+        "orc_mc_run_from": (C.c_int, [vp, vp, vp, dp, dp, dp, dp, C.POINTER(C.c_longlong)]),
         "orc_initial_conf": (C.c_int, [vp, vp, C.c_int, dp]),
         "orc_cluster": (C.c_int, [C.c_int, C.c_int, dp, dp, C.c_double, C.c_int, ip]),
         "orc_visited_feed": (C.c_int, [C.c_int, C.c_int, dp, dp, dp, C.c_int, ip]),
@@ -330,6 +334,18 @@ def opt_test(kind, x0, coef=None, max_steps=100, mode=1):
     return f, x, ne.value
 
 
+def opt_trace(kind, x0, coef=None, max_steps=100, mode=1, max_pts=256):
+    """opt_test, also returning every evaluated point (rows) and its value, in evaluation order."""
+    x = np.array(x0, dtype=np.float64)
+    cf = np.ascontiguousarray(coef if coef is not None else np.ones(len(x)), dtype=np.float64)
+    pts = np.zeros((max_pts, len(x)))
+    fs = np.zeros(max_pts)
+    ne = C.c_int()
+    f = lib().orc_opt_trace(kind, len(x), _d(x), _d(cf), max_steps, mode, max_pts, _d(pts), _d(fs), C.byref(ne))
+    k = min(ne.value, max_pts)
+    return f, x, pts[:k], fs[:k]
+
+
 class MCParams(C.Structure):
     _fields_ = [
         ("chains", C.c_int), ("search_depth", C.c_int), ("bfgs_steps", C.c_int), ("mode", C.c_int),
@@ -354,16 +370,38 @@ def mc_params(chains, depth, bfgs_steps, box_lo, box_hi, seed=42, stream=0, mode
     return p
 
 
-def mc_run(grid, lig, params):
+def mc_run(grid, lig, params, starts=None):
+    """Monte Carlo of params.chains chains -> (e, conf, heavy, evaluations); starts[chains][7+T]
+    gives the starting conformations (else drawn from the shared random stream)."""
     n = params.chains
     e = np.zeros(n)
     conf = np.zeros((n, 7 + lig.T))
     heavy = np.zeros((n, lig.n_heavy, 3))
     ev = C.c_longlong()
-    rc = lib().orc_mc_run(grid.h, lig.h, C.byref(params), _d(e), _d(conf), _d(heavy), C.byref(ev))
+    if starts is not None:
+        st = np.ascontiguousarray(starts, dtype=np.float64)
+        assert st.shape == (n, 7 + lig.T)
+        rc = lib().orc_mc_run_from(grid.h, lig.h, C.byref(params), _d(st), _d(e), _d(conf), _d(heavy), C.byref(ev))
+    else:
+        rc = lib().orc_mc_run(grid.h, lig.h, C.byref(params), _d(e), _d(conf), _d(heavy), C.byref(ev))
     if rc != 0:
         raise RuntimeError(_err())
     return e, conf, heavy, ev.value
+
+
+def mc_trace(grid, lig, params, chain):
+    """One chain's MC steps: dict of which, u, e_prev, e_cand, accept, rg (per step) and the
+    conformations before and after each mutation (rows of 7+T)."""
+    d, nc = params.search_depth, 7 + lig.T
+    which, accept = np.zeros(d, dtype=np.int32), np.zeros(d, dtype=np.int32)
+    u, e_prev, e_cand, rg = (np.zeros(d) for _ in range(4))
+    before, mutated = np.zeros((d, nc)), np.zeros((d, nc))
+    rc = lib().orc_mc_trace(grid.h, lig.h, C.byref(params), chain, _i(which), _d(u), _d(e_prev), _d(e_cand),
+                            _i(accept), _d(rg), _d(before), _d(mutated))
+    if rc != 0:
+        raise RuntimeError(_err())
+    return dict(which=which, u=u, e_prev=e_prev, e_cand=e_cand, accept=accept.astype(bool), rg=rg, before=before,
+                mutated=mutated)
 
 
 def initial_conf(lig, params, chain):

@@ -1197,9 +1197,21 @@ fl gyration_radius(const Ligand& L, const std::vector<V3>& coords, const V3& ori
 struct McState {
   Conf tmp, best; fl rg = 0, best_e = 0; std::vector<V3> best_heavy; Visited vis;
 };
-void mc_init(EvalCtx& ctx, const orc_mc_params& p, int chain, McState& st) {
+// What one MC step decided (orc_mc_trace): the entity mutated, the Metropolis draw, the
+// conformation before and after the mutation, the candidate's energy after its local search, the
+// current energy it was compared with, and the decision.
+struct StepTrace {
+  int which = -1, accept = 0; fl u = 0, e_prev = 0, e_cand = 0, rg = 0;
+  Conf before, mutated;
+};
+void mc_init(EvalCtx& ctx, const orc_mc_params& p, int chain, McState& st, const double* starts = nullptr) {
   const Ligand& L = *ctx.L;
   st.tmp = initial_conf(L, p, chain);
+  if (starts) {   // the caller's starting conformation (kernel_monte's ric, V11/inc/cuda/kernel2.cuh:27)
+    st.tmp = conf_from(L, starts + (size_t)chain * (7 + L.T()));
+    const fl n = std::sqrt(st.tmp.q.a * st.tmp.q.a + st.tmp.q.b * st.tmp.q.b + st.tmp.q.c * st.tmp.q.c + st.tmp.q.d * st.tmp.q.d);
+    st.tmp.q = Q{st.tmp.q.a / n, st.tmp.q.b / n, st.tmp.q.c / n, st.tmp.q.d / n};
+  }
   st.tmp.e = 0;
   Frames F; std::vector<V3> coords;
   set_conf(L, st.tmp, coords, F);
@@ -1207,7 +1219,8 @@ void mc_init(EvalCtx& ctx, const orc_mc_params& p, int chain, McState& st) {
   st.best_e = std::numeric_limits<fl>::infinity();
   if (p.policy == 2) st.vis.rounds = 5;
 }
-void mc_step(EvalCtx& ctx, const orc_mc_params& p, int chain, int step, McState& st, GlobalVisited* glob) {
+void mc_step(EvalCtx& ctx, const orc_mc_params& p, int chain, int step, McState& st, GlobalVisited* glob,
+             StepTrace* tr = nullptr) {
   const Ligand& L = *ctx.L;
   uint32_t key[2] = {p.seed, p.stream}, w[4];
   const int T = L.T();
@@ -1233,8 +1246,10 @@ void mc_step(EvalCtx& ctx, const orc_mc_params& p, int chain, int step, McState&
   } else {
     cand.tors[which - 2] = tor;
   }
+  if (tr) { tr->which = which; tr->u = u_metro; tr->before = st.tmp; tr->mutated = cand; tr->rg = st.rg; tr->e_prev = st.tmp.e; }
   search(cand);
   bool accept = (step == 0) || (cand.e < st.tmp.e) || (u_metro < std::exp((st.tmp.e - cand.e) / p.temperature));  // monte_carlo.cpp:35-39
+  if (tr) { tr->e_cand = cand.e; tr->accept = accept; }
   if (accept) {
     st.tmp = cand;
     Frames F; std::vector<V3> coords;
@@ -1444,8 +1459,14 @@ int orc_qvw_feed(int T, int m, const double* x, const double* f, const double* g
 int orc_qvw_levels(long long chains, long long depth) { return GlobalVisited::levels_for(chains, depth); }
 
 double orc_opt_test(int kind, int n, double* x, const double* coef, int max_steps, int mode, int* n_evals) {
+  return orc_opt_trace(kind, n, x, coef, max_steps, mode, 0, nullptr, nullptr, n_evals);
+}
+
+double orc_opt_trace(int kind, int n, double* x, const double* coef, int max_steps, int mode, int max_pts,
+                     double* pts, double* fs, int* n_evals) {
   int evals = 0;
   auto f = [&](const std::vector<fl>& p, std::vector<fl>& g) -> fl {
+    if (evals < max_pts) for (int i = 0; i < n; ++i) pts[(size_t)evals * n + i] = p[i];
     ++evals; g.assign(n, 0); fl v = 0;
     if (kind == 0) { for (int i = 0; i < n; ++i) { v += 0.5 * coef[i] * p[i] * p[i]; g[i] = coef[i] * p[i]; } }
     else {
@@ -1455,6 +1476,7 @@ double orc_opt_test(int kind, int n, double* x, const double* coef, int max_step
         g[i] += -400 * a * p[i] - 2 * b; g[i + 1] += 200 * a;
       }
     }
+    if (evals <= max_pts) fs[evals - 1] = v;
     return v;
   };
   std::vector<fl> p(x, x + n), g;
@@ -1470,6 +1492,11 @@ int orc_initial_conf(void* lp, const orc_mc_params* p, int chain, double* conf) 
 }
 
 int orc_mc_run(void* gp, void* lp, const orc_mc_params* p, double* e, double* conf, double* heavy_coords, long long* evals) {
+  return orc_mc_run_from(gp, lp, p, nullptr, e, conf, heavy_coords, evals);
+}
+
+int orc_mc_run_from(void* gp, void* lp, const orc_mc_params* p, const double* starts, double* e, double* conf,
+                    double* heavy_coords, long long* evals) {
   Ligand* L = (Ligand*)lp; Grid* G = (Grid*)gp;
   const int T = L->T(), H = L->heavy;
   std::atomic<int> next(0); std::atomic<long long> total(0);
@@ -1486,7 +1513,7 @@ int orc_mc_run(void* gp, void* lp, const orc_mc_params* p, double* e, double* co
     for (int t = 0; t < nth; ++t) pool.emplace_back([&]() {   // parallel_mc.cpp:65-77: one task per chain
       EvalCtx ctx{G, L, p->v_intra, p->v_inter, p->slope};
       for (int c = next++; c < p->chains; c = next++) {
-        mc_init(ctx, *p, c, st[c]);
+        mc_init(ctx, *p, c, st[c], starts);
         for (int s = 0; s < p->search_depth; ++s) mc_step(ctx, *p, c, s, st[c], nullptr);
         output(c);
       }
@@ -1505,7 +1532,7 @@ int orc_mc_run(void* gp, void* lp, const orc_mc_params* p, double* e, double* co
       for (int t = 0; t < nth; ++t) pool.emplace_back([&]() {
         EvalCtx ctx{G, L, p->v_intra, p->v_inter, p->slope};
         for (int c = next++; c < p->chains; c = next++) {
-          if (s < 0) mc_init(ctx, *p, c, st[c]);
+          if (s < 0) mc_init(ctx, *p, c, st[c], starts);
           else mc_step(ctx, *p, c, s, st[c], &glob);
         }
         total += ctx.evals;
@@ -1515,6 +1542,24 @@ int orc_mc_run(void* gp, void* lp, const orc_mc_params* p, double* e, double* co
     for (int c = 0; c < p->chains; ++c) output(c);
   }
   if (evals) *evals = total;
+  return 0;
+}
+
+int orc_mc_trace(void* gp, void* lp, const orc_mc_params* p, int chain, int* which, double* u, double* e_prev,
+                 double* e_cand, int* accept, double* rg, double* before, double* mutated) {
+  Ligand* L = (Ligand*)lp; Grid* G = (Grid*)gp;
+  if (p->policy == 2) return fail("orc_mc_trace: the QuickVina-W policy shares a buffer across chains");
+  const int nc = 7 + L->T();
+  EvalCtx ctx{G, L, p->v_intra, p->v_inter, p->slope};
+  McState st;
+  mc_init(ctx, *p, chain, st);
+  for (int s = 0; s < p->search_depth; ++s) {
+    StepTrace tr;
+    mc_step(ctx, *p, chain, s, st, nullptr, &tr);
+    which[s] = tr.which; u[s] = tr.u; e_prev[s] = tr.e_prev; e_cand[s] = tr.e_cand; accept[s] = tr.accept; rg[s] = tr.rg;
+    conf_to(tr.before, before + (size_t)s * nc);
+    conf_to(tr.mutated, mutated + (size_t)s * nc);
+  }
   return 0;
 }
 

@@ -218,3 +218,28 @@ def test_pair_chunks_cover_every_pair_once_without_conflicts(host_ctx, lanes):
         # (Euler orientation + equitable bipartite colouring) reaches that bound
         deg = np.bincount(np.concatenate([pairs[:, 1], pairs[:, 2]]), minlength=1)
         assert ch.shape[0] == max(-(-len(pairs) // lanes), -(-int(deg.max()) // 2))
+
+
+def test_cpp_consumer_of_the_header():
+    """vina-cuda_amd/examples/vdock_dock, a C++ program built against include/vdock.h and linked
+    with libvdock, reports bad options with status 2 and, on a host without a GPU, the missing
+    device with status 3 (its run on a GPU is test_gpu_example.py)."""
+    import subprocess
+    import vdock
+    exe = os.path.join(ROOT, "vina-cuda_amd", "examples", "vdock_dock")
+    assert os.path.exists(exe), "build with __graft_entry__.build()"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "missing --receptor" in r.stderr
+    args = [exe, "--receptor", os.path.join(ROOT, "tests", "golden", "1u1b_protein.pdbqt"), "--ligand_directory",
+            os.path.join(ROOT, "tests", "golden", "qvw_auto"), "--output_directory", "/nonexistent/out",
+            "--center_x", "0", "--center_y", "0", "--center_z", "0", "--size_x", "10", "--size_y", "10",
+            "--size_z", "10", "--thread", "999", "--seed", "1"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "thread must be at least 1000" in r.stderr
+    h = ctypes.c_void_p()
+    if vdock.load().vd_create(0, ctypes.byref(h)) == vdock.VD_OK:
+        vdock.load().vd_destroy(h)
+        pytest.skip("a GPU is visible here")
+    args[args.index("999")] = "8000"
+    r = subprocess.run(args, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 3 and "no usable GPU" in r.stderr

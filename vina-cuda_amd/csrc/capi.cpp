@@ -160,6 +160,7 @@ struct vd_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int n_cu = 0;
   int lds_max = 64 * 1024;                   // LDS one workgroup may take (queried from the device)
+  DevBuf<float> init;                        // caller's starting conformations (vd_dock, vd_local_search)
   std::map<int, std::pair<int, int>> tuned;  // size bucket -> (lanes per chain, waves per SIMD)
   bool autotune_on = false;                   // timing-based choice of the pair above (vd_set_autotune)
   DevBuf<float> tune_e, tune_conf, tune_heavy;
@@ -385,6 +386,21 @@ double elapsed_ms(vd_ctx* c) {
 // atom (8 x 4 B) and one interpolation quad of the pair table per intramolecular pair (16 B).
 double eval_bytes(const vd_lig_header& h) { return 32.0 * h.n_heavy + 16.0 * h.n_pairs; }
 
+// Kernel mode of a launch: 0 dense BFGS, 1 RILC-BFGS, 2 with the QuickVina2 history, 3 QuickVina-W.
+int mc_mode(const vd_mc_consts& P) {
+  return P.local_search == 0 ? 0 : (P.policy == VD_MC_QVINA2 ? 2 : P.policy == VD_MC_QVINAW ? 3 : 1);
+}
+
+// A tighter register budget only pays when the LDS of a launch lets the extra waves in: the
+// loosest budget (waves per SIMD) that reaches the residency W reaches.
+int loosest_budget(int G, int W, int mode, int lds) {
+  if (mode == 0) return 2;
+  const int occ = vdl::max_resident_mc_blocks(G, W, mode, lds);
+  for (int w = 2; w < W; ++w)
+    if (vdl::max_resident_mc_blocks(G, w, mode, lds) >= occ) return w;
+  return W;
+}
+
 // One MC launch in parts, so that the launches of different lanes-per-chain classes of a batch
 // can run at the same time on their own streams: mc_prepare does the host work and sizes the
 // slot's device buffers (allocation may synchronise the device, so every launch of a batch is
@@ -432,14 +448,8 @@ void mc_prepare(vd_ctx* c, McSlot& s, int G, int W, Packed&& pk, std::vector<uin
   if (lds > c->lds_max)
     throw ArgError(VD_ERR_CAPACITY, "ligand too large for the LDS layout (" + std::to_string(lds) + " bytes of " +
                                         std::to_string(c->lds_max) + " per workgroup)");
-  const int mode = P.local_search == 0 ? 0 : (P.policy == VD_MC_QVINA2 ? 2 : P.policy == VD_MC_QVINAW ? 3 : 1);
-  // a tighter register budget only pays when the LDS of this launch lets the extra waves in:
-  // take the loosest budget that reaches the same residency
-  if (mode != 0) {
-    const int occ = vdl::max_resident_mc_blocks(G, W, mode, lds);
-    for (int w = 2; w < W; ++w)
-      if (vdl::max_resident_mc_blocks(G, w, mode, lds) >= occ) { W = w; break; }
-  }
+  const int mode = mc_mode(P);
+  W = loosest_budget(G, W, mode, lds);
   int blocks = (int)s.h_items.size();
   if (persistent) {
     a.queue = s.queue.p;
@@ -524,10 +534,12 @@ void mc_collect(vd_ctx* c, McSlot& s, vd_stats* st, hipEvent_t base, bool add_ms
 // context's stream, and waits for it.
 void run_mc(vd_ctx* c, int G, int W, const Packed& pk, const std::vector<uint32_t>& streams, const std::vector<int2>& items,
             const std::vector<longlong4>& off, const vd_mc_consts& P, float* d_e, float* d_conf, float* d_heavy,
-            bool persistent, vd_stats* st, bool timed) {
+            bool persistent, vd_stats* st, bool timed, const float* d_init = nullptr, bool ls_only = false) {
   McSlot& s = c->mc[0];
   mc_prepare(c, s, G, W, Packed(pk), std::vector<uint32_t>(streams), std::vector<int2>(items),
              std::vector<longlong4>(off), P, d_e, d_conf, d_heavy, persistent, timed);
+  s.a.init_conf = d_init;
+  s.a.ls_only = ls_only ? 1 : 0;
   mc_enqueue_copies(s);
   mc_enqueue_kernel(s);
   mc_collect(c, s, st, nullptr);
@@ -1236,29 +1248,79 @@ int vd_dock_heuristics(int n_atoms, int n_torsions, int rilc, int* depth, int* b
   return VD_OK;
 }
 
-int vd_dock(vd_ctx* c, int id, const vd_dock_params* p, float* e, float* conf, float* heavy, vd_stats* st) {
+}  // extern "C"
+
+namespace {
+
+// Uploads n starting conformations [n][7+T] (finite; quaternions are normalised on the device).
+const float* upload_starts(vd_ctx* c, const float* conf, long long n, int T) {
+  const size_t m = (size_t)n * (7 + T);
+  for (size_t i = 0; i < m; ++i)
+    if (!std::isfinite(conf[i])) throw ArgError(VD_ERR_ARG, "non-finite starting conformation value");
+  for (long long k = 0; k < n; ++k) {
+    const float* q = conf + (size_t)k * (7 + T) + 3;
+    if (!(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3] > 1e-12f))
+      throw ArgError(VD_ERR_ARG, "zero orientation quaternion in a starting conformation");
+  }
+  c->init.ensure(m);
+  HIP_CHECK(hipMemcpyAsync(c->init.p, conf, m * 4, hipMemcpyHostToDevice, c->stream));
+  return c->init.p;
+}
+
+// One plain launch of ligand id's chains (vd_dock, vd_local_search): results in the batch buffers.
+void dock_one(vd_ctx* c, int id, const vd_dock_params* p, const float* init_conf, bool ls_only, vd_stats* st,
+              LigPlan& pl) {
+  check_params(p);
+  need_device(c);
+  if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no maps");
+  if (st) std::memset(st, 0, sizeof(*st));
+  if (c->autotune_on && (p->chains_per_wave == 0 || p->waves_per_simd == 0)) autotune(c, {id}, p);
+  c->launches.clear();
+  pl = plan_for(c, id, p, true);
+  if (ls_only) pl.depth = 0;
+  prepare_batch(c, {pl});
+  const float* d_init = init_conf ? upload_starts(c, init_conf, p->chains, pl.T) : nullptr;
+  std::vector<int> dv{pl.depth}, bv{pl.bfgs};
+  Packed pk = pack(c, pl.G, {id}, &dv, &bv);
+  const int cpw = 64 / pl.G;
+  std::vector<int2> items;
+  for (int c0 = 0; c0 < p->chains; c0 += cpw) items.push_back(make_int2(0, c0));
+  run_mc(c, pl.G, pl.W, pk, {(uint32_t)id}, items, c->batch.off, make_consts(c, p, pl.depth, pl.bfgs), c->batch.e.p,
+         c->batch.conf.p, c->batch.heavy.p, false, st, true, d_init, ls_only);
+}
+
+}  // namespace
+
+extern "C" {
+
+int vd_dock(vd_ctx* c, int id, const vd_dock_params* p, const float* init_conf, float* e, float* conf, float* heavy,
+            vd_stats* st) {
   if (!c) return VD_ERR_ARG;
   return guard(c, [&]() {
-    check_params(p);
-    need_device(c);
-    if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no maps");
-    if (st) std::memset(st, 0, sizeof(*st));
-    if (c->autotune_on && (p->chains_per_wave == 0 || p->waves_per_simd == 0)) autotune(c, {id}, p);
-    c->launches.clear();
-    LigPlan pl = plan_for(c, id, p, true);
-    prepare_batch(c, {pl});
-    std::vector<int> dv{pl.depth}, bv{pl.bfgs};
-    Packed pk = pack(c, pl.G, {id}, &dv, &bv);
-    const int cpw = 64 / pl.G;
-    std::vector<int2> items;
-    for (int c0 = 0; c0 < p->chains; c0 += cpw) items.push_back(make_int2(0, c0));
-    run_mc(c, pl.G, pl.W, pk, {(uint32_t)id}, items, c->batch.off, make_consts(c, p, pl.depth, pl.bfgs), c->batch.e.p,
-           c->batch.conf.p, c->batch.heavy.p, false, st, true);
+    LigPlan pl;
+    dock_one(c, id, p, init_conf, false, st, pl);
     if (e) HIP_CHECK(hipMemcpy(e, c->batch.e.p, (size_t)p->chains * 4, hipMemcpyDeviceToHost));
     if (conf) HIP_CHECK(hipMemcpy(conf, c->batch.conf.p, (size_t)p->chains * (7 + pl.T) * 4, hipMemcpyDeviceToHost));
     if (heavy) HIP_CHECK(hipMemcpy(heavy, c->batch.heavy.p, (size_t)p->chains * pl.H * 12, hipMemcpyDeviceToHost));
   });
 }
+
+int vd_local_search(vd_ctx* c, int id, int n, const float* conf_in, const vd_dock_params* p, float* conf_out, float* e,
+                    vd_stats* st) {
+  if (!c || n < 1 || !conf_in || !p) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    vd_dock_params q = *p;
+    q.chains = n;
+    LigPlan pl;
+    dock_one(c, id, &q, conf_in, true, st, pl);
+    if (e) HIP_CHECK(hipMemcpy(e, c->batch.e.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (conf_out) HIP_CHECK(hipMemcpy(conf_out, c->batch.conf.p, (size_t)n * (7 + pl.T) * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+}  // extern "C"
+
+extern "C" {
 
 static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, vd_stats* st, bool persistent) {
   if (!c || n < 0 || (n > 0 && !ids)) return VD_ERR_ARG;

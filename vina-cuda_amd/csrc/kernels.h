@@ -17,8 +17,11 @@ struct McArgs {
   const float4* blob;
   const vd_lig_header* headers;     // per ligand slot
   const uint32_t* streams;          // random stream id per ligand slot
-  const int2* items;                // (ligand slot, first chain)
+  const int2* items;                // (ligand slot, first chain); nullptr: item k = (0, k x chains per wave)
   int n_items;
+  const float* init_conf;           // nullptr: starting conformations drawn from the random stream; else
+                                    // per chain [7+T] at the slot's conformation offset (out_off.y)
+  int ls_only;                      // 1: one local search from each starting conformation, no MC steps
   int* queue;                       // nullptr: item = blockIdx.x
   vdk::EvalK K;
   vd_mc_consts P;

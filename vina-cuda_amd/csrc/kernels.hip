@@ -131,6 +131,28 @@ __device__ __forceinline__ void make_views(const float4* __restrict__ blob, cons
   H = reinterpret_cast<float*>(S.fx + G + 1);
 }
 
+// Writes a chain's best energy, conformation and heavy-atom coordinates (call after
+// set_coords of the best conformation).
+template <int G>
+__device__ void write_chain(const LigView& L, const Scratch& S, const Conf& best, float best_e, uint32_t chain, int l,
+                            float* out_e, float* out_conf, float* out_heavy) {
+  if (l == 0) {
+    out_e[chain] = best_e;
+    float* c = out_conf + (size_t)chain * (7 + L.T);
+    c[0] = best.px; c[1] = best.py; c[2] = best.pz; c[3] = best.q.w; c[4] = best.q.x; c[5] = best.q.y; c[6] = best.q.z;
+  }
+  if (l >= 1 && l <= L.T) out_conf[(size_t)chain * (7 + L.T) + 6 + l] = best.tor;
+  for (int i = l; i < L.A; i += G) {
+    const uint32_t meta = fbits(L.atom[i].w);
+    if ((meta >> 16) & 1u) {
+      const int slot = (meta >> 17) & 255;
+      float* o = out_heavy + ((size_t)chain * L.H + slot) * 3;
+      const float4 y = S.xyz[i];
+      o[0] = best.px + y.x; o[1] = best.py + y.y; o[2] = best.pz + y.z;
+    }
+  }
+}
+
 // Dense-BFGS local search of the groups with enabled == true (the RILC-BFGS modes run inside
 // mc_chain_flat's state machine instead).
 template <int G>
@@ -140,11 +162,20 @@ __device__ float local_search(const LigView& L, const Scratch& S, float* H, cons
   return dense_bfgs<G>(L, S, H, H + G * G, K, x, enabled, x_e, max_steps, l, evals);
 }
 
-// Initial conformation of a chain (conf::randomize, V11/lib/conf.h:127-130, 214-217;
-// random_orientation, V11/lib/quaternion.cpp:206-219) from the shared random stream.
+// Initial conformation of a chain: the caller's (the role of kernel_monte's ric,
+// V11/inc/cuda/kernel2.cuh:27; the reference host draws them at main_procedure_cu.cpp:584-635), or
+// conf::randomize (V11/lib/conf.h:127-130, 214-217; random_orientation, V11/lib/quaternion.cpp:206-219)
+// from the shared random stream.  `given` points at the chain's 7+T numbers or is null.
 template <int G>
-__device__ Conf initial_conf(const vd_mc_consts& P, uint32_t stream, uint32_t chain, int l, int T) {
+__device__ Conf initial_conf(const vd_mc_consts& P, uint32_t stream, uint32_t chain, int l, int T,
+                             const float* given = nullptr) {
   Conf c;
+  if (given) {
+    c.px = given[0]; c.py = given[1]; c.pz = given[2];
+    c.q = qnormalize(Quat{given[3], given[4], given[5], given[6]});
+    c.tor = (l >= 1 && l <= T) ? given[6 + l] : 0.0f;
+    return c;
+  }
   const U4 w = philox(chain, VD_INIT_STEP, 0, 0, P.seed, stream);
   c.px = P.box_lo[0] + (P.box_hi[0] - P.box_lo[0]) * u01(w.x);
   c.py = P.box_lo[1] + (P.box_hi[1] - P.box_lo[1]) * u01(w.y);
@@ -175,12 +206,19 @@ __device__ Conf initial_conf(const vd_mc_consts& P, uint32_t stream, uint32_t ch
 // mutate -> local search -> Metropolis(T) -> on accept, if better than the best so far: second
 // local search, keep the best.  Mutation as V11/lib/mutate.cpp:62-127 (see initial_conf for the
 // random stream); the orientation step uses the gyration radius of the current conformation.
+// ls_only: one local search from the starting conformation, no MC steps (vd_local_search).
 template <int G>
 __device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const EvalK& K, const vd_mc_consts& P,
                          int depth, int bfgs_steps, uint32_t stream, uint32_t chain, bool valid, int l, float* out_e,
-                         float* out_conf, float* out_heavy, int& evals) {
+                         float* out_conf, float* out_heavy, int& evals, const float* given, bool ls_only) {
   const uint32_t k0 = P.seed, k1 = stream;
-  Conf tmp = initial_conf<G>(P, stream, chain, l, L.T);
+  Conf tmp = initial_conf<G>(P, stream, chain, l, L.T, given);
+  if (ls_only) {
+    const float e = local_search<G>(L, S, H, K, tmp, valid, 0.0f, bfgs_steps, l, evals);
+    set_coords<G>(L, S, tmp, l);
+    if (valid) write_chain<G>(L, S, tmp, e, chain, l, out_e, out_conf, out_heavy);
+    return;
+  }
   float tmp_e = 0.0f;
   set_coords<G>(L, S, tmp, l);
   float rg = gyration_radius<G>(L, S, l);
@@ -236,28 +274,6 @@ __device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const Eva
   }
 }
 
-// Writes a chain's best energy, conformation and heavy-atom coordinates (call after
-// set_coords of the best conformation).
-template <int G>
-__device__ void write_chain(const LigView& L, const Scratch& S, const Conf& best, float best_e, uint32_t chain, int l,
-                            float* out_e, float* out_conf, float* out_heavy) {
-  if (l == 0) {
-    out_e[chain] = best_e;
-    float* c = out_conf + (size_t)chain * (7 + L.T);
-    c[0] = best.px; c[1] = best.py; c[2] = best.pz; c[3] = best.q.w; c[4] = best.q.x; c[5] = best.q.y; c[6] = best.q.z;
-  }
-  if (l >= 1 && l <= L.T) out_conf[(size_t)chain * (7 + L.T) + 6 + l] = best.tor;
-  for (int i = l; i < L.A; i += G) {
-    const uint32_t meta = fbits(L.atom[i].w);
-    if ((meta >> 16) & 1u) {
-      const int slot = (meta >> 17) & 255;
-      float* o = out_heavy + ((size_t)chain * L.H + slot) * 3;
-      const float4 y = S.xyz[i];
-      o[0] = best.px + y.x; o[1] = best.py + y.y; o[2] = best.pz + y.z;
-    }
-  }
-}
-
 // The MC schedule of mc_chain with RILC-BFGS and the Lewis-Overton line search
 // (V11/inc/cuda/quasi_newton.cuh:1710-1806, 2172-2389), run as a flat state machine: every pass evaluates one conformation per chain group and each group then advances
 // its own state (start of a local search, line-search trial, end of a BFGS iteration, end of a
@@ -272,20 +288,27 @@ __device__ void write_chain(const LigView& L, const Scratch& S, const Conf& best
 template <int G, int POL>
 __device__ void mc_chain_flat(const LigView& L, const Scratch& S, const EvalK& K, const vd_mc_consts& P, int depth,
                               int max_steps, uint32_t stream, uint32_t chain, bool valid, int l, float* out_e,
-                              float* out_conf, float* out_heavy, int& evals, History* hist, const GlobalHist* gh) {
+                              float* out_conf, float* out_heavy, int& evals, History* hist, const GlobalHist* gh,
+                              const float* given, bool ls_only) {
   const uint32_t k0 = P.seed, k1 = stream;
   const int n_ent = P.mutation_quirk ? (1 + L.T) : (2 + L.T);
   enum { kMutate = 0, kStart = 1, kTrial = 2, kDone = 3 };
   // MC state; the current conformation lives in the stash (kSlotTmp)
   {
-    const Conf c0 = initial_conf<G>(P, stream, chain, l, L.T);
+    const Conf c0 = initial_conf<G>(P, stream, chain, l, L.T, given);
     stash_put<G>(S.stash, kSlotTmp, c0);
     stash_put<G>(S.stash, kSlotBest, c0);
     set_coords<G>(L, S, c0, l);
   }
   float rg = gyration_radius<G>(L, S, l);
   float tmp_e = 0.0f, best_e = INFINITY, u_metro = 0.0f;
+  // ls_kind: 1 the search of a mutated candidate, 2 the second search of an accepted one,
+  // 0 a lone search from the starting conformation (ls_only)
   int step = 0, phase = (valid && depth > 0) ? kMutate : kDone, ls_kind = 1;
+  if (ls_only) {
+    phase = valid ? kStart : kDone;
+    ls_kind = 0;
+  }
   // local-search state
   Conf x = stash_get<G>(S.stash, kSlotTmp);
   float x_rg = rg, g = 0.0f, d = 0.0f, gp = 0.0f;
@@ -402,7 +425,9 @@ __device__ void mc_chain_flat(const LigView& L, const Scratch& S, const EvalK& K
     if (ls_done) {           // end of a local search: keep the better of its start and its result
       if (!(fx <= fx_orig)) { fx = fx_orig; x = stash_get<G>(S.stash, kSlotOrig); x_rg = orig_rg; }
       bool step_done = true;
-      if (ls_kind == 1) {    // Metropolis (kernel2.cu:364-392)
+      if (ls_kind == 0) {    // a lone local search: its result is the chain's output
+        stash_put<G>(S.stash, kSlotBest, x); best_e = fx; phase = kDone; step_done = false;
+      } else if (ls_kind == 1) {    // Metropolis (kernel2.cu:364-392)
         const bool accept = step == 0 || fx < tmp_e || u_metro < expf((tmp_e - fx) / P.temperature);
         if (accept) { stash_put<G>(S.stash, kSlotTmp, x); tmp_e = fx; rg = x_rg; }
         if (accept && tmp_e < best_e) { ls_kind = 2; phase = kStart; step_done = false; }   // second search of x
@@ -437,7 +462,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
       item = (it == 0) ? (int)blockIdx.x : a.n_items;
     }
     if (item >= a.n_items) break;
-    const int2 wi = a.items[item];               // (ligand slot, first chain of the block)
+    // (ligand slot, first chain of the block); without an item list, one ligand's chain blocks in order
+    const int2 wi = a.items ? a.items[item] : make_int2(0, item * CPW);
     const vd_lig_header h = a.headers[wi.x];
     if (wi.x != cur) {
       __syncthreads();
@@ -449,6 +475,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
     make_views<G>(a.blob, h, lds, a.Amax, a.Nmax, MODE == 0, group, L, S, H);
     const uint32_t chain = (uint32_t)(wi.y + group);
     const bool valid = (int)chain < a.P.chains;
+    const float* given = (a.init_conf && valid) ? a.init_conf + a.out_off[wi.x].y + (size_t)chain * (7 + h.n_torsions)
+                                                : nullptr;
     int evals = 0;
     History hist;
     if constexpr (MODE == 2) hist.init(a.hist + ((long long)blockIdx.x * CPW + group) * a.hist_stride, h.n_torsions);
@@ -460,11 +488,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
     }
     if constexpr (MODE == 0)
       mc_chain<G>(L, S, H, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l,
-                  a.out_e + a.out_off[wi.x].x, a.out_conf + a.out_off[wi.x].y, a.out_heavy + a.out_off[wi.x].z, evals);
+                  a.out_e + a.out_off[wi.x].x, a.out_conf + a.out_off[wi.x].y, a.out_heavy + a.out_off[wi.x].z, evals,
+                  given, a.ls_only != 0);
     else
       mc_chain_flat<G, MODE - 1>(L, S, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l,
                                  a.out_e + a.out_off[wi.x].x, a.out_conf + a.out_off[wi.x].y,
-                                 a.out_heavy + a.out_off[wi.x].z, evals, &hist, &gh);
+                                 a.out_heavy + a.out_off[wi.x].z, evals, &hist, &gh, given, a.ls_only != 0);
     if (a.slot_evals) {
       int sum = 0;
       for (int gi = 0; gi < CPW; ++gi) sum += __shfl(evals, gi * G);

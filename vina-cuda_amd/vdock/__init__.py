@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libvdock.so")
+# VDOCK_LIBRARY selects another build of the same library (the sanitizer build of the host code)
+LIB_PATH = os.environ.get("VDOCK_LIBRARY") or os.path.join(HERE, "libvdock.so")
 
 VD_OK, VD_ERR_ARG, VD_ERR_CAPACITY, VD_ERR_HIP, VD_ERR_PARSE, VD_ERR_NO_DEVICE = 0, -1, -2, -3, -4, -5
 VD_LS_BFGS, VD_LS_RILC_BFGS = 0, 1
@@ -31,7 +32,7 @@ EXPORTS = (
     "vd_synth_receptor", "vd_synth_ligand", "vd_device_alloc", "vd_device_free", "vd_copy", "vd_random_confs",
     "vd_last_launches", "vd_set_class_overlap", "vd_set_autotune", "vd_ligand_pair_chunks",
     "vd_post_params_default", "vd_refine", "vd_select_modes", "vd_postprocess_batch", "vd_postprocess_docked",
-    "vd_post_results", "vd_write_pdbqt",
+    "vd_post_results", "vd_write_pdbqt", "vd_local_search",
 )
 
 
@@ -128,7 +129,8 @@ def load():
         "vd_eval_last_ms": (C.c_double, [vp]),
         "vd_dock_params_default": (C.c_int, [C.POINTER(DockParams)]),
         "vd_dock_heuristics": (C.c_int, [C.c_int, C.c_int, C.c_int, ip, ip]),
-        "vd_dock": (C.c_int, [vp, C.c_int, C.POINTER(DockParams), fp, fp, fp, C.POINTER(Stats)]),
+        "vd_dock": (C.c_int, [vp, C.c_int, C.POINTER(DockParams), fp, fp, fp, fp, C.POINTER(Stats)]),
+        "vd_local_search": (C.c_int, [vp, C.c_int, C.c_int, fp, C.POINTER(DockParams), fp, fp, C.POINTER(Stats)]),
         "vd_dock_batch": (C.c_int, [vp, C.c_int, ip, C.POINTER(DockParams), C.POINTER(Stats)]),
         "vd_dock_batch_plain": (C.c_int, [vp, C.c_int, ip, C.POINTER(DockParams), C.POINTER(Stats)]),
         "vd_batch_results": (C.c_int, [vp, C.c_int, C.c_int, fp, fp, fp]),
@@ -357,15 +359,34 @@ class Context:
         return self.lib.vd_eval_last_ms(self.h)
 
     # -- docking
-    def dock(self, lid, params):
+    def dock(self, lid, params, init_conf=None):
+        """Plain per-ligand launch -> (e, conf, heavy, stats); init_conf [chains][7+T] gives each
+        chain's starting conformation (else drawn from its random stream)."""
         inf = self.ligand_info(lid)
         n = params.chains
         e = np.zeros(n, dtype=np.float32)
         conf = np.zeros((n, 7 + inf.n_torsions), dtype=np.float32)
         heavy = np.zeros((n, inf.n_heavy, 3), dtype=np.float32)
         st = Stats()
-        self._check(self.lib.vd_dock(self.h, lid, C.byref(params), _f(e), _f(conf), _f(heavy), C.byref(st)))
+        init = None
+        if init_conf is not None:
+            init = np.ascontiguousarray(init_conf, dtype=np.float32)
+            if init.shape != (n, 7 + inf.n_torsions):
+                raise ValueError("init_conf must have shape (chains, 7 + T)")
+        self._check(self.lib.vd_dock(self.h, lid, C.byref(params), _f(init) if init is not None else None, _f(e), _f(conf),
+                                     _f(heavy), C.byref(st)))
         return e, conf, heavy, st
+
+    def local_search(self, lid, conf, params):
+        """One local search (params.local_search, params.bfgs_steps) from every row of conf
+        -> (e, conf_out, stats)."""
+        inf = self.ligand_info(lid)
+        c = np.ascontiguousarray(conf, dtype=np.float32).reshape(-1, 7 + inf.n_torsions)
+        e = np.zeros(len(c), dtype=np.float32)
+        out = np.zeros_like(c)
+        st = Stats()
+        self._check(self.lib.vd_local_search(self.h, lid, len(c), _f(c), C.byref(params), _f(out), _f(e), C.byref(st)))
+        return e, out, st
 
     def dock_batch(self, ids, params, plain=False):
         a = np.ascontiguousarray(ids, dtype=np.int32)
