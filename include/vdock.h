@@ -175,7 +175,9 @@ int vd_last_launches(vd_ctx* ctx, int max, vd_launch* out);
 int vd_set_class_overlap(vd_ctx* ctx, int on);
 /* on = 0 (default): with chains_per_wave or waves_per_simd left at 0, each ligand's
  * (lanes per chain, waves per SIMD) pair comes from a fixed table per size bucket (atoms x
- * torsions), so that the same inputs and seed give the same results in every process.  on = 1:
+ * torsions), so that under the Vina and QuickVina2 policies the same inputs and seed give the same
+ * results in every process (QuickVina-W's shared buffer is filled in the order chains happen to
+ * run, so its results depend on scheduling).  on = 1:
  * the pair is timed per bucket on first use (trial launches of a representative ligand) and the
  * fastest kept for the context; timings that flip between processes then change a run's FP32
  * rounding and with it its trajectories.  Turning it off forgets the timed pairs.  No counterpart

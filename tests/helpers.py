@@ -48,6 +48,29 @@ def map_error_bounds(lig, box, tol, conf):
     return eb, fb * diam
 
 
+_CUTOFF_JUMP = None
+
+
+def intra_cutoff_bound(lig, conf, ulps=8):
+    """The pair table is not zero at the 8 A cutoff (the gauss2 term of two carbons is about
+    -0.0036 kcal/mol there) and a pair at or beyond it contributes nothing: an intramolecular pair
+    whose r^2 lies within FP32 rounding of 64 (coordinates rounded by `ulps` units in the last
+    place, r^2 = 64 +- 2 r (2 d)) may be counted on one side and not on the other, which moves the
+    energy by the table's value there."""
+    global _CUTOFF_JUMP
+    if _CUTOFF_JUMP is None:
+        lut = orc.Lut()
+        _CUTOFF_JUMP = np.array([np.abs(lut.tables(tp)[0][2046:2051]).max() for tp in range(orc.N_PAIRS)])
+    X = lig.set_conf(np.asarray(conf, dtype=np.float64))
+    pairs = lig.pairs()
+    if len(pairs) == 0:
+        return 0.0
+    d = ulps * float(np.spacing(np.float32(np.abs(X).max())))
+    r2 = ((X[pairs[:, 1]] - X[pairs[:, 2]]) ** 2).sum(-1)
+    near = np.abs(r2 - 64.0) <= 2 * 8.0 * 2 * d
+    return float(_CUTOFF_JUMP[pairs[near, 0]].sum())
+
+
 def eval_bound(e_ref, g_ref):
     """The evaluation's own bound on shared maps (north star: 1e-4 kcal/mol absolute / 1e-3
     relative on energies; gradients per component 1e-3 of the largest component plus 1e-4)."""
@@ -72,15 +95,16 @@ def maps_within(dev, ref, slack):
 def check_best_chains(ctx, grid, lig, lid, e, conf, heavy, box, k_best=8):
     """Property checks of one ligand's docking output: every value finite; the energies of the k
     best chains are the oracle's evaluation (search caps: v = 10, slope 1e6) of the returned
-    conformations on the same maps, within max(1e-3, 1e-3 |e|) plus the out-of-box penalty's FP32
-    rounding (poses the search presses against a wall of the box); the heavy-atom coordinates are
-    those of the returned conformation."""
+    conformations on the same maps, within max(1e-3, 1e-3 |e|) plus what FP32 rounding can do to
+    the two discontinuous parts of the energy: the out-of-box penalty (poses the search presses
+    against a wall of the box) and intramolecular pairs at the cutoff; the heavy-atom coordinates
+    are those of the returned conformation."""
     assert np.isfinite(e).all() and np.isfinite(conf).all() and np.isfinite(heavy).all()
     worst = 0.0
     for i in np.argsort(e)[:k_best]:
         c = conf[i].astype(np.float64)
         e_ref = orc.eval_deriv(grid, lig, c)[0]
-        tol = max(1e-3, 1e-3 * abs(e_ref)) + out_of_box_bound(lig, box, c)
+        tol = max(1e-3, 1e-3 * abs(e_ref)) + out_of_box_bound(lig, box, c) + intra_cutoff_bound(lig, c)
         assert abs(e[i] - e_ref) <= tol, (lid, i, e[i], e_ref)
         worst = max(worst, abs(e[i] - e_ref))
         x = lig.set_conf(c)[lig.atoms()["el"] != 0]

@@ -788,15 +788,18 @@ __device__ bool qvw_interesting(const History& h, const GlobalHist& gh, const Co
 
 // Dense BFGS with backtracking line search (V11/lib/bfgs.h:30-140; device form
 // quasi_newton.cuh:1859-1916, 2391-2499).  H (n x n, n = 6+T <= G) lives in LDS, row i in the
-// lane that owns change component i.
+// lane that owns change component i.  The loops over a row stay rolled: unrolled G-long, they
+// kept G LDS values live at once and spilled up to 1.3 KB per lane.
 template <int G>
 __device__ float dense_bfgs(const LigView& L, const Scratch& S, float* H, float* vbuf, const EvalK& K, Conf& x,
                             bool enabled, float x_e, int max_steps, int l, int& evals) {
   // H is G x G, lane l owns row l; vbuf holds G floats for broadcasting a vector
+#pragma unroll 1
   for (int j = 0; j < G; ++j) H[l * G + j] = (j == l) ? 1.0f : 0.0f;
   auto matvec = [&](float v) {     // returns -(H v)_l
     __syncthreads(); vbuf[l] = v; __syncthreads();
     float acc = 0.0f;
+#pragma unroll 1
     for (int j = 0; j < G; ++j) acc += H[l * G + j] * vbuf[j];
     __syncthreads();
     return -acc;
@@ -831,6 +834,7 @@ __device__ float dense_bfgs(const LigView& L, const Scratch& S, float* H, float*
     const float yy = gsum<G>(y * y), yp = gsum<G>(y * p);
     if (step == 0 && upd && fabsf(yy) > kEpsFl) {
       const float diag = alpha * yp / yy;
+#pragma unroll 1
       for (int j = 0; j < G; ++j) H[l * G + j] = (j == l) ? diag : H[l * G + j];
     }
     const float mhy = matvec(y);
@@ -843,6 +847,7 @@ __device__ float dense_bfgs(const LigView& L, const Scratch& S, float* H, float*
     vbuf[l] = mhy; pb[l] = p;
     __syncthreads();
     if (do_upd)
+#pragma unroll 1
       for (int j = 0; j < G; ++j)
         H[l * G + j] += alpha * r * (mhy * pb[j] + vbuf[j] * p) + alpha * alpha * (r * r * yhy + r) * p * pb[j];
     __syncthreads();

@@ -140,11 +140,17 @@ void Precalculated::build(const double w[5]) {
 }
 
 void Precalculated::grid_rows(std::vector<float>& out) const {
-  out.assign((size_t)VD_XS_SIZE * VD_LUT_BINS * VD_GRID_ROW, 0.0f);
+  const size_t rows = (size_t)VD_XS_SIZE * VD_LUT_BINS;
+  out.assign(rows * VD_GRID_ROW + rows, 0.0f);
   for (int t1 = 0; t1 < VD_XS_SIZE; ++t1)
-    for (int b = 0; b < VD_LUT_BINS; ++b)
-      for (int t2 = 0; t2 < VD_XS_SIZE; ++t2)
-        out[((size_t)t1 * VD_LUT_BINS + b) * VD_GRID_ROW + t2] = (float)fast[(size_t)type_pair_index(t1, t2) * VD_LUT_BINS + b];
+    for (int b = 0; b < VD_LUT_BINS; ++b) {
+      const size_t r = (size_t)t1 * VD_LUT_BINS + b;
+      for (int t2 = 0; t2 < VD_XS_SIZE; ++t2) {
+        const float v = (float)fast[(size_t)type_pair_index(t1, t2) * VD_LUT_BINS + b];
+        if (t2 < VD_GRID_ROW) out[r * VD_GRID_ROW + t2] = v;
+        else out[rows * VD_GRID_ROW + r] = v;
+      }
+    }
 }
 
 void Precalculated::pair_quads(std::vector<float>& out) const {

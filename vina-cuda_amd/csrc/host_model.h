@@ -37,7 +37,7 @@ struct Precalculated {
   std::vector<double> smooth_e, smooth_dor, fast;   // [153][VD_LUT_BINS]
   void build(const double w[5]);
   // FP32 device layouts:
-  //   grid rows  [t1][bin][VD_GRID_ROW]: fast value for every t2 (zero padded)
+  //   grid rows  [t1][bin][VD_GRID_ROW]: fast value for t2 = 0..15, then the column [t1][bin] of t2 = 16
   //   pair quads [tp][bin] = (e_i, dor_i, e_{i+1}, dor_{i+1})
   void grid_rows(std::vector<float>& out) const;
   void pair_quads(std::vector<float>& out) const;

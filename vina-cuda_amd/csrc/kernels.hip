@@ -22,14 +22,16 @@ namespace {
 // ------------------------------------------------------------------------------------------
 // Grid maps.  A workgroup of 256 lanes owns a 16x4x4 tile of points (x fastest); the receptor
 // atoms that can reach the tile come from a host-built cell list (cells of VD_GCELL A over the
-// box grown by the 8 A cutoff).  For every atom within the cutoff of a point, one 20-float row
-// of the map LUT gives the contribution to all 17 maps (rows indexed [t1][bin]).
+// box grown by the 8 A cutoff).  For every atom within the cutoff of a point, one 16-float row
+// of the map LUT (rows indexed [t1][bin]) and one value of the column after the rows give the
+// contribution to all 17 maps: 68 bytes, in four aligned 16-byte loads and one 4-byte load.
 constexpr int TX = 16, TY = 4, TZ = 4;
 constexpr int GU = 2;   // receptor atoms per pass of the grid kernel's loop (4 measured slower)
 
 __global__ __launch_bounds__(256) void vd_grid_kernel(const float4* __restrict__ atoms, const int2* __restrict__ cells,
                                                       GridCells gc, const float4* __restrict__ lut_rows,
-                                                      float* __restrict__ maps, vd_grid_desc gd, int tiles_x, int tiles_y) {
+                                                      const float* __restrict__ lut_last, float* __restrict__ maps,
+                                                      vd_grid_desc gd, int tiles_x, int tiles_y) {
   const int tile = blockIdx.x;
   const int tx0 = (tile % tiles_x) * TX, ty0 = ((tile / tiles_x) % tiles_y) * TY, tz0 = (tile / (tiles_x * tiles_y)) * TZ;
   const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
@@ -57,7 +59,8 @@ __global__ __launch_bounds__(256) void vd_grid_kernel(const float4* __restrict__
       // GU atoms per pass: their table rows are loaded before any is added (each lane still adds
       // its atoms in list order, so the sums are those of a one-by-one loop)
       for (int i = begin; i < end; i += GU) {
-        float4 rv[GU][5];
+        float4 rv[GU][4];
+        float last[GU];
         bool use[GU];
 #pragma unroll
         for (int u = 0; u < GU; ++u) {
@@ -71,9 +74,11 @@ __global__ __launch_bounds__(256) void vd_grid_kernel(const float4* __restrict__
           use[u] = i + u < end && ddx * ddx + ddy * ddy + ddz * ddz <= VD_CUTOFF_SQR && inside && r2 <= VD_CUTOFF_SQR;
           const int t1 = (int)__float_as_uint(a.w);
           const int bin = use[u] ? (int)(VD_LUT_FACTOR * r2) : 0;
-          const float4* rw = lut_rows + ((size_t)(use[u] ? t1 : 0) * VD_LUT_BINS + bin) * (VD_GRID_ROW / 4);
+          const size_t r = (size_t)(use[u] ? t1 : 0) * VD_LUT_BINS + bin;
+          const float4* rw = lut_rows + r * (VD_GRID_ROW / 4);
 #pragma unroll
-          for (int c = 0; c < 5; ++c) rv[u][c] = use[u] ? rw[c] : make_float4(0, 0, 0, 0);
+          for (int c = 0; c < 4; ++c) rv[u][c] = use[u] ? rw[c] : make_float4(0, 0, 0, 0);
+          last[u] = use[u] ? lut_last[r] : 0.0f;
         }
 #pragma unroll
         for (int u = 0; u < GU; ++u) {
@@ -82,7 +87,7 @@ __global__ __launch_bounds__(256) void vd_grid_kernel(const float4* __restrict__
           acc[4] += rv[u][1].x; acc[5] += rv[u][1].y; acc[6] += rv[u][1].z; acc[7] += rv[u][1].w;
           acc[8] += rv[u][2].x; acc[9] += rv[u][2].y; acc[10] += rv[u][2].z; acc[11] += rv[u][2].w;
           acc[12] += rv[u][3].x; acc[13] += rv[u][3].y; acc[14] += rv[u][3].z; acc[15] += rv[u][3].w;
-          acc[16] += rv[u][4].x;
+          acc[16] += last[u];
         }
       }
     }
@@ -599,7 +604,9 @@ int mc_lds_bytes(int G, int Amax, int Nmax, int dense) {
 hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& gc, const float4* lut_rows, float* maps,
                        const vd_grid_desc& gd, hipStream_t s) {
   const int tx = (gd.dim[0] + TX - 1) / TX, ty = (gd.dim[1] + TY - 1) / TY, tz = (gd.dim[2] + TZ - 1) / TZ;
-  hipLaunchKernelGGL(vd_grid_kernel, dim3(tx * ty * tz), dim3(256), 0, s, atoms, cells, gc, lut_rows, maps, gd, tx, ty);
+  const float* lut_last = reinterpret_cast<const float*>(lut_rows) + (size_t)VD_XS_SIZE * VD_LUT_BINS * VD_GRID_ROW;
+  hipLaunchKernelGGL(vd_grid_kernel, dim3(tx * ty * tz), dim3(256), 0, s, atoms, cells, gc, lut_rows, lut_last, maps, gd,
+                     tx, ty);
   return hipGetLastError();
 }
 

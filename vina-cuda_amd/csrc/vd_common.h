@@ -11,7 +11,9 @@
 #define VD_LUT_FACTOR 32.0f
 #define VD_CUTOFF_SQR 64.0f
 #define VD_ZERO_QUAD (VD_NUM_TYPE_PAIRS * VD_LUT_BINS)   // first quad of the all-zero row after the pair table
-#define VD_GRID_ROW 20               // per (t1, bin) row of the map LUT: 17 t2 values padded to 20
+#define VD_GRID_ROW 16               // per (t1, bin) row of the map LUT: the values of t2 = 0..15 (64 B,
+                                     // four aligned 16-byte loads); the t2 = 16 values follow all rows
+                                     // as a [t1][bin] column
 #define VD_MAX_ATOMS 255             // per ligand: atom indices are 8-bit fields of the pair words and node
                                      // atom ranges (the reference caps at 128, MAX_NUM_OF_ATOMS, V11/inc/kernel2.h:35)
 #define VD_MAX_NODES 64              // rigid nodes (torsions + 1)
