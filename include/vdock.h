@@ -203,6 +203,18 @@ int vd_dock(vd_ctx* ctx, int id, const vd_dock_params* p, const float* init_conf
 int vd_local_search(vd_ctx* ctx, int id, int n, const float* conf_in, const vd_dock_params* p, float* conf_out,
                     float* e, vd_stats* st);
 
+/* vd_dock for callers that cannot allocate (code under HIP graph capture, tight serving loops):
+ * no counterpart in the reference.  All pointers are device pointers the caller owns: d_init_conf
+ * [chains][7+T] or NULL (drawn), d_e [chains], d_conf [chains][7+T], d_heavy [chains][n_heavy][3].
+ * Every other device buffer of the call (the ligand record, launch bookkeeping, the QuickVina2 /
+ * QuickVina-W histories) lives in a 4 MiB array per device (VD_NOALLOC_SCRATCH_BYTES in the
+ * library), which the device holds until the process ends: the call allocates no device memory,
+ * and returns VD_ERR_CAPACITY when a ligand and chain count need more than the array.  The
+ * launch shape is the fixed per-bucket table (no timing trials).  All calls on one device share
+ * its array: calls that overlap on one device, from any context or thread, race. */
+int vd_dock_noalloc(vd_ctx* ctx, int id, const vd_dock_params* p, const float* d_init_conf, float* d_e,
+                    float* d_conf, float* d_heavy, vd_stats* st);
+
 /* Many ligands of uneven size in one persistent launch fed by a device work queue.
  * Results stay on the device; fetch them per ligand with vd_batch_results. */
 int vd_dock_batch(vd_ctx* ctx, int n, const int* ids, const vd_dock_params* p, vd_stats* st);

@@ -160,6 +160,7 @@ struct vd_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int n_cu = 0;
   int lds_max = 64 * 1024;                   // LDS one workgroup may take (queried from the device)
+  unsigned char* scratch = nullptr;          // this device's scratch array (vd_dock_noalloc)
   DevBuf<float> init;                        // caller's starting conformations (vd_dock, vd_local_search)
   std::map<int, std::pair<int, int>> tuned;  // size bucket -> (lanes per chain, waves per SIMD)
   bool autotune_on = false;                   // timing-based choice of the pair above (vd_set_autotune)
@@ -185,6 +186,21 @@ struct vd_ctx {
 };
 
 namespace {
+
+// The device address of each device's scratch array, looked up once per device (with that device
+// current) and kept for the process: every context on a device shares its array.
+std::mutex g_scratch_mutex;
+std::map<int, unsigned char*> g_scratch;
+unsigned char* scratch_for_current_device(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  auto it = g_scratch.find(device);
+  if (it != g_scratch.end()) return it->second;
+  void* p = nullptr;
+  HIP_CHECK(vdl::scratch_address(&p));
+  if (!p) throw HipError("hipGetSymbolAddress returned no address for the scratch array");
+  g_scratch[device] = static_cast<unsigned char*>(p);
+  return g_scratch[device];
+}
 
 int guard(vd_ctx* ctx, const std::function<void()>& f) {
   try {
@@ -950,6 +966,7 @@ int vd_create(int device, vd_ctx** out) {
     HIP_CHECK(hipGetDeviceProperties(&prop, device));
     c->n_cu = prop.multiProcessorCount;
     c->lds_max = (int)prop.sharedMemPerBlock;
+    c->scratch = scratch_for_current_device(device);
     upload_lut(c);
   });
   if (rc != VD_OK) { delete c; return rc; }
@@ -1320,7 +1337,111 @@ int vd_local_search(vd_ctx* c, int id, int n, const float* conf_in, const vd_doc
 
 }  // extern "C"
 
+namespace {
+
+// The no-allocation docking call: the plain launch of vd_dock with every buffer beyond the
+// caller's inputs and outputs carved out of this device's scratch array.
+void dock_noalloc(vd_ctx* c, int id, const vd_dock_params* p, const float* d_init, float* d_e, float* d_conf,
+                  float* d_heavy, vd_stats* st) {
+  check_params(p);
+  need_device(c);
+  if (!c->maps_ready) throw ArgError(VD_ERR_ARG, "no maps");
+  if (!d_e || !d_conf || !d_heavy) throw ArgError(VD_ERR_ARG, "output pointers are NULL");
+  if (!c->scratch) throw ArgError(VD_ERR_ARG, "no scratch array on this device");
+  if (st) std::memset(st, 0, sizeof(*st));
+  LigPlan pl = plan_for(c, id, p, true);
+  std::vector<int> dv{pl.depth}, bv{pl.bfgs};
+  Packed pk = pack(c, pl.G, {id}, &dv, &bv);
+  const int cpw = 64 / pl.G;
+  const int blocks = (p->chains + cpw - 1) / cpw;
+  vd_mc_consts P = make_consts(c, p, pl.depth, pl.bfgs);
+  const int mode = mc_mode(P);
+  const int lds = vdl::mc_lds_bytes(pl.G, pk.Amax, pk.Nmax, mode == 0);
+  if (lds > c->lds_max) throw ArgError(VD_ERR_CAPACITY, "ligand too large for the LDS layout");
+  // scratch layout, 256-byte aligned regions
+  size_t used = 0;
+  auto take = [&](size_t bytes) { const size_t o = used; used += (bytes + 255) / 256 * 256; return o; };
+  const size_t o_hdr = take(sizeof(vd_lig_header)), o_stream = take(4), o_off = take(sizeof(longlong4));
+  const size_t o_evals = take(8), o_blob = take(pk.blob.size() * 4);
+  long long hist_stride = 0, glob_floats = 0;
+  int axis = 0;
+  size_t o_hist = 0, o_glob = 0, o_goff = 0, o_gaxis = 0, o_gT = 0;
+  if (mode == 2 || mode == 3) {
+    hist_stride = vdk::History::floats_per_chain(pl.T, mode == 3 ? 5 : 10);
+    o_hist = take((size_t)blocks * cpw * hist_stride * 4);
+  }
+  if (mode == 3) {
+    axis = 1 << qvw_levels(p->chains, pl.depth);
+    pk.headers[0].reserved = axis;
+    glob_floats = vdk::GlobalHist::floats(axis, pl.T);
+    o_glob = take((size_t)glob_floats * 4);
+    o_goff = take(8); o_gaxis = take(4); o_gT = take(4);
+  }
+  if (used > VD_NOALLOC_SCRATCH_BYTES)
+    throw ArgError(VD_ERR_CAPACITY, "vd_dock_noalloc needs " + std::to_string(used) + " bytes of scratch; the array holds " +
+                                        std::to_string((size_t)VD_NOALLOC_SCRATCH_BYTES) +
+                                        " (fewer chains, or vd_dock, which allocates what it needs)");
+  unsigned char* base = c->scratch;
+  hipStream_t s = c->stream;
+  const uint32_t stream_id = (uint32_t)id;
+  const longlong4 off = make_longlong4(0, 0, 0, 0);
+  HIP_CHECK(hipMemcpyAsync(base + o_hdr, pk.headers.data(), sizeof(vd_lig_header), hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(base + o_stream, &stream_id, 4, hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(base + o_off, &off, sizeof(longlong4), hipMemcpyHostToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(base + o_blob, pk.blob.data(), pk.blob.size() * 4, hipMemcpyHostToDevice, s));
+  HIP_CHECK(vdl::zero_u64(reinterpret_cast<unsigned long long*>(base + o_evals), 1, s));
+  const long long goff = 0;
+  const int T = pl.T;
+  if (mode == 3) {
+    HIP_CHECK(hipMemcpyAsync(base + o_goff, &goff, 8, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(base + o_gaxis, &axis, 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(hipMemcpyAsync(base + o_gT, &T, 4, hipMemcpyHostToDevice, s));
+    HIP_CHECK(vdl::glob_init(reinterpret_cast<float*>(base + o_glob), reinterpret_cast<const long long*>(base + o_goff),
+                             reinterpret_cast<const int*>(base + o_gaxis), reinterpret_cast<const int*>(base + o_gT), 1, s));
+  }
+  McArgs a{};
+  a.blob = reinterpret_cast<const float4*>(base + o_blob);
+  a.headers = reinterpret_cast<const vd_lig_header*>(base + o_hdr);
+  a.streams = reinterpret_cast<const uint32_t*>(base + o_stream);
+  a.items = nullptr;
+  a.n_items = blocks;
+  a.init_conf = d_init;
+  a.ls_only = 0;
+  a.K = make_evalk(c, P.v_intra, P.v_inter, P.slope);
+  a.P = P;
+  a.out_e = d_e; a.out_conf = d_conf; a.out_heavy = d_heavy;
+  a.out_off = reinterpret_cast<const longlong4*>(base + o_off);
+  a.slot_evals = reinterpret_cast<unsigned long long*>(base + o_evals);
+  a.Amax = pk.Amax; a.Nmax = pk.Nmax;
+  if (mode == 2 || mode == 3) { a.hist = reinterpret_cast<float*>(base + o_hist); a.hist_stride = hist_stride; }
+  if (mode == 3) a.glob = reinterpret_cast<float*>(base + o_glob);
+  const int W = loosest_budget(pl.G, pl.W, mode, lds);
+  McSlot& slot = c->mc[0];
+  if (slot.busy) { HIP_CHECK(hipStreamSynchronize(slot.stream)); slot.busy = false; }
+  HIP_CHECK(hipEventRecord(slot.ev0, s));
+  HIP_CHECK(vdl::launch_mc(pl.G, W, a, blocks, s));
+  HIP_CHECK(hipEventRecord(slot.ev1, s));
+  unsigned long long evals = 0;
+  HIP_CHECK(hipMemcpyAsync(&evals, base + o_evals, 8, hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
+  float ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&ms, slot.ev0, slot.ev1));
+  vd_launch rec{};
+  rec.lanes_per_chain = pl.G; rec.mode = mode; rec.waves_per_simd = W; rec.blocks = blocks; rec.items = blocks;
+  rec.ligands = 1; rec.kernel_ms = ms; rec.evals = evals; rec.eval_bytes = (double)evals * eval_bytes(pk.headers[0]);
+  c->launches.assign(1, rec);
+  if (st) { st->kernel_ms = ms; st->evals = evals; st->chains_per_wave = cpw; st->blocks = blocks; }
+}
+
+}  // namespace
+
 extern "C" {
+
+int vd_dock_noalloc(vd_ctx* c, int id, const vd_dock_params* p, const float* d_init_conf, float* d_e, float* d_conf,
+                    float* d_heavy, vd_stats* st) {
+  if (!c) return VD_ERR_ARG;
+  return guard(c, [&]() { dock_noalloc(c, id, p, d_init_conf, d_e, d_conf, d_heavy, st); });
+}
 
 static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, vd_stats* st, bool persistent) {
   if (!c || n < 0 || (n > 0 && !ids)) return VD_ERR_ARG;

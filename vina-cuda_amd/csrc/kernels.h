@@ -73,4 +73,8 @@ hipError_t launch_mc(int G, int W, const McArgs& a, int blocks, hipStream_t s);
 hipError_t reset_counter(int* p, hipStream_t s);
 hipError_t glob_init(float* base, const long long* off, const int* per_axis, const int* T, int n, hipStream_t s);
 hipError_t random_confs(float* conf, long long n, int T, uint32_t seed, const float lo[3], const float hi[3], hipStream_t s);
+hipError_t zero_u64(unsigned long long* p, int n, hipStream_t s);
+// The device address of this device's scratch array (the memory of the no-allocation entry
+// point); call with the device current.
+hipError_t scratch_address(void** p);
 }  // namespace vdl

@@ -588,7 +588,14 @@ __global__ __launch_bounds__(64) void vd_refine_kernel(RefineArgs a) {
 
 }  // namespace
 
+// Device memory of the no-allocation docking entry point (vd_dock_noalloc): one array per device,
+// held until the process ends.  Host code never passes the array's name to a kernel or a copy: it
+// takes the device address from scratch_address once per device and uses that.
+static __device__ __align__(256) unsigned char scratch[VD_NOALLOC_SCRATCH_BYTES];
+
 namespace vdl {
+
+hipError_t scratch_address(void** p) { return hipGetSymbolAddress(p, HIP_SYMBOL(scratch)); }
 
 int refine_lds_bytes(int Amax, int Nmax, int nmax) { return 8 * vdr::pose_lds_doubles(Amax, Nmax, nmax); }
 
@@ -660,6 +667,13 @@ hipError_t launch_mc(int G, int W, const McArgs& a, int blocks, hipStream_t s) {
 }
 
 __global__ void vd_fill_int(int* p, int v) { *p = v; }
+__global__ void vd_zero_u64_kernel(unsigned long long* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+hipError_t zero_u64(unsigned long long* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(vd_zero_u64_kernel, dim3(1), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
 
 // Clears QuickVina-W global buffers: counters 0, conformation numbers 1e30 (never near),
 // energies 0, zero-derivative masks all ones (an entry nobody wrote never explains a point).

@@ -73,3 +73,7 @@ struct vd_mc_consts {
 };
 
 #define VD_INIT_STEP 0xFFFFFFFFu
+
+// Device memory of the no-allocation docking entry point (vd_dock_noalloc): one array per device
+// for the ligand record, launch bookkeeping and search histories of one call.
+#define VD_NOALLOC_SCRATCH_BYTES (4u << 20)

@@ -32,7 +32,7 @@ EXPORTS = (
     "vd_synth_receptor", "vd_synth_ligand", "vd_device_alloc", "vd_device_free", "vd_copy", "vd_random_confs",
     "vd_last_launches", "vd_set_class_overlap", "vd_set_autotune", "vd_ligand_pair_chunks",
     "vd_post_params_default", "vd_refine", "vd_select_modes", "vd_postprocess_batch", "vd_postprocess_docked",
-    "vd_post_results", "vd_write_pdbqt", "vd_local_search",
+    "vd_post_results", "vd_write_pdbqt", "vd_local_search", "vd_dock_noalloc",
 )
 
 
@@ -131,6 +131,7 @@ def load():
         "vd_dock_heuristics": (C.c_int, [C.c_int, C.c_int, C.c_int, ip, ip]),
         "vd_dock": (C.c_int, [vp, C.c_int, C.POINTER(DockParams), fp, fp, fp, fp, C.POINTER(Stats)]),
         "vd_local_search": (C.c_int, [vp, C.c_int, C.c_int, fp, C.POINTER(DockParams), fp, fp, C.POINTER(Stats)]),
+        "vd_dock_noalloc": (C.c_int, [vp, C.c_int, C.POINTER(DockParams), vp, vp, vp, vp, C.POINTER(Stats)]),
         "vd_dock_batch": (C.c_int, [vp, C.c_int, ip, C.POINTER(DockParams), C.POINTER(Stats)]),
         "vd_dock_batch_plain": (C.c_int, [vp, C.c_int, ip, C.POINTER(DockParams), C.POINTER(Stats)]),
         "vd_batch_results": (C.c_int, [vp, C.c_int, C.c_int, fp, fp, fp]),
@@ -387,6 +388,12 @@ class Context:
         st = Stats()
         self._check(self.lib.vd_local_search(self.h, lid, len(c), _f(c), C.byref(params), _f(out), _f(e), C.byref(st)))
         return e, out, st
+
+    def dock_noalloc(self, lid, params, d_e, d_conf, d_heavy, d_init=None):
+        """The no-allocation docking call on device pointers (integers) the caller owns."""
+        st = Stats()
+        self._check(self.lib.vd_dock_noalloc(self.h, lid, C.byref(params), d_init, d_e, d_conf, d_heavy, C.byref(st)))
+        return st
 
     def dock_batch(self, ids, params, plain=False):
         a = np.ascontiguousarray(ids, dtype=np.int32)
