@@ -141,6 +141,16 @@ def copy_bandwidth(ctx, nbytes=1 << 30, reps=5):
     return 2 * nbytes / best / 1e9
 
 
+def copy_calibration(ctx, n=256 << 20):
+    """Two device-to-device copies of n bytes: the reference dispatch that scales the memory
+    counters of a profile (bytes counted for a copy of known size)."""
+    src, dst = ctx.alloc(n), ctx.alloc(n)
+    for _ in range(2):
+        ctx.copy_device(dst, src, n)
+    ctx.free(src)
+    ctx.free(dst)
+
+
 def _timed(f):
     t0 = time.perf_counter()
     f()
@@ -443,6 +453,8 @@ def main():
         ms = [ctx.grid_populate(box) for _ in range(5)]
         _, _, b2 = W.large_box_scene(vdock, ctx)
         ms2 = [ctx.grid_populate(b2) for _ in range(3)]
+        if args.copy_calibration:
+            copy_calibration(ctx)
         print(json.dumps({"grid_only": True, "screening_ms": ms, "large_box_ms": ms2}), flush=True)
         return
     log("scene: receptor + maps")
@@ -621,11 +633,7 @@ def main():
             out["large_box"] = large_box_leg(vdock, log)
 
     if args.copy_calibration:
-        n = 256 << 20
-        src, dst = ctx.alloc(n), ctx.alloc(n)
-        for _ in range(2):
-            ctx.copy_device(dst, src, n)
-        ctx.free(src); ctx.free(dst)
+        copy_calibration(ctx)
     if not args.no_cpu:
         log("cpu baseline and oracle checks")
         threads, host_info = host_threads()

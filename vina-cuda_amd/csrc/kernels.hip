@@ -20,21 +20,31 @@ using namespace vdk;
 namespace {
 
 // ------------------------------------------------------------------------------------------
-// Grid maps.  A workgroup of 256 lanes owns a 16x4x4 tile of points (x fastest); the receptor
-// atoms that can reach the tile come from a host-built cell list (cells of VD_GCELL A over the
-// box grown by the 8 A cutoff).  For every atom within the cutoff of a point, one 16-float row
-// of the map LUT (rows indexed [t1][bin]) and one value of the column after the rows give the
-// contribution to all 17 maps: 68 bytes, in four aligned 16-byte loads and one 4-byte load.
-constexpr int TX = 16, TY = 4, TZ = 4;
-constexpr int GU = 2;   // receptor atoms per pass of the grid kernel's loop (4 measured slower)
+// Grid maps.  A workgroup is one wavefront and owns a 4x4x4 tile of points (x fastest), compact so
+// that an atom within the cutoff of the tile box reaches most of its points.  The receptor atoms
+// that can reach the tile come from a host-built cell list (cells of VD_GCELL A over the box grown by the
+// 8 A cutoff), one contiguous atom range per x-row of cells.  Each range is read 64 atoms at a
+// time, one per lane in a single coalesced load; a ballot of the lanes whose atom lies within the
+// cutoff of the tile box then lists the atoms worth a pass, and the loop takes them in index order
+// (GU per pass, all their table rows in flight before any is added), reading each atom from its
+// lane with readlane.  For every atom within the cutoff of a point, one 16-float row of the map
+// LUT (rows indexed [t1][bin]) and one value of the column after the rows give the contribution to
+// all 17 maps: 68 bytes, in four aligned 16-byte loads and one 4-byte load.  Every point adds its
+// atoms in the order of the cell list, so the tile shape does not change a map value.
+constexpr int TX = 4, TY = 4, TZ = 4;
+constexpr int GU = 2;   // receptor atoms per pass of the grid kernel's loop
 
-__global__ __launch_bounds__(256) void vd_grid_kernel(const float4* __restrict__ atoms, const int2* __restrict__ cells,
-                                                      GridCells gc, const float4* __restrict__ lut_rows,
-                                                      const float* __restrict__ lut_last, float* __restrict__ maps,
-                                                      vd_grid_desc gd, int tiles_x, int tiles_y) {
-  const int tile = blockIdx.x;
+__device__ __forceinline__ float lane_f(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
+__global__ __launch_bounds__(64) void vd_grid_kernel(const float4* __restrict__ atoms, const int2* __restrict__ cells,
+                                                     GridCells gc, const float4* __restrict__ lut_rows,
+                                                     const float* __restrict__ lut_last, float* __restrict__ maps,
+                                                     vd_grid_desc gd, int tiles_x, int tiles_y) {
+  const int tile = blockIdx.x, lane = threadIdx.x;
   const int tx0 = (tile % tiles_x) * TX, ty0 = ((tile / tiles_x) % tiles_y) * TY, tz0 = (tile / (tiles_x * tiles_y)) * TZ;
-  const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
+  const int lx = lane % TX, ly = (lane / TX) % TY, lz = lane / (TX * TY);
   const int x = tx0 + lx, y = ty0 + ly, z = tz0 + lz;
   const bool inside = x < gd.dim[0] && y < gd.dim[1] && z < gd.dim[2];
   // probe coordinates as grid::index_to_argument (V11/lib/grid.h:49-53), relative to the cell origin
@@ -56,38 +66,51 @@ __global__ __launch_bounds__(256) void vd_grid_kernel(const float4* __restrict__
     for (int cy = cy0; cy <= cy1; ++cy) {
       const int row = (cz * gc.n[1] + cy) * gc.n[0];
       const int begin = cells[row + cx0].x, end = cells[row + cx1].y;   // cells of one x-row are contiguous
-      // GU atoms per pass: their table rows are loaded before any is added (each lane still adds
-      // its atoms in list order, so the sums are those of a one-by-one loop)
-      for (int i = begin; i < end; i += GU) {
-        float4 rv[GU][4];
-        float last[GU];
-        bool use[GU];
+      for (int base = begin; base < end; base += 64) {
+        // this lane's atom of the chunk, and whether it can reach the tile at all
+        const int k = base + lane;
+        const float4 mine = k < end ? atoms[k] : make_float4(0, 0, 0, 0);
+        const float ddx = fmaxf(fmaxf(lo_x - mine.x, mine.x - hi_x), 0.0f);
+        const float ddy = fmaxf(fmaxf(lo_y - mine.y, mine.y - hi_y), 0.0f);
+        const float ddz = fmaxf(fmaxf(lo_z - mine.z, mine.z - hi_z), 0.0f);
+        uint64_t todo = __ballot(k < end && ddx * ddx + ddy * ddy + ddz * ddz <= VD_CUTOFF_SQR);
+        while (todo) {
+          // the next GU atoms of the list, in index order (fewer when fewer are left)
+          int j[GU];
+          bool has[GU];
 #pragma unroll
-        for (int u = 0; u < GU; ++u) {
-          const int k = i + u < end ? i + u : i;
-          const float4 a = atoms[k];
-          // skip atoms farther than the cutoff from the whole tile box (uniform test)
-          const float ddx = fmaxf(fmaxf(lo_x - a.x, a.x - hi_x), 0.0f), ddy = fmaxf(fmaxf(lo_y - a.y, a.y - hi_y), 0.0f),
-                      ddz = fmaxf(fmaxf(lo_z - a.z, a.z - hi_z), 0.0f);
-          const float dx = a.x - px, dy = a.y - py, dz = a.z - pz;
-          const float r2 = dx * dx + dy * dy + dz * dz;
-          use[u] = i + u < end && ddx * ddx + ddy * ddy + ddz * ddz <= VD_CUTOFF_SQR && inside && r2 <= VD_CUTOFF_SQR;
-          const int t1 = (int)__float_as_uint(a.w);
-          const int bin = use[u] ? (int)(VD_LUT_FACTOR * r2) : 0;
-          const size_t r = (size_t)(use[u] ? t1 : 0) * VD_LUT_BINS + bin;
-          const float4* rw = lut_rows + r * (VD_GRID_ROW / 4);
+          for (int u = 0; u < GU; ++u) {
+            has[u] = todo != 0;
+            j[u] = has[u] ? __builtin_ctzll(todo) : 0;
+            todo &= todo - 1;
+          }
+          float4 rv[GU][4];
+          float last[GU];
+          bool use[GU];
 #pragma unroll
-          for (int c = 0; c < 4; ++c) rv[u][c] = use[u] ? rw[c] : make_float4(0, 0, 0, 0);
-          last[u] = use[u] ? lut_last[r] : 0.0f;
-        }
+          for (int u = 0; u < GU; ++u) {
+            const float ax = lane_f(mine.x, j[u]), ay = lane_f(mine.y, j[u]), az = lane_f(mine.z, j[u]);
+            const int t1 = __builtin_amdgcn_readlane(__float_as_int(mine.w), j[u]);
+            const float dx = ax - px, dy = ay - py, dz = az - pz;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            use[u] = has[u] && inside && r2 <= VD_CUTOFF_SQR;
+            if (use[u]) {
+              const size_t r = (size_t)t1 * VD_LUT_BINS + (int)(VD_LUT_FACTOR * r2);
+              const float4* rw = lut_rows + r * (VD_GRID_ROW / 4);
 #pragma unroll
-        for (int u = 0; u < GU; ++u) {
-          if (!use[u]) continue;
-          acc[0] += rv[u][0].x; acc[1] += rv[u][0].y; acc[2] += rv[u][0].z; acc[3] += rv[u][0].w;
-          acc[4] += rv[u][1].x; acc[5] += rv[u][1].y; acc[6] += rv[u][1].z; acc[7] += rv[u][1].w;
-          acc[8] += rv[u][2].x; acc[9] += rv[u][2].y; acc[10] += rv[u][2].z; acc[11] += rv[u][2].w;
-          acc[12] += rv[u][3].x; acc[13] += rv[u][3].y; acc[14] += rv[u][3].z; acc[15] += rv[u][3].w;
-          acc[16] += last[u];
+              for (int c = 0; c < 4; ++c) rv[u][c] = rw[c];
+              last[u] = lut_last[r];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < GU; ++u) {
+            if (!use[u]) continue;
+            acc[0] += rv[u][0].x; acc[1] += rv[u][0].y; acc[2] += rv[u][0].z; acc[3] += rv[u][0].w;
+            acc[4] += rv[u][1].x; acc[5] += rv[u][1].y; acc[6] += rv[u][1].z; acc[7] += rv[u][1].w;
+            acc[8] += rv[u][2].x; acc[9] += rv[u][2].y; acc[10] += rv[u][2].z; acc[11] += rv[u][2].w;
+            acc[12] += rv[u][3].x; acc[13] += rv[u][3].y; acc[14] += rv[u][3].z; acc[15] += rv[u][3].w;
+            acc[16] += last[u];
+          }
         }
       }
     }
@@ -612,7 +635,7 @@ hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& 
                        const vd_grid_desc& gd, hipStream_t s) {
   const int tx = (gd.dim[0] + TX - 1) / TX, ty = (gd.dim[1] + TY - 1) / TY, tz = (gd.dim[2] + TZ - 1) / TZ;
   const float* lut_last = reinterpret_cast<const float*>(lut_rows) + (size_t)VD_XS_SIZE * VD_LUT_BINS * VD_GRID_ROW;
-  hipLaunchKernelGGL(vd_grid_kernel, dim3(tx * ty * tz), dim3(256), 0, s, atoms, cells, gc, lut_rows, lut_last, maps, gd,
+  hipLaunchKernelGGL(vd_grid_kernel, dim3(tx * ty * tz), dim3(64), 0, s, atoms, cells, gc, lut_rows, lut_last, maps, gd,
                      tx, ty);
   return hipGetLastError();
 }
