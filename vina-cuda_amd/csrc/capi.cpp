@@ -144,6 +144,8 @@ struct vd_ctx {
   vd_box box{};
   vd_grid_desc gd{};
   DevBuf<float> maps;
+  DevBuf<float4> corners;   // the maps as the searches read them: eight corners per cell
+  double corners_ms = 0.0;
   DevBuf<float4> grid_atoms;
   DevBuf<int2> grid_cells;
   double grid_ms = 0, eval_ms = 0;
@@ -371,7 +373,9 @@ vd_mc_consts make_consts(vd_ctx* c, const vd_dock_params* p, int depth, int bfgs
 
 vdk::EvalK make_evalk(vd_ctx* c, float v_intra, float v_inter, float slope) {
   vdk::EvalK k;
-  k.maps = c->maps.p; k.gd = c->gd; k.lutq = reinterpret_cast<const float4*>(c->lutq.p);
+  k.corners = c->corners.p; k.gd = c->gd; k.lutq = reinterpret_cast<const float4*>(c->lutq.p);
+  k.cd0 = c->gd.dim[0] - 1; k.cd01 = (c->gd.dim[0] - 1) * (c->gd.dim[1] - 1);
+  k.cplane = (long long)k.cd01 * (c->gd.dim[2] - 1);
   k.v_intra = v_intra; k.v_inter = v_inter; k.slope = slope;
   return k;
 }
@@ -396,6 +400,16 @@ double elapsed_ms(vd_ctx* c) {
   float ms = 0;
   HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   return ms;
+}
+
+// The corner records of the context's maps, timed on the context's stream.
+void make_corners(vd_ctx* c) {
+  c->corners.ensure((size_t)2 * VD_XS_SIZE * (c->gd.dim[0] - 1) * (c->gd.dim[1] - 1) * (c->gd.dim[2] - 1));
+  HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+  HIP_CHECK(vdl::launch_corners(c->maps.p, c->corners.p, c->gd, c->stream));
+  HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->corners_ms = elapsed_ms(c);
 }
 
 // Algorithmic bytes of one evaluation in the MC kernel: the eight map corners of every typed
@@ -979,7 +993,8 @@ void vd_destroy(vd_ctx* c) {
   if (c->host_only) { delete c; return; }
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  c->lut_rows.release(); c->lutq.release(); c->maps.release(); c->grid_atoms.release(); c->grid_cells.release();
+  c->lut_rows.release(); c->lutq.release(); c->maps.release(); c->corners.release(); c->grid_atoms.release();
+  c->grid_cells.release();
   for (int k = 0; k < kMcSlots; ++k) {
     if (c->mc[k].stream) (void)hipStreamSynchronize(c->mc[k].stream);
     c->mc[k].release();
@@ -1126,6 +1141,7 @@ int vd_grid_populate(vd_ctx* c, const vd_box* box) {
     HIP_CHECK(hipEventRecord(c->ev1, c->stream));
     HIP_CHECK(hipStreamSynchronize(c->stream));
     c->grid_ms = elapsed_ms(c);
+    make_corners(c);
     c->maps_ready = true;
     c->nc_ready = false;
   });
@@ -1151,6 +1167,7 @@ int vd_grid_upload(vd_ctx* c, const vd_box* box, const float* maps) {
     c->gd = make_desc(*box);
     c->maps.ensure((size_t)VD_XS_SIZE * c->gd.plane);
     HIP_CHECK(hipMemcpy(c->maps.p, maps, (size_t)VD_XS_SIZE * c->gd.plane * 4, hipMemcpyHostToDevice));
+    make_corners(c);
     c->maps_ready = true;
     c->nc_ready = false;
   });

@@ -166,8 +166,14 @@ struct Scratch {               // per chain, LDS
   float4* nacc;                // 2N: per-node force and moment sums
   float* stash;                // per wave: conformations parked out of registers, [slot][field][lane]
 };
+// Maps as the searches read them (corner records): per type, one 32-byte record per grid cell (cell (i,j,k) spans
+// points i..i+1, j..j+1, k..k+1), [type][k][j][i], holding the cell's eight corner values as
+// {f000, f100, f010, f110} {f001, f101, f011, f111}.  One lookup is then two 16-byte loads from
+// one cache line instead of four row gathers.
 struct EvalK {
-  const float* __restrict__ maps; vd_grid_desc gd;
+  const float4* __restrict__ corners; vd_grid_desc gd;
+  int cd0, cd01;                 // cells per x-row, per z-plane
+  long long cplane;              // cells per type
   const float4* __restrict__ lutq;
   float v_intra, v_inter, slope;
 };
@@ -183,10 +189,11 @@ __device__ __forceinline__ float mul_rn(float a, float b) {
   return a * b;
 }
 
-// Trilinear map value and derivative at p (grid::evaluate_aux, V11/lib/grid.cpp:50-167).
+// Trilinear map value and derivative at p (grid::evaluate_aux, V11/lib/grid.cpp:50-167), from the
+// corner records of one type (cd0, cd01: cells per x-row, per z-plane).
 // prel = position relative to the map origin.  NaN/inf inputs stay in bounds.
-__device__ __forceinline__ float map_eval(const float* __restrict__ m, const vd_grid_desc& gd, float3 prel,
-                                          float slope, float v, float3& deriv) {
+__device__ __forceinline__ float map_eval(const float4* __restrict__ m, int cd0, int cd01, const vd_grid_desc& gd,
+                                          float3 prel, float slope, float v, float3& deriv) {
   float s[3] = {prel.x * gd.factor[0], prel.y * gd.factor[1], prel.z * gd.factor[2]};
   float miss[3]; int region[3], a[3];
 #pragma unroll
@@ -201,10 +208,10 @@ __device__ __forceinline__ float map_eval(const float* __restrict__ m, const vd_
     }
   }
   const float penalty = slope * (miss[0] * gd.factor_inv[0] + miss[1] * gd.factor_inv[1] + miss[2] * gd.factor_inv[2]);
-  const int d0 = gd.dim[0], d01 = gd.dim[0] * gd.dim[1];
-  const float* b = m + (a[0] + d0 * (a[1] + gd.dim[1] * a[2]));
-  const float f000 = b[0], f100 = b[1], f010 = b[d0], f110 = b[d0 + 1];
-  const float f001 = b[d01], f101 = b[d01 + 1], f011 = b[d01 + d0], f111 = b[d01 + d0 + 1];
+  const float4* b = m + 2 * ((long long)cd01 * a[2] + cd0 * a[1] + a[0]);
+  const float4 lo = b[0], hi = b[1];
+  const float f000 = lo.x, f100 = lo.y, f010 = lo.z, f110 = lo.w;
+  const float f001 = hi.x, f101 = hi.y, f011 = hi.z, f111 = hi.w;
   const float x = s[0], y = s[1], z = s[2], mx = 1 - x, my = 1 - y, mz = 1 - z;
   float f = f000 * mx * my * mz + f100 * x * my * mz + f010 * mx * y * mz + f110 * x * y * mz +
             f001 * mx * my * z + f101 * x * my * z + f011 * mx * y * z + f111 * x * y * z;
@@ -466,8 +473,8 @@ __device__ float eval_deriv(const LigView& L, const Scratch& S, const EvalK& K, 
         heavy[u] = (meta >> 16) & 1u;
         const int xs = (meta >> 8) & 255;
         if (xs < VD_XS_SIZE)
-          ev[u] = map_eval(K.maps + (size_t)xs * K.gd.plane, K.gd, make_float3(prel.x + y[u].x, prel.y + y[u].y, prel.z + y[u].z),
-                           K.slope, K.v_inter, d[u]);
+          ev[u] = map_eval(K.corners + 2 * xs * K.cplane, K.cd0, K.cd01, K.gd,
+                           make_float3(prel.x + y[u].x, prel.y + y[u].y, prel.z + y[u].z), K.slope, K.v_inter, d[u]);
       }
     }
 #pragma unroll

@@ -68,6 +68,9 @@ int mc_lds_bytes(int G, int Amax, int Nmax, int dense);
 int max_resident_mc_blocks(int G, int W, int mode, int lds_bytes);
 hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& gc, const float4* lut_rows, float* maps,
                        const vd_grid_desc& gd, hipStream_t s);
+// Corner records of the maps (VD_XS_SIZE * (dim0-1)(dim1-1)(dim2-1) * 2 float4) from the
+// canonical maps, both on the device.
+hipError_t launch_corners(const float* maps, float4* corners, const vd_grid_desc& gd, hipStream_t s);
 hipError_t launch_eval(int G, const EvalArgs& a, int blocks, hipStream_t s);
 hipError_t launch_mc(int G, int W, const McArgs& a, int blocks, hipStream_t s);
 hipError_t reset_counter(int* p, hipStream_t s);

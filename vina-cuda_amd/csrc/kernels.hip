@@ -121,6 +121,25 @@ __global__ __launch_bounds__(64) void vd_grid_kernel(const float4* __restrict__ 
   }
 }
 
+// Corner records of the maps (the layout the searches read, described at EvalK): thread per
+// (type, cell), x fastest; reads eight values of the canonical maps, writes 32 bytes.
+__global__ __launch_bounds__(256) void vd_corner_kernel(const float* __restrict__ maps, float4* __restrict__ corners,
+                                                        vd_grid_desc gd, long long total) {
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= total) return;
+  const int c0 = gd.dim[0] - 1, c1 = gd.dim[1] - 1, c2 = gd.dim[2] - 1;
+  const long long per_type = (long long)c0 * c1 * c2;
+  const int t = (int)(id / per_type);
+  long long r = id - (long long)t * per_type;
+  const int i = (int)(r % c0); r /= c0;
+  const int j = (int)(r % c1);
+  const int k = (int)(r / c1);
+  const long long d0 = gd.dim[0], d01 = (long long)gd.dim[0] * gd.dim[1];
+  const float* b = maps + (long long)t * gd.plane + i + d0 * j + d01 * k;
+  corners[2 * id] = make_float4(b[0], b[1], b[d0], b[d0 + 1]);
+  corners[2 * id + 1] = make_float4(b[d01], b[d01 + 1], b[d01 + d0], b[d01 + d0 + 1]);
+}
+
 // ------------------------------------------------------------------------------------------
 // LDS layout of one wave: ligand (atoms 4A, node_o 4N, node_a 4N floats), the conformation
 // stash, then per chain group the node area 12N, xyz 4A, frc 4A, the pair-force exchange row
@@ -637,6 +656,13 @@ hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& 
   const float* lut_last = reinterpret_cast<const float*>(lut_rows) + (size_t)VD_XS_SIZE * VD_LUT_BINS * VD_GRID_ROW;
   hipLaunchKernelGGL(vd_grid_kernel, dim3(tx * ty * tz), dim3(64), 0, s, atoms, cells, gc, lut_rows, lut_last, maps, gd,
                      tx, ty);
+  return hipGetLastError();
+}
+
+hipError_t launch_corners(const float* maps, float4* corners, const vd_grid_desc& gd, hipStream_t s) {
+  const long long total = (long long)VD_XS_SIZE * (gd.dim[0] - 1) * (gd.dim[1] - 1) * (gd.dim[2] - 1);
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(vd_corner_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, maps, corners, gd, total);
   return hipGetLastError();
 }
 
