@@ -200,17 +200,20 @@ def traffic_from_profile(kernel, algorithmic_bytes):
 
 
 def profile_figures(kernel):
-    """VALU busy share and resident waves per CU that the committed counter profile measured for
-    `kernel` on this workload (None when it has no figure for it)."""
+    """What the committed counter profile of this workload measured for `kernel`, run alone: VALU
+    busy share, resident waves per CU, TA and TD busy shares (the vector-memory path) and the L1
+    and L2 hit rates (None when it has no figure for it)."""
     path = os.path.join(ROOT, "profiles", "vd_mc_kernel_batched_screening.md")
     if not os.path.exists(path):
         return None
+    out = {}
     for line in open(path):
-        if line.startswith("<!-- valu:"):
-            fields = dict(f.split("=", 1) for f in line.strip("<!-> \n").split()[1:] if "=" in f)
-            if fields.get("kernel") == kernel.replace(" ", ""):
-                return {"valu_busy": float(fields["valu_busy"]), "waves_per_cu": float(fields["waves_per_cu"])}
-    return None
+        for tag in ("<!-- valu:", "<!-- vmem:"):
+            if line.startswith(tag):
+                fields = dict(f.split("=", 1) for f in line.strip("<!-> \n").split()[1:] if "=" in f)
+                if fields.pop("kernel", None) == kernel.replace(" ", ""):
+                    out.update({k: float(v) for k, v in fields.items()})
+    return out or None
 
 
 def grid_terms(xyz, xs, box):
@@ -436,6 +439,10 @@ def main():
     ap.add_argument("--no-post", action="store_true", help="skip the refinement + final-score leg")
     ap.add_argument("--grid-only", action="store_true",
                     help="only the 17-map precompute: screening scene x5, large-box scene x3 (for a kernel profile)")
+    ap.add_argument("--classes-alone", action="store_true",
+                    help="dock each lanes-per-chain class of the screening batch by itself (one launch each, "
+                         "with the grid it has alone), print the kernel times and exit: the run that counter "
+                         "profiles of one class at a time use")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, copy 256 MiB device to device twice (reference dispatch for HBM counters)")
     args = ap.parse_args()
@@ -463,6 +470,24 @@ def main():
     texts = screening_ligands(vdock, args.ligands)
     ids = [ctx.add_ligand(t) for t in texts]
     params = vdock.default_params(chains=args.chains, seed=7)
+
+    if args.classes_alone:
+        # the launch-shape table puts ligands under 80 atoms with at most 9 torsions in the 16-lane
+        # class and the rest in the 64-lane class; each subset must run as one launch
+        infos = [ctx.ligand_info(i) for i in ids]
+        narrow = [i for i, f in zip(ids, infos) if f.n_atoms < 80 and f.n_torsions <= 9]
+        wide = [i for i, f in zip(ids, infos) if not (f.n_atoms < 80 and f.n_torsions <= 9)]
+        res = {}
+        for sub in (narrow, wide):
+            st = ctx.dock_batch(sub, params)
+            L = ctx.last_launches()
+            kinds = sorted({r.kernel for r in L})
+            assert len(kinds) == 1, kinds
+            res[kinds[0]] = {"ligands": len(sub), "kernel_ms": round(st.kernel_ms, 3), "evals": int(st.evals),
+                             "blocks": [r.blocks for r in L]}
+            log(f"{kinds[0]}: {res[kinds[0]]}")
+        print(json.dumps({"classes_alone": res}), flush=True)
+        return
 
     for w in range(args.warmup):
         log(f"warmup {w}")
@@ -529,8 +554,11 @@ def main():
                                          "algorithmic byte from the committed FETCH_SIZE/WRITE_SIZE profile",
                        "hbm_traffic_GBs": round(traffic / (avg_span / 1000.0) / 1e9, 1) if traffic else None,
                        "hbm_traffic_frac": round(traffic / (avg_span / 1000.0) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
-                       "binding_resource": "VALU issue and memory latency (no dense contraction, so no MFMA bound)",
-                       "valu_busy_by_instantiation_profiled": valu,
+                       "binding_resource": "the vector-memory data path (TA/TD) that serves the gathers of map "
+                                           "corner records and pair-table quads through L1/L2; the 64-lane "
+                                           "instantiation is also VALU-bound (no dense contraction, so no MFMA "
+                                           "bound)",
+                       "counters_by_instantiation_profiled": valu,
                        "measured_copy_GBs": round(copy_gbs, 1),
                        "frac_of_measured_copy": round(achieved / copy_gbs, 5),
                        "steps": len(spans), "avg_span_ms": round(avg_span, 3),
