@@ -784,7 +784,8 @@ void check_post(const vd_post_params* p) {
 
 // FP64 ligand record of the refinement (vdr::RefHeader): atom words (local xyz, node | xs << 8 |
 // heavy << 16), node origin words (rel_origin, (parent + 1) | begin << 8 | end << 16), node axis
-// words (rel_axis, subtree end); pair words in the conflict-free chunks of 64.
+// words (rel_axis, subtree end | depth << 8 | (last child + 1) << 16 | (previous sibling + 1) << 24);
+// pair words in the conflict-free chunks of 64.
 vdr::RefHeader pack_ref(const vd::Ligand& L, std::vector<double4>& geo, std::vector<uint32_t>& pw, int max_steps) {
   vdr::RefHeader h{};
   h.A = (int)L.atoms.size(); h.N = (int)L.nodes.size(); h.T = L.n_torsions(); h.heavy = L.n_heavy;
@@ -797,8 +798,21 @@ vdr::RefHeader pack_ref(const vd::Ligand& L, std::vector<double4>& geo, std::vec
   for (const vd::LigNode& nd : L.nodes)
     geo.push_back(make_double4(nd.rel_origin[0], nd.rel_origin[1], nd.rel_origin[2],
                                (double)((uint32_t)(nd.parent + 1) | ((uint32_t)nd.begin << 8) | ((uint32_t)nd.end << 16))));
-  for (const vd::LigNode& nd : L.nodes)
-    geo.push_back(make_double4(nd.rel_axis[0], nd.rel_axis[1], nd.rel_axis[2], (double)nd.subtree_end));
+  // tree links of the level-by-level kinematics and adjoint fold (nodes are in preorder, so a
+  // parent precedes its children)
+  const int N = (int)L.nodes.size();
+  std::vector<int> last_child(N, -1), prev_sibling(N, -1);
+  for (int k = 1; k < N; ++k) {
+    const int par = L.nodes[k].parent;
+    prev_sibling[k] = last_child[par];
+    last_child[par] = k;
+  }
+  for (int k = 0; k < N; ++k) {
+    const vd::LigNode& nd = L.nodes[k];
+    const uint32_t links = (uint32_t)nd.subtree_end | ((uint32_t)nd.depth << 8) | ((uint32_t)(last_child[k] + 1) << 16) |
+                           ((uint32_t)(prev_sibling[k] + 1) << 24);
+    geo.push_back(make_double4(nd.rel_axis[0], nd.rel_axis[1], nd.rel_axis[2], (double)links));
+  }
   const std::vector<uint32_t>& ch = vd::pair_chunks(L, 64);
   h.C = (int)(ch.size() / 64);
   h.off_pairs = (long long)pw.size();

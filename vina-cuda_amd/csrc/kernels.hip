@@ -596,6 +596,7 @@ __global__ __launch_bounds__(64) void vd_refine_kernel(RefineArgs a) {
   P.pairs = a.pairs + h.off_pairs;
   P.A = h.A; P.N = h.N; P.T = h.T; P.C = h.C; P.n = 6 + h.T;
   for (int i = l; i < h.A; i += 64) const_cast<double4*>(P.atom)[i] = a.geo[h.off_atoms + i];
+  vdr::load_links(P, l);
   const double* ci = a.conf_in + a.conf_off[pose];
   vdr::Conf x;
   x.pos = vdr::D3{ci[0], ci[1], ci[2]};

@@ -5,9 +5,11 @@
 // minimum its BFGS line search decides on energy changes of 1e-7 kcal/mol, finer than FP32 atom
 // positions resolve, so this path computes in FP64 throughout (the MI355X vector ALUs issue FP64
 // FMAs at the rate of unpacked FP32) and keeps the reference's operation order where that is
-// cheap: kinematics and the adjoint run serially in preorder on one lane, BFGS dot products and
-// matrix rows are summed in index order, and nothing is contracted into FMAs, so the optimiser's
-// decisions match an FP64 CPU run except where a summation order differs.
+// cheap: each node's kinematics and each parent's fold of its children take the same operations
+// in the same order as the reference's preorder walks (the nodes of one tree depth at once, one
+// lane per node), BFGS dot products and matrix rows are summed in index order, and nothing is
+// contracted into FMAs, so the optimiser's decisions match an FP64 CPU run except where a
+// summation order differs.
 //
 // Lane layout (lane l of the wavefront):
 //   change vector (n = 6 + T <= 64):  component j in lane j -- 0-2 position, 3-5 orientation,
@@ -108,6 +110,14 @@ __device__ __forceinline__ D3 mat_vec(const double* r, D3 v) {
 }
 __device__ __forceinline__ uint32_t meta_of(double w) { return (uint32_t)w; }
 
+// The value of v in lane j (j wave-uniform), in every lane.
+__device__ __forceinline__ double lane_value(double v, int j) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, j);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), j);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Butterfly sum over `width` lanes (a power of two up to 64); every lane ends with the same sum.
 template <int WIDTH>
 __device__ __forceinline__ double lane_sum(double v) {
@@ -117,18 +127,18 @@ __device__ __forceinline__ double lane_sum(double v) {
 }
 
 // LDS of one pose (doubles): atoms 4A, frames 20N (R row-major 9, origin 3, lab axis 3, q 4, pad),
-// xyz 4A, frc 4A, node sums 8N, torsion half-angle cosines 64 and sines 64, gradient 64, three
-// vector buffers 3 x 64, H n x n.
+// xyz 4A, frc 4A, node sums 8N, tree links N (one int per node: previous sibling + 1), torsion
+// half-angle cosines 64 and sines 64, gradient 64, H n x n.
 struct Pose {
   const double4* atom;      // LDS copy of the atom words
   const double4* node_o;    // global: rel_origin xyz + (parent + 1) | begin << 8 | end << 16
   const double4* node_a;    // global: rel_axis xyz + subtree end
   const uint32_t* pairs;    // global
   int A, N, T, C, n;
-  double* frame; double4* xyz; double4* frc; double4* nsum;
-  double* tcos; double* tsin; double* grad; double* vb; double* H;
+  double* frame; double4* xyz; double4* frc; double4* nsum; int* sib;
+  double* tcos; double* tsin; double* grad; double* H;
 };
-__host__ __device__ inline int pose_lds_doubles(int A, int N, int n) { return 12 * A + 28 * N + 6 * 64 + n * n; }
+__host__ __device__ inline int pose_lds_doubles(int A, int N, int n) { return 12 * A + 29 * N + 3 * 64 + n * n; }
 
 __device__ inline void carve(Pose& P, double* lds, int Amax, int Nmax) {
   double* p = lds;
@@ -137,35 +147,60 @@ __device__ inline void carve(Pose& P, double* lds, int Amax, int Nmax) {
   P.xyz = reinterpret_cast<double4*>(p); p += 4 * Amax;
   P.frc = reinterpret_cast<double4*>(p); p += 4 * Amax;
   P.nsum = reinterpret_cast<double4*>(p); p += 8 * Nmax;
+  P.sib = reinterpret_cast<int*>(p); p += Nmax;
   P.tcos = p; p += 64;
   P.tsin = p; p += 64;
   P.grad = p; p += 64;
-  P.vb = p; p += 3 * 64;
   P.H = p;
 }
 
 // Conformation: position and orientation in every lane, torsion k in lane 6 + k.
 struct Conf { D3 pos; DQ q; double tor; };
 
+// Tree links of node k: depth, last child and previous sibling (-1: none).
+struct Links { int depth, last_child, prev_sibling; };
+__device__ __forceinline__ Links links_of(const Pose& P, int k) {
+  const uint32_t w = meta_of(P.node_a[k].w);
+  return Links{(int)((w >> 8) & 255u), (int)((w >> 16) & 255u) - 1, (int)(w >> 24) - 1};
+}
+// Copies every node's previous sibling into LDS (the adjoint fold walks siblings there).  Call
+// once per pose, before the first evaluation.
+__device__ inline void load_links(const Pose& P, int l) {
+  if (l < P.N) P.sib[l] = links_of(P, l).prev_sibling;
+  __syncthreads();
+}
+// Depth of the deepest node (wave-uniform).
+__device__ __forceinline__ int max_depth(const Pose& P, int l) {
+  const int d = l < P.N ? links_of(P, l).depth : 0;
+  int D = 0;
+  while (__any(d > D)) ++D;
+  return D;
+}
+
 // model::set for one pose (tree.h:205-209, 282-292 as the oracle's set_conf restates it):
-// lane 0 walks the nodes in preorder, q_k = angle_to_quaternion(R_parent rel_axis, theta_k) *
-// q_parent (normalize_approx), o_k = o_parent + R_parent rel_origin; then every lane places its
-// atoms, x_i = o_node + R_node local_i.  The half-angle cosine and sine of angle_to_quaternion
-// are computed beforehand by each torsion's own lane, so the serial walk only reads them.
-// Leaves frames and absolute coordinates in LDS.
+// q_k = angle_to_quaternion(R_parent rel_axis, theta_k) * q_parent (normalize_approx),
+// o_k = o_parent + R_parent rel_origin, node by node from the root: lane k computes node k once
+// every node of the depth above is done (the operations of the reference's preorder walk, each
+// node's from its parent's frame); then every lane places its atoms, x_i = o_node + R_node
+// local_i.  The half-angle cosine and sine of angle_to_quaternion are computed beforehand by each
+// torsion's own lane.  Leaves frames and absolute coordinates in LDS.
 __device__ inline void set_coords(const Pose& P, const Conf& c, int l) {
   if (l >= 6 && l < 6 + P.T) {
     const double a = normalize_angle(c.tor);
     P.tcos[l - 6] = cos(a / 2);
     P.tsin[l - 6] = sin(a / 2);
   }
-  __syncthreads();
   if (l == 0) {
     double* f = P.frame;
     qmat(c.q, f);
     f[9] = c.pos.x; f[10] = c.pos.y; f[11] = c.pos.z; f[12] = 0; f[13] = 0; f[14] = 0;
     f[15] = c.q.a; f[16] = c.q.b; f[17] = c.q.c; f[18] = c.q.d;
-    for (int k = 1; k < P.N; ++k) {
+  }
+  const int depth = l < P.N ? links_of(P, l).depth : 0;
+  for (int d = 1; __any(d <= depth); ++d) {
+    __syncthreads();
+    if (l >= 1 && l < P.N && depth == d) {
+      const int k = l;
       const double4 no = P.node_o[k], na = P.node_a[k];
       const int par = (int)(meta_of(no.w) & 255u) - 1;
       const double* fp = P.frame + 20 * par;
@@ -321,7 +356,9 @@ __device__ inline double pair_terms(const Pose& P, const Consts& K, int l) {
 // model::eval_deriv over non_cache with the intramolecular pairs (model.cpp:1324-1334) at the
 // given slope.  Returns e (wave-uniform); g = this lane's gradient component (0 beyond n).
 // The adjoint follows the oracle's order: per-node force and torque sums over the node's atoms
-// in index order, then lane 0 folds children into parents in reverse preorder.
+// in index order, then every parent folds in its children in decreasing index order, which is
+// the order the reference's reverse-preorder walk adds them in; parents one depth at a time from
+// the deepest, so a child is complete before it is folded.
 __device__ inline double eval_deriv(const Pose& P, const Consts& K, double slope, const Conf& c, int l, double& g) {
   set_coords(P, c, l);
   double e = nc_inter<false>(P, K, slope, l);
@@ -344,21 +381,26 @@ __device__ inline double eval_deriv(const Pose& P, const Consts& K, double slope
     P.nsum[2 * l] = make_double4(F.x, F.y, F.z, 0.0);
     P.nsum[2 * l + 1] = make_double4(T.x, T.y, T.z, 0.0);
   }
+  const Links lk = l < P.N ? links_of(P, l) : Links{-1, -1, -1};
+  for (int d = max_depth(P, l) - 1; d >= 0; --d) {
+    __syncthreads();
+    if (l < P.N && lk.depth == d && lk.last_child >= 0) {
+      const double* fp = P.frame + 20 * l;
+      double4 Fp = P.nsum[2 * l], Tp = P.nsum[2 * l + 1];
+      for (int k = lk.last_child; k >= 0; k = P.sib[k]) {
+        const double* f = P.frame + 20 * k;
+        const double4 Fk = P.nsum[2 * k], Tk = P.nsum[2 * k + 1];
+        P.grad[6 + k - 1] = dot(D3{Tk.x, Tk.y, Tk.z}, D3{f[12], f[13], f[14]});
+        const D3 arm = sub(D3{f[9], f[10], f[11]}, D3{fp[9], fp[10], fp[11]});
+        const D3 c = cross(arm, D3{Fk.x, Fk.y, Fk.z});
+        Fp.x += Fk.x; Fp.y += Fk.y; Fp.z += Fk.z;
+        Tp.x = (Tp.x + c.x) + Tk.x; Tp.y = (Tp.y + c.y) + Tk.y; Tp.z = (Tp.z + c.z) + Tk.z;
+      }
+      P.nsum[2 * l] = Fp; P.nsum[2 * l + 1] = Tp;
+    }
+  }
   __syncthreads();
   if (l == 0) {
-    for (int k = P.N - 1; k >= 1; --k) {
-      const double* f = P.frame + 20 * k;
-      const int par = (int)(meta_of(P.node_o[k].w) & 255u) - 1;
-      const double* fp = P.frame + 20 * par;
-      const double4 Fk = P.nsum[2 * k], Tk = P.nsum[2 * k + 1];
-      P.grad[6 + k - 1] = dot(D3{Tk.x, Tk.y, Tk.z}, D3{f[12], f[13], f[14]});
-      const D3 arm = sub(D3{f[9], f[10], f[11]}, D3{fp[9], fp[10], fp[11]});
-      const D3 c = cross(arm, D3{Fk.x, Fk.y, Fk.z});
-      double4 Fp = P.nsum[2 * par], Tp = P.nsum[2 * par + 1];
-      Fp.x += Fk.x; Fp.y += Fk.y; Fp.z += Fk.z;
-      Tp.x = (Tp.x + c.x) + Tk.x; Tp.y = (Tp.y + c.y) + Tk.y; Tp.z = (Tp.z + c.z) + Tk.z;
-      P.nsum[2 * par] = Fp; P.nsum[2 * par + 1] = Tp;
-    }
     const double4 F0 = P.nsum[0], T0 = P.nsum[1];
     P.grad[0] = F0.x; P.grad[1] = F0.y; P.grad[2] = F0.z; P.grad[3] = T0.x; P.grad[4] = T0.y; P.grad[5] = T0.z;
   }
@@ -382,25 +424,23 @@ __device__ inline Conf increment(const Conf& x, double d, double factor, int l, 
   return y;
 }
 
-// Dot product of two lane vectors summed in index order (every lane gets the same value).
+// Dot product of two lane vectors summed in index order (every lane gets the same value): each
+// lane's product, then the products of lanes 0 .. n-1 added in turn.
 __device__ inline double ordered_dot(const Pose& P, double a, double b, int l) {
-  __syncthreads();
-  P.vb[l] = a; P.vb[64 + l] = b;
-  __syncthreads();
+  const double ab = a * b;
   double s = 0;
-  for (int j = 0; j < P.n; ++j) s += P.vb[j] * P.vb[64 + j];
-  __syncthreads();
+  for (int j = 0; j < P.n; ++j) s += lane_value(ab, j);
   return s;
 }
-// -(H v) for this lane's row, summed in index order.
+// -(H v) for this lane's row, summed in index order.  The rows are read after a fence, so every
+// update of H before the call is seen.
 __device__ inline double minus_hv(const Pose& P, double v, int l) {
   __syncthreads();
-  P.vb[128 + l] = v;
-  __syncthreads();
   double s = 0;
-  if (l < P.n)
-    for (int j = 0; j < P.n; ++j) s += P.H[l * P.n + j] * P.vb[128 + j];
-  __syncthreads();
+  for (int j = 0; j < P.n; ++j) {
+    const double vj = lane_value(v, j);
+    if (l < P.n) s += P.H[l * P.n + j] * vj;
+  }
   return -s;
 }
 
@@ -450,15 +490,14 @@ __device__ inline double bfgs(const Pose& P, const Consts& K, double slope, Conf
     const double r = 1 / (alpha * yp);
     // lane l updates H[l][j] for j >= l and mirrors it, as the reference's packed triangle
     __syncthreads();
-    P.vb[l] = mhy; P.vb[64 + l] = p;
-    __syncthreads();
-    if (l < n)
-      for (int j = l; j < n; ++j) {
-        const double h = P.H[l * n + j] + (alpha * r * (mhy * P.vb[64 + j] + P.vb[j] * p) +
-                                           alpha * alpha * (r * r * yhy + r) * p * P.vb[64 + j]);
+    for (int j = 0; j < n; ++j) {
+      const double pj = lane_value(p, j), mhyj = lane_value(mhy, j);
+      if (l < n && j >= l) {
+        const double h = P.H[l * n + j] + (alpha * r * (mhy * pj + mhyj * p) + alpha * alpha * (r * r * yhy + r) * p * pj);
         P.H[l * n + j] = h;
         P.H[j * n + l] = h;
       }
+    }
     __syncthreads();
   }
   if (!(f0 <= f_orig)) { f0 = f_orig; x = x_orig; }
