@@ -70,12 +70,18 @@ int vd_box_from_center(const double center[3], const double size[3], double gran
 
 /* All 17 XS maps over the box on the GPU (replaces kernel_grid, V11/inc/cuda/kernel1.cuh:30-41,
  * and cache::populate, V11/lib/cache.cpp:148-208).  Maps stay resident on the device in the
- * plain FP32 layout map[t][z][y][x]. */
+ * plain FP32 layout map[t][z][y][x], beside the copy the searches read: per map and grid cell,
+ * the cell's eight corner values (32 bytes), built from the plain maps by the same call.  The
+ * device then holds 17*4*(n0+1)*(n1+1)*(n2+1) + 17*32*n0*n1*n2 bytes of maps (314 MB for an
+ * 80^3-step box); VD_ERR_HIP if that does not fit. */
 int vd_grid_populate(vd_ctx* ctx, const vd_box* box);
-/* Kernel time of the last vd_grid_populate in milliseconds. */
+/* Kernel time of the last vd_grid_populate in milliseconds (the map precompute, without the
+ * corner copy). */
 double vd_grid_last_ms(const vd_ctx* ctx);
 /* Host copies of the resident maps: 17*(n0+1)*(n1+1)*(n2+1) floats. */
 int vd_grid_download(vd_ctx* ctx, float* maps);
+/* Maps from the host in the plain layout (as vd_grid_download returns them); builds the corner
+ * copy as vd_grid_populate does. */
 int vd_grid_upload(vd_ctx* ctx, const vd_box* box, const float* maps);
 
 /* Ligands (parse_ligand_pdbqt, V11/lib/parse_pdbqt.cpp:618-627, flattened for the device as
