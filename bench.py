@@ -282,6 +282,35 @@ def plain_accuracy(vdock, ctx, log, ids, chains, seeds=(7, 8, 9, 10), every=8):
             "top_pose_within_2A_fraction": round(float((rms <= 2.0).mean()), 4)}
 
 
+def graph_leg(vdock, ctx, log, ids, chains, calls=3):
+    """Batched docking as a captured hipGraph against the eager calls: the whole screening batch
+    and its first 11 ligands (a small batch, where host work weighs most).  Call 0 of each mode is
+    the eager warm call or the graph's capture; the next `calls` use new seeds (the graph's kernel
+    nodes updated), so every call docks anew.  The results of a replay equal the eager path's bit
+    for bit (tests/test_gpu_graph.py)."""
+    out = {}
+    for name, sub in (("screening", ids), ("first_11_ligands", ids[:11])):
+        r = {"ligands": len(sub)}
+        for mode in ("eager", "graph"):
+            ctx.set_graphs(mode == "graph")
+            walls, kms = [], []
+            for k in range(calls + 1):
+                log(f"graph leg: {name} {mode} call {k}")
+                p = vdock.default_params(chains=chains, seed=100 + k)
+                t0 = time.perf_counter()
+                st = ctx.dock_batch(sub, p)
+                walls.append(1000.0 * (time.perf_counter() - t0))
+                kms.append(st.kernel_ms)
+            r[mode] = {"first_call_ms": round(walls[0], 3), "ms_per_call": round(float(np.median(walls[1:])), 3),
+                       "kernel_ms": round(float(np.median(kms[1:])), 3),
+                       "host_ms": round(float(np.median(np.array(walls[1:]) - np.array(kms[1:]))), 3)}
+        ctx.set_graphs(False)
+        r["ligands_per_s_eager"] = round(len(sub) / (r["eager"]["ms_per_call"] / 1000.0), 2)
+        r["ligands_per_s_graph"] = round(len(sub) / (r["graph"]["ms_per_call"] / 1000.0), 2)
+        out[name] = r
+    return out
+
+
 def single_ligand_leg(vdock, ctx, log):
     """One 32-heavy-atom, 8-torsion ligand with 8,192 chains through the plain per-ligand launch
     (the drop-in for kernel_monte) on the screening scene's maps."""
@@ -635,6 +664,7 @@ def main():
                           "ms_by_chains_per_wave": {str(k): round(v, 3) for k, v in rates.items()},
                           "finite_fraction": float(np.isfinite(e_host).mean())}
         out["single_ligand"] = single_ligand_leg(vdock, ctx, log)
+        out["graph"] = graph_leg(vdock, ctx, log, ids, args.chains)
         # the screening step with dense BFGS (--rilc_bfgs 0) and its own heuristics
         log("dense BFGS")
         pdn = vdock.default_params(chains=args.chains, seed=7, local_search=vdock.VD_LS_BFGS)

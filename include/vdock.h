@@ -179,6 +179,19 @@ int vd_last_launches(vd_ctx* ctx, int max, vd_launch* out);
  * or more, run one class after the other.  on = 0: always one after the other.  No counterpart
  * in the reference (one kernel_monte launch per ligand). */
 int vd_set_class_overlap(vd_ctx* ctx, int on);
+/* on = 1: vd_dock_batch runs as a captured hipGraph.  The first call for a set of ligands and
+ * parameters captures its stages -- uploads of the ligand records and launch tables (from pinned
+ * staging the graph owns), the work-queue and evaluation-count clears, the QuickVina-W buffer
+ * clear, the MC launch of every class (concurrent branches when the classes overlap) and the
+ * read-back of the evaluation counts -- and later calls with the same ligand ids (same order) and
+ * the same parameters replay it without packing or re-uploading anything on the host; the seed may
+ * differ between calls (the graph's kernel nodes are updated).  A replay gives the results of the
+ * eager path bit for bit.  Up to four captured batches are kept per context (least recently used
+ * dropped); a batch whose buffers another call has grown since is captured again, and batches of
+ * more than four classes run eagerly.  on = 0 (default) drops the captured graphs.  vd_last_launches
+ * then gives each launch the whole graph's span.  No counterpart in the reference (one
+ * kernel_monte launch per ligand, main_procedure_cu.cpp:527-956). */
+int vd_set_graphs(vd_ctx* ctx, int on);
 /* on = 0 (default): with chains_per_wave or waves_per_simd left at 0, each ligand's
  * (lanes per chain, waves per SIMD) pair comes from a fixed table per size bucket (atoms x
  * torsions), so that under the Vina and QuickVina2 policies the same inputs and seed give the same

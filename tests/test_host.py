@@ -161,6 +161,9 @@ def test_invalid_arguments(host_ctx):
     assert lib.vd_create(0, None) == vdock.VD_ERR_ARG
     b = vdock.Box()
     assert lib.vd_box_from_center(None, None, 0.375, ctypes.byref(b)) == vdock.VD_ERR_ARG
+    assert lib.vd_set_graphs(None, 1) == vdock.VD_ERR_ARG
+    assert lib.vd_set_graphs(host_ctx.h, 2) == vdock.VD_ERR_ARG
+    assert lib.vd_set_graphs(host_ctx.h, 0) == vdock.VD_OK
     with pytest.raises(vdock.VdockError):
         host_ctx.set_receptor(np.array([[np.nan, 0, 0]], dtype=np.float32), np.array([0], dtype=np.int32))
     with pytest.raises(vdock.VdockError):

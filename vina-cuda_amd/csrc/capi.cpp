@@ -15,6 +15,7 @@
 #include <limits>
 #include <tuple>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -120,6 +121,47 @@ struct McSlot {
 };
 constexpr int kMcSlots = 4;   // at most four hardware queues per process on these devices
 
+// A persistent batch captured as a hipGraph (vd_set_graphs): every stage of one vd_dock_batch
+// call -- the uploads of the ligand records and launch tables from pinned staging, the clears of
+// the work-queue counter and the evaluation counts (fill kernels, so the graph holds no memset
+// node), the QuickVina-W buffer clear, the MC launch of every class (two branches when the classes
+// overlap) and the read-back of the evaluation counts -- replayed when the same ligands are docked
+// again with the same parameters.  The seed may differ between replays: it is a field of the MC
+// kernels' argument, updated in the instantiated graph's kernel nodes.
+struct GraphClass {
+  int slot = 0;                       // launch slot whose device buffers the graph uses
+  McArgs a{};                         // the kernel argument as captured; P.seed follows the replays
+  hipGraphNode_t node = nullptr;      // its kernel node (null: not found, seed changes re-capture)
+  hipKernelNodeParams kp{};
+  void* kargs[1] = {nullptr};
+  vd_launch rec{};
+  std::vector<vd_lig_header> headers;
+  size_t evals_at = 0;                // first evaluation count of its ligands in GraphEntry::evals
+  std::vector<const void*> ptrs;      // the slot's device buffers at capture time
+};
+struct GraphEntry {
+  std::vector<int> ids;
+  vd_dock_params p{};                 // seed zeroed
+  std::vector<std::pair<int, int>> shapes;   // (lanes per chain, waves per SIMD) per ligand
+  bool overlap = false;
+  std::vector<const void*> outs;     // the batch's result buffers at capture time
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t seed = 0;                  // the seed the instantiated graph holds
+  std::vector<GraphClass> cls;
+  PinnedBuf<unsigned char> staging;   // every host-to-device copy's source
+  PinnedBuf<unsigned long long> evals;
+  GraphEntry() = default;
+  GraphEntry(const GraphEntry&) = delete;
+  GraphEntry& operator=(const GraphEntry&) = delete;
+  ~GraphEntry() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    staging.release(); evals.release();
+  }
+};
+constexpr size_t kMaxGraphs = 4;      // captured batches kept per context, least recently used dropped
+
 struct PostLigand {   // modes of one ligand after refinement and selection
   int id = 0;
   std::vector<vd_mode> modes;
@@ -155,6 +197,8 @@ struct vd_ctx {
   McSlot mc[kMcSlots];
   bool overlap_classes = true;               // batch classes run at the same time
   hipEvent_t ev_base = nullptr;
+  bool graphs_on = false;                    // persistent batches run as captured graphs (vd_set_graphs)
+  std::vector<std::unique_ptr<GraphEntry>> graphs;   // least recently used first
   DevBuf<float4> blob;                       // ligand record of vd_eval
   DevBuf<vd_lig_header> headers;
   DevBuf<float> io_a, io_b, io_c;
@@ -1009,6 +1053,7 @@ void vd_destroy(vd_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   c->lut_rows.release(); c->lutq.release(); c->maps.release(); c->corners.release(); c->grid_atoms.release();
   c->grid_cells.release();
+  c->graphs.clear();
   for (int k = 0; k < kMcSlots; ++k) {
     if (c->mc[k].stream) (void)hipStreamSynchronize(c->mc[k].stream);
     c->mc[k].release();
@@ -1474,6 +1519,244 @@ int vd_dock_noalloc(vd_ctx* c, int id, const vd_dock_params* p, const float* d_i
   return guard(c, [&]() { dock_noalloc(c, id, p, d_init_conf, d_e, d_conf, d_heavy, st); });
 }
 
+}  // extern "C"
+
+namespace {
+
+// The launch classes of a persistent batch: ligand indices per (lanes per chain, waves per SIMD).
+using Classes = std::map<std::pair<int, int>, std::vector<int>>;
+Classes batch_classes(const std::vector<LigPlan>& plans) {
+  Classes classes;
+  for (size_t k = 0; k < plans.size(); ++k) classes[std::make_pair(plans[k].G, plans[k].W)].push_back((int)k);
+  return classes;
+}
+
+// Overlap applies to batches of exactly two classes.  Measured on the screening batch (a 16-lane
+// class whose waves mostly wait, a 64-lane class that keeps the vector units busy): 3 % shorter
+// steps.  With more classes it lost: on a 4,096-ligand batch of five classes a fourth stream
+// shared a hardware queue with the third (four queues per process), and capped classes that
+// outlast the uncapped one keep half their CU slots empty to the end.
+bool overlap_of(const vd_ctx* c, const Classes& classes) { return c->overlap_classes && classes.size() == 2; }
+
+// Host work of one class of a persistent batch -- its ligand records, launch tables and work items
+// in decreasing estimated cost -- prepared into launch slot s.  share: the fraction of its resident
+// blocks per CU the class launches.  With overlap, every class but the last (the widest groups)
+// takes half, which leaves room for the last class's waves beside it; the last class launches its
+// full grid, whose blocks that do not fit at first take over the CU slots the others give up when
+// their queues run dry.  Measured on the screening batch (16- and 64-lane classes): 0.5 to 0.75 of
+// the residency are equally good, below 0.4 the first class runs too long.
+void prepare_class(vd_ctx* c, McSlot& s, int G, int W, const std::vector<int>& ks, const std::vector<LigPlan>& plans,
+                   const vd_dock_params* p, double share) {
+  std::vector<int> lig_ids, dv, bv;
+  for (int k : ks) { lig_ids.push_back(plans[k].id); dv.push_back(plans[k].depth); bv.push_back(plans[k].bfgs); }
+  Packed pk = pack(c, G, lig_ids, &dv, &bv);
+  std::vector<uint32_t> streams;
+  std::vector<longlong4> off;
+  for (int k : ks) { streams.push_back((uint32_t)plans[k].id); off.push_back(c->batch.off[k]); }
+  // cost estimate per chain: steps x iterations x (atoms + pairs)
+  std::vector<std::pair<double, int>> order;
+  for (size_t s = 0; s < ks.size(); ++s)
+    order.push_back({-(double)dv[s] * bv[s] * (pk.headers[s].n_atoms + 0.5 * pk.headers[s].n_chunks * G + 16), (int)s});
+  std::stable_sort(order.begin(), order.end());
+  const int cpw = 64 / G;
+  std::vector<int2> items;
+  for (auto& o : order)
+    for (int c0 = 0; c0 < p->chains; c0 += cpw) items.push_back(make_int2(o.second, c0));
+  if (!s.stream) { HIP_CHECK(hipStreamCreate(&s.stream)); s.own_stream = true; }
+  mc_prepare(c, s, G, W, std::move(pk), std::move(streams), std::move(items), std::move(off), make_consts(c, p, 0, 0),
+             c->batch.e.p, c->batch.conf.p, c->batch.heavy.p, true, true, share);
+}
+
+// Every device buffer of a launch slot that a captured graph reads or writes.  DevBuf::ensure
+// frees and reallocates only when it grows, so a replay is valid while these are unchanged.
+std::vector<const void*> slot_ptrs(const McSlot& s) {
+  return {s.blob.p, s.headers.p, s.streams.p, s.items.p, s.offs.p, s.queue.p, s.evals.p,
+          s.hist.p, s.glob.p, s.glob_off.p, s.glob_axis.p, s.glob_T.p};
+}
+
+// Captures the batch's stages (classes already prepared into slots 0..n-1) into a graph.
+std::unique_ptr<GraphEntry> capture_batch(vd_ctx* c, const Classes& classes, bool overlap) {
+  auto g = std::make_unique<GraphEntry>();
+  const int n = (int)classes.size();
+  // pinned staging: every host-to-device copy's source, 256-byte aligned pieces
+  size_t bytes = 0, n_evals = 0;
+  auto piece = [&](size_t b) { const size_t at = bytes; bytes += (b + 255) / 256 * 256; return at; };
+  struct At { size_t goff, gaxis, gT, blob, headers, streams, items, offs; };
+  std::vector<At> at(n);
+  for (int k = 0; k < n; ++k) {
+    const McSlot& s = c->mc[k];
+    at[k] = At{piece(s.h_goff.size() * 8), piece(s.h_gaxis.size() * 4), piece(s.h_gT.size() * 4),
+               piece(s.pk.blob.size() * 4), piece(s.h_headers.size() * sizeof(vd_lig_header)),
+               piece(s.h_streams.size() * 4), piece(s.h_items.size() * sizeof(int2)),
+               piece(s.h_offs.size() * sizeof(longlong4))};
+    n_evals += s.h_headers.size();
+  }
+  g->staging.ensure(bytes);
+  g->evals.ensure(n_evals);
+  unsigned char* base = g->staging.p;
+  size_t evals_at = 0;
+  for (int k = 0; k < n; ++k) {
+    const McSlot& s = c->mc[k];
+    auto put = [&](size_t off, const void* src, size_t b) { if (b) std::memcpy(base + off, src, b); };
+    put(at[k].goff, s.h_goff.data(), s.h_goff.size() * 8);
+    put(at[k].gaxis, s.h_gaxis.data(), s.h_gaxis.size() * 4);
+    put(at[k].gT, s.h_gT.data(), s.h_gT.size() * 4);
+    put(at[k].blob, s.pk.blob.data(), s.pk.blob.size() * 4);
+    put(at[k].headers, s.h_headers.data(), s.h_headers.size() * sizeof(vd_lig_header));
+    put(at[k].streams, s.h_streams.data(), s.h_streams.size() * 4);
+    put(at[k].items, s.h_items.data(), s.h_items.size() * sizeof(int2));
+    put(at[k].offs, s.h_offs.data(), s.h_offs.size() * sizeof(longlong4));
+    GraphClass gc;
+    gc.slot = k; gc.a = s.a; gc.rec = s.rec; gc.headers = s.h_headers; gc.evals_at = evals_at;
+    gc.ptrs = slot_ptrs(s);
+    evals_at += s.h_headers.size();
+    g->cls.push_back(std::move(gc));
+  }
+  hipStream_t main = c->stream;
+  HIP_CHECK(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
+  try {
+    if (overlap) {
+      HIP_CHECK(hipEventRecord(c->ev_base, main));
+      for (int k = 1; k < n; ++k) HIP_CHECK(hipStreamWaitEvent(c->mc[k].stream, c->ev_base, 0));
+    }
+    auto h2d = [&](hipStream_t st, void* dst, size_t off, size_t b) {
+      if (b) HIP_CHECK(hipMemcpyAsync(dst, base + off, b, hipMemcpyHostToDevice, st));
+    };
+    // every class's copies before any kernel, as in the eager path
+    for (int k = 0; k < n; ++k) {
+      McSlot& s = c->mc[k];
+      hipStream_t st = overlap ? s.stream : main;
+      if (!s.h_goff.empty()) {
+        h2d(st, s.glob_off.p, at[k].goff, s.h_goff.size() * 8);
+        h2d(st, s.glob_axis.p, at[k].gaxis, s.h_gaxis.size() * 4);
+        h2d(st, s.glob_T.p, at[k].gT, s.h_gT.size() * 4);
+        HIP_CHECK(vdl::glob_init(s.glob.p, s.glob_off.p, s.glob_axis.p, s.glob_T.p, (int)s.h_goff.size(), st));
+      }
+      h2d(st, s.blob.p, at[k].blob, s.pk.blob.size() * 4);
+      h2d(st, s.headers.p, at[k].headers, s.h_headers.size() * sizeof(vd_lig_header));
+      h2d(st, s.streams.p, at[k].streams, s.h_streams.size() * 4);
+      h2d(st, s.items.p, at[k].items, s.h_items.size() * sizeof(int2));
+      h2d(st, s.offs.p, at[k].offs, s.h_offs.size() * sizeof(longlong4));
+      HIP_CHECK(vdl::zero_u64(s.evals.p, (int)s.h_headers.size(), st));
+      HIP_CHECK(vdl::reset_counter(s.queue.p, st));
+    }
+    for (int k = 0; k < n; ++k) {
+      McSlot& s = c->mc[k];
+      HIP_CHECK(vdl::launch_mc(s.G, s.W, s.a, s.blocks, overlap ? s.stream : main));
+    }
+    if (overlap)
+      for (int k = 1; k < n; ++k) {
+        HIP_CHECK(hipEventRecord(c->mc[k].ev1, c->mc[k].stream));
+        HIP_CHECK(hipStreamWaitEvent(main, c->mc[k].ev1, 0));
+      }
+    for (int k = 0; k < n; ++k) {
+      const McSlot& s = c->mc[k];
+      HIP_CHECK(hipMemcpyAsync(g->evals.p + g->cls[k].evals_at, s.evals.p, s.h_headers.size() * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, main));
+    }
+  } catch (...) {
+    hipGraph_t partial = nullptr;
+    (void)hipStreamEndCapture(main, &partial);
+    if (partial) (void)hipGraphDestroy(partial);
+    throw;
+  }
+  HIP_CHECK(hipStreamEndCapture(main, &g->graph));
+  HIP_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  // the MC kernel node of each class, for seed updates: the launched function, and the captured
+  // argument's ligand records (classes with the same instantiation differ there)
+  size_t nn = 0;
+  HIP_CHECK(hipGraphGetNodes(g->graph, nullptr, &nn));
+  std::vector<hipGraphNode_t> nodes(nn);
+  if (nn) HIP_CHECK(hipGraphGetNodes(g->graph, nodes.data(), &nn));
+  for (hipGraphNode_t node : nodes) {
+    hipGraphNodeType type;
+    HIP_CHECK(hipGraphNodeGetType(node, &type));
+    if (type != hipGraphNodeTypeKernel) continue;
+    hipKernelNodeParams kp{};
+    HIP_CHECK(hipGraphKernelNodeGetParams(node, &kp));
+    for (GraphClass& gc : g->cls) {
+      if (gc.node || kp.func != vdl::mc_kernel_fn(gc.rec.lanes_per_chain, gc.rec.waves_per_simd, gc.a)) continue;
+      if (!kp.kernelParams || !kp.kernelParams[0] || static_cast<const McArgs*>(kp.kernelParams[0])->blob != gc.a.blob)
+        continue;
+      gc.node = node; gc.kp = kp;
+      break;
+    }
+  }
+  g->seed = g->cls.empty() ? 0 : g->cls[0].a.P.seed;
+  return g;
+}
+
+// Runs a persistent batch as a graph: replays the captured graph of the same ligands and
+// parameters when one is kept and its buffers are still those it captured, or captures one.
+// Returns false when the batch has more classes than launch slots (then it runs eagerly).
+bool graph_dock(vd_ctx* c, const std::vector<int>& ids, const std::vector<LigPlan>& plans, const Classes& classes,
+                const vd_dock_params* p, vd_stats* st) {
+  if (classes.size() > (size_t)kMcSlots) return false;
+  const bool overlap = overlap_of(c, classes);
+  vd_dock_params key = *p;
+  key.seed = 0;
+  std::vector<std::pair<int, int>> shapes;
+  for (const LigPlan& pl : plans) shapes.push_back(std::make_pair(pl.G, pl.W));
+  const std::vector<const void*> outs{c->batch.e.p, c->batch.conf.p, c->batch.heavy.p};
+  std::unique_ptr<GraphEntry> g;
+  for (size_t k = 0; k < c->graphs.size(); ++k) {
+    GraphEntry& e = *c->graphs[k];
+    if (e.ids != ids || std::memcmp(&e.p, &key, sizeof key) != 0 || e.shapes != shapes || e.overlap != overlap) continue;
+    bool valid = e.outs == outs;
+    for (const GraphClass& gc : e.cls) valid = valid && gc.ptrs == slot_ptrs(c->mc[gc.slot]);
+    bool nodes = true;   // a kernel node to update per class
+    for (const GraphClass& gc : e.cls) nodes = nodes && gc.node;
+    if (valid && (e.seed == p->seed || nodes)) g = std::move(c->graphs[k]);
+    c->graphs.erase(c->graphs.begin() + k);
+    break;
+  }
+  if (!g) {
+    int slot = 0;
+    for (auto& cl : classes) {
+      const bool last = std::next(classes.find(cl.first)) == classes.end();
+      prepare_class(c, c->mc[slot++], cl.first.first, cl.first.second, cl.second, plans, p, (overlap && !last) ? 0.5 : 1.0);
+    }
+    g = capture_batch(c, classes, overlap);
+    g->ids = ids; g->p = key; g->shapes = shapes; g->overlap = overlap; g->outs = outs;
+  }
+  if (g->seed != p->seed) {
+    for (GraphClass& gc : g->cls) {
+      gc.a.P.seed = p->seed;
+      gc.kargs[0] = &gc.a;
+      gc.kp.kernelParams = gc.kargs;
+      gc.kp.extra = nullptr;
+      HIP_CHECK(hipGraphExecKernelNodeSetParams(g->exec, gc.node, &gc.kp));
+    }
+    g->seed = p->seed;
+  }
+  HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+  HIP_CHECK(hipGraphLaunch(g->exec, c->stream));
+  HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  float ms = 0;
+  HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  for (const GraphClass& gc : g->cls) {
+    // launches of one graph are timed together: each record carries the graph's span
+    vd_launch rec = gc.rec;
+    rec.kernel_ms = ms; rec.start_ms = 0; rec.evals = 0; rec.eval_bytes = 0;
+    for (size_t k = 0; k < gc.headers.size(); ++k) {
+      const unsigned long long v = g->evals.p[gc.evals_at + k];
+      rec.evals += v;
+      rec.eval_bytes += (double)v * eval_bytes(gc.headers[k]);
+    }
+    c->launches.push_back(rec);
+    if (st) { st->evals += rec.evals; st->blocks += rec.blocks; st->chains_per_wave = 64 / rec.lanes_per_chain; }
+  }
+  if (st) st->kernel_ms += ms;
+  c->graphs.push_back(std::move(g));
+  if (c->graphs.size() > kMaxGraphs) c->graphs.erase(c->graphs.begin());
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
 static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, vd_stats* st, bool persistent) {
   if (!c || n < 0 || (n > 0 && !ids)) return VD_ERR_ARG;
   return guard(c, [&]() {
@@ -1506,14 +1789,9 @@ static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, 
     // one persistent launch per (group width, register budget) class; items ordered by decreasing
     // estimated cost.  With class overlap the launches are all prepared first (buffer growth may
     // synchronise the device), then enqueued on one stream each, then collected.
-    std::map<std::pair<int, int>, std::vector<int>> classes;
-    for (int k = 0; k < n; ++k) classes[std::make_pair(plans[k].G, plans[k].W)].push_back(k);
-    // Overlap applies to batches of exactly two classes.  Measured on the screening batch (a
-    // 16-lane class whose waves mostly wait, a 64-lane class that keeps the vector units busy):
-    // 3 % shorter steps.  With more classes it lost: on a 4,096-ligand batch of five classes a
-    // fourth stream shared a hardware queue with the third (four queues per process), and capped
-    // classes that outlast the uncapped one keep half their CU slots empty to the end.
-    const bool overlap = c->overlap_classes && classes.size() == 2;
+    const Classes classes = batch_classes(plans);
+    const bool overlap = overlap_of(c, classes);
+    if (c->graphs_on && graph_dock(c, std::vector<int>(ids, ids + n), plans, classes, p, st)) return;
     int used = 0;
     auto flush = [&]() {
       if (used == 0) return;
@@ -1532,36 +1810,9 @@ static int dock_many(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, 
       used = 0;
     };
     for (auto& cl : classes) {
-      const int G = cl.first.first, W = cl.first.second;
-      std::vector<int>& ks = cl.second;
-      std::vector<int> lig_ids, dv, bv;
-      for (int k : ks) { lig_ids.push_back(plans[k].id); dv.push_back(plans[k].depth); bv.push_back(plans[k].bfgs); }
-      Packed pk = pack(c, G, lig_ids, &dv, &bv);
-      std::vector<uint32_t> streams;
-      std::vector<longlong4> off;
-      for (int k : ks) { streams.push_back((uint32_t)plans[k].id); off.push_back(c->batch.off[k]); }
-      // cost estimate per chain: steps x iterations x (atoms + pairs)
-      std::vector<std::pair<double, int>> order;
-      for (size_t s = 0; s < ks.size(); ++s)
-        order.push_back({-(double)dv[s] * bv[s] * (pk.headers[s].n_atoms + 0.5 * pk.headers[s].n_chunks * G + 16), (int)s});
-      std::stable_sort(order.begin(), order.end());
-      const int cpw = 64 / G;
-      std::vector<int2> items;
-      for (auto& o : order)
-        for (int c0 = 0; c0 < p->chains; c0 += cpw) items.push_back(make_int2(o.second, c0));
       if (used == (overlap ? kMcSlots : 1)) flush();
-      McSlot& s = c->mc[used];
-      if (!s.stream) { HIP_CHECK(hipStreamCreate(&s.stream)); s.own_stream = true; }
-      // with overlap, every class but the last (the widest groups) takes half of its resident
-      // blocks per CU, which leaves room for the last class's waves beside it; the last class
-      // launches its full grid, whose blocks that do not fit at first take over the CU slots the
-      // others give up when their queues run dry.  Measured on the screening batch (16- and
-      // 64-lane classes): 0.5 to 0.75 of the residency are equally good, below 0.4 the first
-      // class runs too long.
       const bool last = std::next(classes.find(cl.first)) == classes.end();
-      const double share = (overlap && !last) ? 0.5 : 1.0;
-      mc_prepare(c, s, G, W, std::move(pk), std::move(streams), std::move(items), std::move(off),
-                 make_consts(c, p, 0, 0), c->batch.e.p, c->batch.conf.p, c->batch.heavy.p, true, true, share);
+      prepare_class(c, c->mc[used], cl.first.first, cl.first.second, cl.second, plans, p, (overlap && !last) ? 0.5 : 1.0);
       ++used;
     }
     flush();
@@ -1579,6 +1830,17 @@ int vd_set_class_overlap(vd_ctx* c, int on) {
   if (!c || (on != 0 && on != 1)) return VD_ERR_ARG;
   c->overlap_classes = on == 1;
   return VD_OK;
+}
+
+int vd_set_graphs(vd_ctx* c, int on) {
+  if (!c || (on != 0 && on != 1)) return VD_ERR_ARG;
+  return guard(c, [&]() {
+    c->graphs_on = on == 1;
+    if (!c->graphs_on && !c->host_only) {
+      HIP_CHECK(hipStreamSynchronize(c->stream));
+      c->graphs.clear();
+    }
+  });
 }
 
 int vd_dock_batch(vd_ctx* c, int n, const int* ids, const vd_dock_params* p, vd_stats* st) {

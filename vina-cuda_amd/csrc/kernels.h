@@ -73,6 +73,8 @@ hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& 
 hipError_t launch_corners(const float* maps, float4* corners, const vd_grid_desc& gd, hipStream_t s);
 hipError_t launch_eval(int G, const EvalArgs& a, int blocks, hipStream_t s);
 hipError_t launch_mc(int G, int W, const McArgs& a, int blocks, hipStream_t s);
+// The kernel function launch_mc launches for these arguments (to find its node in a captured graph).
+const void* mc_kernel_fn(int G, int W, const McArgs& a);
 hipError_t reset_counter(int* p, hipStream_t s);
 hipError_t glob_init(float* base, const long long* off, const int* per_axis, const int* T, int n, hipStream_t s);
 hipError_t random_confs(float* conf, long long n, int T, uint32_t seed, const float lo[3], const float hi[3], hipStream_t s);

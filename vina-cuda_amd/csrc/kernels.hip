@@ -716,6 +716,34 @@ hipError_t launch_mc(int G, int W, const McArgs& a, int blocks, hipStream_t s) {
   }
 }
 
+template <int MODE, int W>
+static const void* mc_fn_mode(int G) {
+  switch (G) {
+    case 64: return reinterpret_cast<const void*>(&vd_mc_kernel<64, MODE, W>);
+    case 32: return reinterpret_cast<const void*>(&vd_mc_kernel<32, MODE, W>);
+    case 16: return reinterpret_cast<const void*>(&vd_mc_kernel<16, MODE, W>);
+    default: return nullptr;
+  }
+}
+
+// The instantiation launch_mc launches for (G, W, a), the same dispatch.
+const void* mc_kernel_fn(int G, int W, const McArgs& a) {
+  if (a.P.local_search == 0) return mc_fn_mode<0, 2>(G);
+  const int mode = a.P.policy == 1 ? 2 : a.P.policy == 2 ? 3 : 1;
+  const int w = (W == 3 || W == 4) ? W : 2;
+  switch (mode * 8 + w) {
+    case 1 * 8 + 2: return mc_fn_mode<1, 2>(G);
+    case 1 * 8 + 3: return mc_fn_mode<1, 3>(G);
+    case 1 * 8 + 4: return mc_fn_mode<1, 4>(G);
+    case 2 * 8 + 2: return mc_fn_mode<2, 2>(G);
+    case 2 * 8 + 3: return mc_fn_mode<2, 3>(G);
+    case 2 * 8 + 4: return mc_fn_mode<2, 4>(G);
+    case 3 * 8 + 2: return mc_fn_mode<3, 2>(G);
+    case 3 * 8 + 3: return mc_fn_mode<3, 3>(G);
+    default: return mc_fn_mode<3, 4>(G);
+  }
+}
+
 __global__ void vd_fill_int(int* p, int v) { *p = v; }
 __global__ void vd_zero_u64_kernel(unsigned long long* p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;

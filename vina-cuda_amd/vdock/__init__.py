@@ -30,7 +30,7 @@ EXPORTS = (
     "vd_eval", "vd_eval_last_ms", "vd_dock_params_default", "vd_dock_heuristics", "vd_dock",
     "vd_dock_batch", "vd_dock_batch_plain", "vd_batch_results", "vd_batch_best", "vd_cluster",
     "vd_synth_receptor", "vd_synth_ligand", "vd_device_alloc", "vd_device_free", "vd_copy", "vd_random_confs",
-    "vd_last_launches", "vd_set_class_overlap", "vd_set_autotune", "vd_ligand_pair_chunks",
+    "vd_last_launches", "vd_set_class_overlap", "vd_set_graphs", "vd_set_autotune", "vd_ligand_pair_chunks",
     "vd_post_params_default", "vd_refine", "vd_select_modes", "vd_postprocess_batch", "vd_postprocess_docked",
     "vd_post_results", "vd_write_pdbqt", "vd_local_search", "vd_dock_noalloc",
 )
@@ -139,6 +139,7 @@ def load():
         "vd_last_launches": (C.c_int, [vp, C.c_int, C.POINTER(Launch)]),
         "vd_set_class_overlap": (C.c_int, [vp, C.c_int]),
         "vd_set_autotune": (C.c_int, [vp, C.c_int]),
+        "vd_set_graphs": (C.c_int, [vp, C.c_int]),
         "vd_ligand_pair_chunks": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint32)]),
         "vd_cluster": (C.c_int, [C.c_int, C.c_int, fp, fp, C.c_double, C.c_int, ip]),
         "vd_device_alloc": (C.c_int, [vp, C.c_size_t, C.POINTER(C.c_void_p)]),
@@ -433,6 +434,11 @@ class Context:
         """Run the lanes-per-chain classes of a batch at the same time (True, the default) or one
         after the other (False)."""
         self._check(self.lib.vd_set_class_overlap(self.h, 1 if on else 0))
+
+    def set_graphs(self, on):
+        """Run persistent batches as captured hipGraphs, replayed when the same ligands are docked
+        again with the same parameters (any seed) (True), or eagerly (False, the default)."""
+        self._check(self.lib.vd_set_graphs(self.h, 1 if on else 0))
 
     def set_autotune(self, on):
         """Time each size bucket's (lanes per chain, waves per SIMD) pair on first use (True) or
