@@ -172,3 +172,29 @@ def assert_parity(r):
     assert abs(r["mean_diff"]) <= r["mean_bound"] + 3 * r["se_mean"]
     assert r["best_diff"] <= r["best_bound"] + 3 * r["se_best"]
     assert r["success_ok"]
+
+
+def compare_chain_distributions(name, G, O, gb, ob, gev, oev, check=True):
+    """Two docking runs compared chain by chain: G, O = per-run arrays of every chain's final
+    energy, gb, ob = per-run best energies, gev, oev = evaluation counts.  Criteria: two-sample
+    Kolmogorov-Smirnov p >= 1e-3 on the pooled chains, the 1/5/25/50 % quantiles within 0.05
+    kcal/mol, the mean per-run best within 3.5 standard errors of the difference, the evaluation
+    counts within 1 %.  Returns the figures; asserts them unless check is False."""
+    from scipy.stats import ks_2samp
+    G, O = np.concatenate(G), np.concatenate(O)
+    gb, ob = np.asarray(gb), np.asarray(ob)
+    qs = [0.01, 0.05, 0.25, 0.5]
+    qg, qo = np.quantile(G, qs), np.quantile(O, qs)
+    ks = ks_2samp(G, O).pvalue
+    se = np.sqrt(gb.var(ddof=1) / len(gb) + ob.var(ddof=1) / len(ob))
+    r = dict(ks=ks, dq=float(np.abs(qg - qo).max()), dbest=float(abs(gb.mean() - ob.mean())), se=float(se),
+             devals=abs(gev - oev) / max(oev, 1))
+    say(name, "chains", len(G), "KS p", ks, "quantiles", np.round(qg, 4), np.round(qo, 4),
+        "mean best", gb.mean(), ob.mean(), "se", se, "evals", gev, oev)
+    r["ok"] = bool(ks >= 1e-3 and r["dq"] <= 0.05 and r["dbest"] <= 3.5 * se and r["devals"] <= 0.01)
+    if check:
+        assert ks >= 1e-3, (name, ks)
+        assert r["dq"] <= 0.05, (name, qg, qo)
+        assert r["dbest"] <= 3.5 * se, (name, gb.mean(), ob.mean(), se)
+        assert r["devals"] <= 0.01, (name, gev, oev)
+    return r
