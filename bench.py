@@ -124,6 +124,8 @@ def large_box_leg(vdock, log, n_ligands=4096, chains=128):
             "ligands_per_s": round(n_ligands / wall, 3), "ms": round(1000 * wall, 2), "kernel_ms": round(st.kernel_ms, 2),
             "evals_per_s": round(st.evals / (st.kernel_ms / 1000.0), 1), "mean_torsions": round(float(np.mean(tors)), 2),
             "mean_best_score": round(float(np.mean(ctx.batch_best())), 4),
+            "launches": [{"kernel": r.kernel, "ligands": r.ligands, "blocks": r.blocks, "start_ms": round(r.start_ms, 2),
+                          "kernel_ms": round(r.kernel_ms, 2)} for r in ctx.last_launches()],
             "grid": {"points": npts, "maps_MB": round(gbytes / 1e6, 1), "kernel_ms": round(g_ms, 3),
                      "GB_per_s": round(gbytes / (g_ms / 1000.0) / 1e9, 2)}}
 

@@ -3,7 +3,10 @@
  *
  * Plain C types and pointers only.  Every call returns VD_OK (0) or a negative vd_status and
  * never throws; the message of the last failure is available from vd_last_error().  A context
- * owns one HIP stream on one device and must be used from one host thread at a time.
+ * works on one device and must be used from one host thread at a time.  The contexts of a
+ * process on one device share the library's HIP streams (four per device, made on first use and
+ * kept until the process ends), so contexts used from several threads at once run their device
+ * work in turn.
  * Device buffers are owned by the library; host buffers belong to the caller and are copied.
  *
  * The reference path this replaces (paths under the reference root, V11 = Vina-CUDA_v1.1):

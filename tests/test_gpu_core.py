@@ -228,6 +228,31 @@ def test_class_overlap_same_results(ctx, synth_setup, policy):
         assert out[True][2].evals == out[False][2].evals
 
 
+def test_class_overlap_on_a_later_context(ctx, synth_setup):
+    """A context made while others exist still runs the two classes of a batch at the same time:
+    the contexts of a process share the library's streams, so its overlapped launches do not land
+    on streams that share a hardware queue (which would run them one after the other)."""
+    import vdock
+    s = synth_setup
+    other = vdock.Context(0)
+    try:
+        other.grid_upload(s["box"], s["maps"])
+        ids = [other.add_ligand(vdock.synth_ligand(14 if k % 2 == 0 else 40, 2 if k % 2 == 0 else 12, 400 + k))
+               for k in range(6)]
+        p = vdock.default_params(chains=256, search_depth=3, bfgs_steps=8, seed=3)
+        other.dock_batch(ids, p)
+        a, b = sorted(other.last_launches(), key=lambda r: r.start_ms)
+        say("later context:", a.kernel, a.start_ms, a.kernel_ms, b.kernel, b.start_ms, b.kernel_ms)
+        assert {a.lanes_per_chain, b.lanes_per_chain} == {16, 64}
+        assert b.start_ms < a.start_ms + 0.5 * a.kernel_ms
+        # the first context still works beside it
+        ctx.grid_upload(s["box"], s["maps"])
+        st = ctx.dock_batch(ids[:2], p)
+        assert st.evals > 0
+    finally:
+        other.close()
+
+
 def test_fixed_launch_shapes_and_autotune(ctx, synth_setup):
     """Left at 0, lanes per chain and waves per SIMD come from the fixed table per size bucket:
     16 lanes for a small ligand with few torsions, 64 for one with 12, and a second identical call

@@ -85,7 +85,6 @@ struct Packed {
 // (the sources of its asynchronous copies stay alive until it is collected).
 struct McSlot {
   hipStream_t stream = nullptr;
-  bool own_stream = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf<float4> blob;
   DevBuf<vd_lig_header> headers;
@@ -115,8 +114,7 @@ struct McSlot {
     evals.release(); hist.release(); glob.release(); glob_off.release(); glob_axis.release(); glob_T.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    if (own_stream && stream) (void)hipStreamDestroy(stream);
-    ev0 = ev1 = nullptr; stream = nullptr; own_stream = false;
+    ev0 = ev1 = nullptr; stream = nullptr;   // the streams are the process's (device_stream)
   }
 };
 constexpr int kMcSlots = 4;   // at most four hardware queues per process on these devices
@@ -246,6 +244,22 @@ unsigned char* scratch_for_current_device(int device) {
   if (!p) throw HipError("hipGetSymbolAddress returned no address for the scratch array");
   g_scratch[device] = static_cast<unsigned char*>(p);
   return g_scratch[device];
+}
+
+// The library's streams, one set of kMcSlots per device for the whole process, created on first
+// use and kept until the process ends.  A process's streams are served by at most four hardware
+// queues; two streams that share one run their kernels one after the other, so the two launches of
+// an overlapped batch would serialise on a context created after others.  Every context on a
+// device therefore uses the same streams: slot 0 carries all of a context's work, slots 1.. the
+// concurrent launches of a batch.
+std::mutex g_stream_mutex;
+std::map<int, std::vector<hipStream_t>> g_streams;
+hipStream_t device_stream(int device, int k) {
+  std::lock_guard<std::mutex> lock(g_stream_mutex);
+  std::vector<hipStream_t>& v = g_streams[device];
+  if (v.empty()) v.assign(kMcSlots, nullptr);
+  if (!v[k]) HIP_CHECK(hipStreamCreate(&v[k]));
+  return v[k];
 }
 
 int guard(vd_ctx* ctx, const std::function<void()>& f) {
@@ -1025,7 +1039,7 @@ int vd_create(int device, vd_ctx** out) {
   int rc = guard(c, [&]() {
     c->device = device;
     HIP_CHECK(hipSetDevice(device));
-    HIP_CHECK(hipStreamCreate(&c->stream));
+    c->stream = device_stream(device, 0);
     HIP_CHECK(hipEventCreate(&c->ev0));
     HIP_CHECK(hipEventCreate(&c->ev1));
     HIP_CHECK(hipEventCreate(&c->ev_base));
@@ -1069,7 +1083,6 @@ void vd_destroy(vd_ctx* c) {
   c->pin_cout.release(); c->pin_xout.release(); c->pin_res.release();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -1562,7 +1575,7 @@ void prepare_class(vd_ctx* c, McSlot& s, int G, int W, const std::vector<int>& k
   std::vector<int2> items;
   for (auto& o : order)
     for (int c0 = 0; c0 < p->chains; c0 += cpw) items.push_back(make_int2(o.second, c0));
-  if (!s.stream) { HIP_CHECK(hipStreamCreate(&s.stream)); s.own_stream = true; }
+  if (!s.stream) s.stream = device_stream(c->device, (int)(&s - c->mc));
   mc_prepare(c, s, G, W, std::move(pk), std::move(streams), std::move(items), std::move(off), make_consts(c, p, 0, 0),
              c->batch.e.p, c->batch.conf.p, c->batch.heavy.p, true, true, share);
 }
