@@ -196,6 +196,7 @@ struct vd_ctx {
   bool overlap_classes = true;               // batch classes run at the same time
   hipEvent_t ev_base = nullptr;
   bool graphs_on = false;                    // persistent batches run as captured graphs (vd_set_graphs)
+  hipStream_t cap[2] = {nullptr, nullptr};   // this context's capture-only streams (nothing runs on them)
   std::vector<std::unique_ptr<GraphEntry>> graphs;   // least recently used first
   DevBuf<float4> blob;                       // ligand record of vd_eval
   DevBuf<vd_lig_header> headers;
@@ -1068,6 +1069,8 @@ void vd_destroy(vd_ctx* c) {
   c->lut_rows.release(); c->lutq.release(); c->maps.release(); c->corners.release(); c->grid_atoms.release();
   c->grid_cells.release();
   c->graphs.clear();
+  for (hipStream_t& cs : c->cap)
+    if (cs) { (void)hipStreamDestroy(cs); cs = nullptr; }
   for (int k = 0; k < kMcSlots; ++k) {
     if (c->mc[k].stream) (void)hipStreamSynchronize(c->mc[k].stream);
     c->mc[k].release();
@@ -1625,12 +1628,16 @@ std::unique_ptr<GraphEntry> capture_batch(vd_ctx* c, const Classes& classes, boo
     evals_at += s.h_headers.size();
     g->cls.push_back(std::move(gc));
   }
-  hipStream_t main = c->stream;
+  // captured on streams of this context's own: the launch streams are shared by every context of
+  // the process, and work another context issued to a stream while it captures would be captured
+  for (int k = 0; k < 2; ++k)
+    if (!c->cap[k]) HIP_CHECK(hipStreamCreate(&c->cap[k]));
+  hipStream_t main = c->cap[0], side = c->cap[1];
   HIP_CHECK(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal));
   try {
-    if (overlap) {
+    if (overlap) {   // two classes: the second on a branch of its own
       HIP_CHECK(hipEventRecord(c->ev_base, main));
-      for (int k = 1; k < n; ++k) HIP_CHECK(hipStreamWaitEvent(c->mc[k].stream, c->ev_base, 0));
+      HIP_CHECK(hipStreamWaitEvent(side, c->ev_base, 0));
     }
     auto h2d = [&](hipStream_t st, void* dst, size_t off, size_t b) {
       if (b) HIP_CHECK(hipMemcpyAsync(dst, base + off, b, hipMemcpyHostToDevice, st));
@@ -1638,7 +1645,7 @@ std::unique_ptr<GraphEntry> capture_batch(vd_ctx* c, const Classes& classes, boo
     // every class's copies before any kernel, as in the eager path
     for (int k = 0; k < n; ++k) {
       McSlot& s = c->mc[k];
-      hipStream_t st = overlap ? s.stream : main;
+      hipStream_t st = overlap && k == 1 ? side : main;
       if (!s.h_goff.empty()) {
         h2d(st, s.glob_off.p, at[k].goff, s.h_goff.size() * 8);
         h2d(st, s.glob_axis.p, at[k].gaxis, s.h_gaxis.size() * 4);
@@ -1655,13 +1662,12 @@ std::unique_ptr<GraphEntry> capture_batch(vd_ctx* c, const Classes& classes, boo
     }
     for (int k = 0; k < n; ++k) {
       McSlot& s = c->mc[k];
-      HIP_CHECK(vdl::launch_mc(s.G, s.W, s.a, s.blocks, overlap ? s.stream : main));
+      HIP_CHECK(vdl::launch_mc(s.G, s.W, s.a, s.blocks, overlap && k == 1 ? side : main));
     }
-    if (overlap)
-      for (int k = 1; k < n; ++k) {
-        HIP_CHECK(hipEventRecord(c->mc[k].ev1, c->mc[k].stream));
-        HIP_CHECK(hipStreamWaitEvent(main, c->mc[k].ev1, 0));
-      }
+    if (overlap) {
+      HIP_CHECK(hipEventRecord(c->mc[1].ev1, side));
+      HIP_CHECK(hipStreamWaitEvent(main, c->mc[1].ev1, 0));
+    }
     for (int k = 0; k < n; ++k) {
       const McSlot& s = c->mc[k];
       HIP_CHECK(hipMemcpyAsync(g->evals.p + g->cls[k].evals_at, s.evals.p, s.h_headers.size() * sizeof(unsigned long long),
