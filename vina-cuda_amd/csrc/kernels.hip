@@ -20,31 +20,43 @@ using namespace vdk;
 namespace {
 
 // ------------------------------------------------------------------------------------------
-// Grid maps.  A workgroup is one wavefront and owns a 4x4x4 tile of points (x fastest), compact so
-// that an atom within the cutoff of the tile box reaches most of its points.  The receptor atoms
-// that can reach the tile come from a host-built cell list (cells of VD_GCELL A over the box grown by the
-// 8 A cutoff), one contiguous atom range per x-row of cells.  Each range is read 64 atoms at a
-// time, one per lane in a single coalesced load; a ballot of the lanes whose atom lies within the
-// cutoff of the tile box then lists the atoms worth a pass, and the loop takes them in index order
-// (GU per pass, all their table rows in flight before any is added), reading each atom from its
-// lane with readlane.  For every atom within the cutoff of a point, one 16-float row of the map
-// LUT (rows indexed [t1][bin]) and one value of the column after the rows give the contribution to
-// all 17 maps: 68 bytes, in four aligned 16-byte loads and one 4-byte load.  Every point adds its
-// atoms in the order of the cell list, so the tile shape does not change a map value.
-constexpr int TX = 4, TY = 4, TZ = 4;
+// Grid maps.  A workgroup is one wavefront and owns a compact tile of points (x fastest), LP
+// lanes per point: lane l works on point l / LP and on the maps t2 = (l % LP) * 16/LP ..
+// +16/LP-1, the first lane of a point also on t2 = 16.  The receptor atoms that can reach the tile
+// come from a host-built cell list (cells of VD_GCELL A over the box grown by the 8 A cutoff, one
+// contiguous atom range per x-row of cells).  Each range is read 64 atoms at a time, one per lane
+// in a single coalesced load; a ballot of the lanes whose atom lies within the cutoff of the tile
+// box lists the atoms worth a pass, and the loop takes them in index order (GU per pass, all their
+// table reads in flight before any is added), reading each atom from its lane with readlane.  For
+// every atom within the cutoff of a point, one 16-float row of the map LUT (rows indexed
+// [t1][bin]) and one value of the column after the rows give the contribution to all 17 maps;
+// the LP lanes of the point read the row's 16-byte chunks side by side, so one load instruction
+// fetches whole rows of 64 / LP points, one cache line each.  Every point adds its atoms in the
+// order of the cell list, whatever the tile shape or LP, so neither changes a map value.
 constexpr int GU = 2;   // receptor atoms per pass of the grid kernel's loop
 
 __device__ __forceinline__ float lane_f(float v, int j) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
 }
 
+template <int LP> struct GridTile {
+  static constexpr int NP = 64 / LP;                 // points per wavefront
+  static constexpr int TX = 4, TY = NP >= 32 ? 4 : 2, TZ = NP / (TX * TY);
+  static constexpr int K = VD_GRID_ROW / LP;         // row values per lane
+  static constexpr int CH = K / 4;                   // 16-byte chunks per lane
+};
+
+template <int LP>
 __global__ __launch_bounds__(64) void vd_grid_kernel(const float4* __restrict__ atoms, const int2* __restrict__ cells,
                                                      GridCells gc, const float4* __restrict__ lut_rows,
                                                      const float* __restrict__ lut_last, float* __restrict__ maps,
                                                      vd_grid_desc gd, int tiles_x, int tiles_y) {
+  using T = GridTile<LP>;
+  constexpr int TX = T::TX, TY = T::TY, TZ = T::TZ, K = T::K, CH = T::CH;
   const int tile = blockIdx.x, lane = threadIdx.x;
+  const int pt = lane / LP, q = lane % LP;
   const int tx0 = (tile % tiles_x) * TX, ty0 = ((tile / tiles_x) % tiles_y) * TY, tz0 = (tile / (tiles_x * tiles_y)) * TZ;
-  const int lx = lane % TX, ly = (lane / TX) % TY, lz = lane / (TX * TY);
+  const int lx = pt % TX, ly = (pt / TX) % TY, lz = pt / (TX * TY);
   const int x = tx0 + lx, y = ty0 + ly, z = tz0 + lz;
   const bool inside = x < gd.dim[0] && y < gd.dim[1] && z < gd.dim[2];
   // probe coordinates as grid::index_to_argument (V11/lib/grid.h:49-53), relative to the cell origin
@@ -59,9 +71,10 @@ __global__ __launch_bounds__(64) void vd_grid_kernel(const float4* __restrict__ 
   const int cx0 = max(0, (int)floorf((lo_x - cut) * inv)), cx1 = min(gc.n[0] - 1, (int)floorf((hi_x + cut) * inv));
   const int cy0 = max(0, (int)floorf((lo_y - cut) * inv)), cy1 = min(gc.n[1] - 1, (int)floorf((hi_y + cut) * inv));
   const int cz0 = max(0, (int)floorf((lo_z - cut) * inv)), cz1 = min(gc.n[2] - 1, (int)floorf((hi_z + cut) * inv));
-  float acc[VD_XS_SIZE];
+  const bool first = q == 0;   // this lane also sums t2 = 16
+  float acc[K], acc16 = 0.0f;
 #pragma unroll
-  for (int t = 0; t < VD_XS_SIZE; ++t) acc[t] = 0.0f;
+  for (int t = 0; t < K; ++t) acc[t] = 0.0f;
   for (int cz = cz0; cz <= cz1; ++cz)
     for (int cy = cy0; cy <= cy1; ++cy) {
       const int row = (cz * gc.n[1] + cy) * gc.n[0];
@@ -84,7 +97,7 @@ __global__ __launch_bounds__(64) void vd_grid_kernel(const float4* __restrict__ 
             j[u] = has[u] ? __builtin_ctzll(todo) : 0;
             todo &= todo - 1;
           }
-          float4 rv[GU][4];
+          float4 rv[GU][CH];
           float last[GU];
           bool use[GU];
 #pragma unroll
@@ -94,22 +107,24 @@ __global__ __launch_bounds__(64) void vd_grid_kernel(const float4* __restrict__ 
             const float dx = ax - px, dy = ay - py, dz = az - pz;
             const float r2 = dx * dx + dy * dy + dz * dz;
             use[u] = has[u] && inside && r2 <= VD_CUTOFF_SQR;
+            last[u] = 0.0f;
             if (use[u]) {
               const size_t r = (size_t)t1 * VD_LUT_BINS + (int)(VD_LUT_FACTOR * r2);
-              const float4* rw = lut_rows + r * (VD_GRID_ROW / 4);
+              const float4* rw = lut_rows + r * (VD_GRID_ROW / 4) + q * CH;
 #pragma unroll
-              for (int c = 0; c < 4; ++c) rv[u][c] = rw[c];
-              last[u] = lut_last[r];
+              for (int c = 0; c < CH; ++c) rv[u][c] = rw[c];
+              if (first) last[u] = lut_last[r];
             }
           }
 #pragma unroll
           for (int u = 0; u < GU; ++u) {
             if (!use[u]) continue;
-            acc[0] += rv[u][0].x; acc[1] += rv[u][0].y; acc[2] += rv[u][0].z; acc[3] += rv[u][0].w;
-            acc[4] += rv[u][1].x; acc[5] += rv[u][1].y; acc[6] += rv[u][1].z; acc[7] += rv[u][1].w;
-            acc[8] += rv[u][2].x; acc[9] += rv[u][2].y; acc[10] += rv[u][2].z; acc[11] += rv[u][2].w;
-            acc[12] += rv[u][3].x; acc[13] += rv[u][3].y; acc[14] += rv[u][3].z; acc[15] += rv[u][3].w;
-            acc[16] += last[u];
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+              acc[4 * c] += rv[u][c].x; acc[4 * c + 1] += rv[u][c].y;
+              acc[4 * c + 2] += rv[u][c].z; acc[4 * c + 3] += rv[u][c].w;
+            }
+            acc16 += last[u];
           }
         }
       }
@@ -117,7 +132,8 @@ __global__ __launch_bounds__(64) void vd_grid_kernel(const float4* __restrict__ 
   if (inside) {
     const size_t o = (size_t)x + (size_t)gd.dim[0] * ((size_t)y + (size_t)gd.dim[1] * z);
 #pragma unroll
-    for (int t = 0; t < VD_XS_SIZE; ++t) maps[(size_t)t * gd.plane + o] = acc[t];
+    for (int t = 0; t < K; ++t) maps[(size_t)(q * K + t) * gd.plane + o] = acc[t];
+    if (first) maps[(size_t)VD_GRID_ROW * gd.plane + o] = acc16;
   }
 }
 
@@ -651,12 +667,16 @@ int mc_lds_bytes(int G, int Amax, int Nmax, int dense) {
   return 4 * (lig_floats(Amax, Nmax) + (64 / G) * group_floats(Amax, Nmax, G, dense));
 }
 
+// Lanes per grid point of the grid kernel.
+constexpr int kGridLanesPerPoint = 2;
+
 hipError_t launch_grid(const float4* atoms, const int2* cells, const GridCells& gc, const float4* lut_rows, float* maps,
                        const vd_grid_desc& gd, hipStream_t s) {
-  const int tx = (gd.dim[0] + TX - 1) / TX, ty = (gd.dim[1] + TY - 1) / TY, tz = (gd.dim[2] + TZ - 1) / TZ;
+  using T = GridTile<kGridLanesPerPoint>;
+  const int tx = (gd.dim[0] + T::TX - 1) / T::TX, ty = (gd.dim[1] + T::TY - 1) / T::TY, tz = (gd.dim[2] + T::TZ - 1) / T::TZ;
   const float* lut_last = reinterpret_cast<const float*>(lut_rows) + (size_t)VD_XS_SIZE * VD_LUT_BINS * VD_GRID_ROW;
-  hipLaunchKernelGGL(vd_grid_kernel, dim3(tx * ty * tz), dim3(64), 0, s, atoms, cells, gc, lut_rows, lut_last, maps, gd,
-                     tx, ty);
+  hipLaunchKernelGGL(vd_grid_kernel<kGridLanesPerPoint>, dim3(tx * ty * tz), dim3(64), 0, s, atoms, cells, gc, lut_rows,
+                     lut_last, maps, gd, tx, ty);
   return hipGetLastError();
 }
 
