@@ -130,6 +130,76 @@ def large_box_leg(vdock, log, n_ligands=4096, chains=128):
                      "GB_per_s": round(gbytes / (g_ms / 1000.0) / 1e9, 2)}}
 
 
+# The reference's own per-kernel timings on its batch example: QuickVina-W-CUDA's
+# input_file_example/auto_test/log_1u1b.txt (NVIDIA GeForce RTX 3090, 8,192 chains per ligand --
+# the logged ric buffer of 1,867,776 B = 8,192 x 228 B), receptor 1u1b, box of
+# auto_test/1u1b_config.txt: kernel_grid over the 30 runs of the log (mean 10.0 ms) and
+# kernel_monte per ligand, with the search depth and BFGS steps the log prints for it.
+REFERENCE_LOG = {
+    "kernel_grid_ms_mean": 10.0,
+    "kernel_monte_s": {"drugbank1": 0.306398, "drugbank2": 1.406387, "drugbank3": 0.166095, "drugbank4": 1.146933,
+                       "drugbank5": 0.070353, "drugbank6": 0.163469, "drugbank7": 0.264657, "drugbank8": 0.070946,
+                       "drugbank9": 0.308865, "drugbank10": 0.364207},
+    "depth_bfgs": {"drugbank1": (6, 16), "drugbank2": (19, 23), "drugbank3": (3, 15), "drugbank4": (15, 21),
+                   "drugbank5": (1, 12), "drugbank6": (3, 15), "drugbank7": (5, 16), "drugbank8": (1, 12),
+                   "drugbank9": (5, 15), "drugbank10": (6, 16)},
+}
+
+
+def reference_log_leg(vdock, log, chains=8192):
+    """The inputs of the reference's own timing log (REFERENCE_LOG): the 1u1b receptor and its
+    auto_test box (65 x 71 x 61 points), and drugbank1-10 docked with 8,192 chains and the
+    heuristic search depth and BFGS steps, each through the plain per-ligand launch (the
+    reference's one kernel_monte call per ligand) and all ten in one batched launch.  The
+    reference ran QuickVina-W, whose gate as executed never skips a search, so the Vina policy
+    (the same schedule and heuristics) is what is timed here.  Other hardware: for scale only."""
+    root = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden")
+    ctx = vdock.Context(0)
+    try:
+        with open(os.path.join(root, "1u1b_protein.pdbqt")) as f:
+            ctx.set_receptor_pdbqt(f.read())
+        box = vdock.box_from_center([-30.661, -29.540, 17.010], [24.0, 26.25, 22.5], 0.375)
+        g_ms = float(np.median([ctx.grid_populate(box) for _ in range(5)]))
+        names = ["drugbank%d" % k for k in range(1, 11)]
+        ids = []
+        for n in names:
+            with open(os.path.join(root, "qvw_auto", n + ".pdbqt")) as f:
+                ids.append(ctx.add_ligand(f.read()))
+        p = vdock.default_params(chains=chains, seed=1)
+        per = {}
+        for n, lid in zip(names, ids):
+            inf = ctx.ligand_info(lid)
+            ctx.dock(lid, p)
+            log(f"reference log leg: {n}")
+            t0 = time.perf_counter()
+            _, _, _, st = ctx.dock(lid, p)
+            wall = 1000.0 * (time.perf_counter() - t0)
+            per[n] = {"atoms": inf.n_atoms, "torsions": inf.n_torsions,
+                      "depth_bfgs": list(vdock.heuristics(inf.n_atoms, inf.n_torsions)),
+                      "reference_depth_bfgs": list(REFERENCE_LOG["depth_bfgs"][n]),
+                      "kernel_ms": round(st.kernel_ms, 3), "wall_ms": round(wall, 3),
+                      "reference_kernel_monte_ms": round(1000.0 * REFERENCE_LOG["kernel_monte_s"][n], 3)}
+        ctx.dock_batch(ids, p)
+        t0 = time.perf_counter()
+        st_b = ctx.dock_batch(ids, p)
+        wall_b = 1000.0 * (time.perf_counter() - t0)
+    finally:
+        ctx.close()
+    ref_sum = 1000.0 * sum(REFERENCE_LOG["kernel_monte_s"].values())
+    plain_sum = sum(v["kernel_ms"] for v in per.values())
+    return {"workload": "the reference's auto_test log inputs: 1u1b receptor, 24 x 26.25 x 22.5 A box at 0.375 A "
+                        "(65 x 71 x 61 points), drugbank1-10, %d chains each, heuristic depth and BFGS steps, "
+                        "Vina policy" % chains,
+            "reference": "QuickVina-W-CUDA input_file_example/auto_test/log_1u1b.txt, NVIDIA GeForce RTX 3090 "
+                         "(other hardware: for scale only)",
+            "grid_ms": round(g_ms, 4), "reference_kernel_grid_ms": REFERENCE_LOG["kernel_grid_ms_mean"],
+            "per_ligand": per,
+            "plain_kernel_ms_sum": round(plain_sum, 3), "reference_kernel_monte_ms_sum": round(ref_sum, 3),
+            "batched_kernel_ms": round(st_b.kernel_ms, 3), "batched_wall_ms": round(wall_b, 3),
+            "plain_vs_reference": round(ref_sum / plain_sum, 2),
+            "batched_vs_reference": round(ref_sum / st_b.kernel_ms, 2)}
+
+
 def copy_bandwidth(ctx, nbytes=1 << 30, reps=5):
     """HBM bandwidth a plain device-to-device copy reaches on this device: bytes read plus bytes
     written over the best of `reps` timed copies (each call waits for its copy)."""
@@ -691,6 +761,7 @@ def main():
         out["small_scene"], small = small_scene_leg(vdock, log)
         if not args.no_large_box:
             out["large_box"] = large_box_leg(vdock, log)
+        out["reference_log"] = reference_log_leg(vdock, log)
 
     if args.copy_calibration:
         copy_calibration(ctx)
