@@ -247,8 +247,9 @@ int vd_dock_batch_plain(vd_ctx* ctx, int n, const int* ids, const vd_dock_params
  * into e[], conf[][7+T], heavy[][n_heavy][3] (host pointers, any may be NULL).  Returns the number
  * of chains the ligand was docked with (>= 0; max_chains = 0 asks for it) or an error. */
 int vd_batch_results(vd_ctx* ctx, int k, int max_chains, float* e, float* conf, float* heavy);
-/* Best energy per ligand of the last batch, best_e[n]. */
-int vd_batch_best(vd_ctx* ctx, float* best_e);
+/* Best energy per ligand of the last batch: copies at most max values into best_e (NULL when
+ * max = 0) and returns the batch's ligand count (>= 0) or an error. */
+int vd_batch_best(vd_ctx* ctx, int max, float* best_e);
 
 /* Output pose list: add_to_output_container over chains in order (min_rmsd, max_size;
  * V11/lib/coords.cpp:47-62).  Writes kept chain indices sorted by energy; returns the count. */
@@ -297,7 +298,8 @@ typedef struct {
 /* The selection of V11/main/main.cpp:425-470 over n refined poses of ligand id (host only, no
  * device work): sort by refined energy, eval_adjusted against the best pose's intramolecular
  * energy, sort, remove_redundant (min_rmsd), keep up to num_modes poses within energy_range of
- * the best; coords [n][n_atoms][3] as vd_refine returns them.  Returns the number of modes. */
+ * the best; coords [n][n_atoms][3] as vd_refine returns them.  Writes at most min(n,
+ * p->num_modes) entries into modes; returns the number of modes. */
 int vd_select_modes(vd_ctx* ctx, int id, int n, const vd_refined* r, const float* coords,
                     const vd_post_params* p, vd_mode* modes);
 

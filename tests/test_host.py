@@ -164,6 +164,11 @@ def test_invalid_arguments(host_ctx):
     assert lib.vd_set_graphs(None, 1) == vdock.VD_ERR_ARG
     assert lib.vd_set_graphs(host_ctx.h, 2) == vdock.VD_ERR_ARG
     assert lib.vd_set_graphs(host_ctx.h, 0) == vdock.VD_OK
+    # result fetches take the caller's capacity: with no batch docked there is nothing to copy
+    assert lib.vd_batch_best(None, 0, None) == vdock.VD_ERR_ARG
+    assert lib.vd_batch_best(host_ctx.h, -1, None) == vdock.VD_ERR_ARG
+    assert lib.vd_batch_best(host_ctx.h, 1, None) == vdock.VD_ERR_ARG
+    assert lib.vd_batch_best(host_ctx.h, 0, None) == 0
     with pytest.raises(vdock.VdockError):
         host_ctx.set_receptor(np.array([[np.nan, 0, 0]], dtype=np.float32), np.array([0], dtype=np.int32))
     with pytest.raises(vdock.VdockError):

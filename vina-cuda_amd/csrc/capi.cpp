@@ -1906,21 +1906,26 @@ int vd_batch_results(vd_ctx* c, int k, int max_chains, float* e, float* conf, fl
   return rc == VD_OK ? count : rc;
 }
 
-int vd_batch_best(vd_ctx* c, float* best_e) {
-  if (!c || !best_e) return VD_ERR_ARG;
-  return guard(c, [&]() {
-    need_device(c);
+int vd_batch_best(vd_ctx* c, int max, float* best_e) {
+  if (!c || max < 0 || (max > 0 && !best_e)) return VD_ERR_ARG;
+  int n = 0;
+  const int rc = guard(c, [&]() {
     BatchState& b = c->batch;
+    n = (int)b.ids.size();
+    const int m = std::min(n, max);
+    if (m == 0) return;
+    need_device(c);
     size_t tot = 0;
-    for (int ch : b.chains) tot += ch;
+    for (int k = 0; k < m; ++k) tot = std::max<size_t>(tot, (size_t)(b.off[k].x + b.chains[k]));
     std::vector<float> e(tot);
     if (tot) HIP_CHECK(hipMemcpy(e.data(), b.e.p, tot * 4, hipMemcpyDeviceToHost));
-    for (size_t k = 0; k < b.ids.size(); ++k) {
-      float m = INFINITY;
-      for (int i = 0; i < b.chains[k]; ++i) m = std::min(m, e[b.off[k].x + i]);
-      best_e[k] = m;
+    for (int k = 0; k < m; ++k) {
+      float v = INFINITY;
+      for (int i = 0; i < b.chains[k]; ++i) v = std::min(v, e[b.off[k].x + i]);
+      best_e[k] = v;
     }
   });
+  return rc == VD_OK ? n : rc;
 }
 
 int vd_cluster(int n, int n_heavy, const float* e, const float* heavy, double min_rmsd, int max_size, int* kept) {

@@ -135,7 +135,7 @@ def load():
         "vd_dock_batch": (C.c_int, [vp, C.c_int, ip, C.POINTER(DockParams), C.POINTER(Stats)]),
         "vd_dock_batch_plain": (C.c_int, [vp, C.c_int, ip, C.POINTER(DockParams), C.POINTER(Stats)]),
         "vd_batch_results": (C.c_int, [vp, C.c_int, C.c_int, fp, fp, fp]),
-        "vd_batch_best": (C.c_int, [vp, fp]),
+        "vd_batch_best": (C.c_int, [vp, C.c_int, fp]),
         "vd_last_launches": (C.c_int, [vp, C.c_int, C.POINTER(Launch)]),
         "vd_set_class_overlap": (C.c_int, [vp, C.c_int]),
         "vd_set_autotune": (C.c_int, [vp, C.c_int]),
@@ -418,8 +418,10 @@ class Context:
         return e, conf, heavy
 
     def batch_best(self):
-        out = np.zeros(len(self._batch_ids), dtype=np.float32)
-        self._check(self.lib.vd_batch_best(self.h, _f(out)))
+        n = self.lib.vd_batch_best(self.h, 0, None)
+        self._check(min(n, 0))
+        out = np.zeros(n, dtype=np.float32)
+        self._check(min(self.lib.vd_batch_best(self.h, n, _f(out)), 0))
         return out
 
     def pair_chunks(self, lid, lanes):
