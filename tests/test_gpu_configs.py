@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import oracle as orc
-from test_gpu_dock import minima_distance, no_worse_success, rmsd
+from helpers import assert_parity, multi_ligand_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -46,28 +46,14 @@ def test_cpu_runnable_config_docking_parity(ctx):
     depth, bfgs = vdock.heuristics(inf.n_atoms, inf.n_torsions)
     assert (inf.n_heavy, inf.n_torsions, depth, bfgs) == (20, 5, 5, 16)
     b, e, _ = box.as_tuple()
-    gb, ob, gp, op, ga, oa = [], [], [], [], [], []
+    gb, ob, gp, op = [], [], [], []
     for seed in range(16):
         p = vdock.default_params(chains=512, seed=seed)        # heuristic depth and steps
         eg, _, hg, _ = ctx.dock(lid, p)
         eo, _, ho, _ = orc.mc_run(g, lig, orc.mc_params(512, depth, bfgs, b, e, seed=seed, stream=lid, threads=8))
         gb.append(float(eg.min())); ob.append(float(eo.min()))
         gp.append(hg[int(np.argmin(eg))].astype(np.float64)); op.append(ho[int(np.argmin(eo))])
-        ga.append((eg.astype(np.float64), hg.astype(np.float64)))
-        oa.append((np.asarray(eo, dtype=np.float64), np.asarray(ho, dtype=np.float64)))
-    g_all = (np.concatenate([x[0] for x in ga]), np.concatenate([x[1] for x in ga]))
-    o_all = (np.concatenate([x[0] for x in oa]), np.concatenate([x[1] for x in oa]))
-    gb, ob = np.array(gb), np.array(ob)
-    top_o = op[int(np.argmin(ob))]
-    succ_g = np.mean([rmsd(x, top_o) <= 2.0 for x in gp])
-    succ_o = np.mean([rmsd(x, top_o) <= 2.0 for x in op])
-    d = minima_distance(gb, gp, g_all, ob, op, o_all)
-    say("config 0: best gpu", gb.min(), "oracle", ob.min(), "mean", gb.mean(), ob.mean(), "minima distance", d,
-        "success", succ_g, succ_o)
-    assert abs(gb.min() - ob.min()) <= 0.1
-    assert abs(gb.mean() - ob.mean()) <= 0.1
-    assert d <= 2.0
-    assert no_worse_success(succ_g, succ_o)
+    assert_parity(multi_ligand_parity([list(zip(gb, gp))], [list(zip(ob, op))], ["config 0: %d/%d" % (lig.n_atoms, lig.T)]))
 
 
 def test_single_ligand_config_plain_path(ctx):
