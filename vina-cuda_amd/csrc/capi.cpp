@@ -1750,6 +1750,10 @@ bool graph_dock(vd_ctx* c, const std::vector<int>& ids, const std::vector<LigPla
     g = capture_batch(c, classes, overlap);
     g->ids = ids; g->p = key; g->shapes = shapes; g->overlap = overlap; g->outs = outs;
   }
+  // a launch that a failed eager call left in flight still uses its slot's buffers, which the
+  // graph rewrites from another stream: wait for it first
+  for (McSlot& s : c->mc)
+    if (s.busy) { HIP_CHECK(hipStreamSynchronize(s.stream)); s.busy = false; }
   if (g->seed != p->seed) {
     for (GraphClass& gc : g->cls) {
       gc.a.P.seed = p->seed;
