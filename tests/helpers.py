@@ -72,7 +72,7 @@ def intra_cutoff_bound(lig, conf, ulps=8):
 
 
 def eval_bound(e_ref, g_ref):
-    """The evaluation's own bound on shared maps (north star: 1e-4 kcal/mol absolute / 1e-3
+    """The evaluation's own bound on shared maps (the parity target: 1e-4 kcal/mol absolute / 1e-3
     relative on energies; gradients per component 1e-3 of the largest component plus 1e-4)."""
     return max(1e-4, 1e-3 * abs(e_ref)), 1e-3 * max(1.0, float(np.abs(g_ref).max())) + 1e-4
 
@@ -128,9 +128,9 @@ def multi_ligand_parity(dev, orc_, names, mw_alpha=1e-3, mean_bound=0.1, best_bo
       - per ligand, the runs' best scores pass a two-sided Mann-Whitney test at level mw_alpha (a
         shift either way is caught; 12 ligands at 1e-3 keep the family-wise false alarm near 1 %);
       - the mean per-run best score, averaged over ligands, differs by at most mean_bound kcal/mol
-        (the north star's 0.1) beyond three standard errors of its measurement (bootstrap over
+        (the 0.1 kcal/mol target) beyond three standard errors of its measurement (bootstrap over
         each side's runs): a violation is declared only where the seeds resolve it;
-      - the best score over all seeds (the north star's "best score"), averaged over ligands, is
+      - the best score over all seeds (the docking target's "best score"), averaged over ligands, is
         at most best_bound above the oracle's, beyond three bootstrap standard errors; per ligand
         it scatters as much between two seed sets of the oracle itself as between the two
         implementations;

@@ -4,7 +4,7 @@ Tolerances (stated per test):
   maps      |gpu - ref| <= 1e-4 * max(1, |ref|) + slack, where slack is the oracle's bound on the
             change caused by an r^2 within 0.003 LUT bins of a bin edge (FP32 vs FP64 rounding of
             r^2 is below 1e-3 bins here and can move such an r^2 to the neighbouring bin).
-  energies  |gpu - ref| <= max(1e-4, 1e-3 * |ref|) kcal/mol.
+  energies  |gpu - ref| <= max(1e-4, 1e-3 * |ref|) kcal/mol (the parity target).
   gradients |gpu - ref| <= 1e-3 * max(1, |ref|_inf) + 1e-4 per component.
 """
 import numpy as np
