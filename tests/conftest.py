@@ -28,6 +28,17 @@ def ctx():
     c.close()
 
 
+@pytest.fixture(scope="module")
+def module_ctx():
+    """A device context of one test module's own.  Ligand ids count from 0 in every context and
+    key the docking random streams (with the seed), so the statistical tests that dock through it
+    draw the same streams whatever other tests ran before them."""
+    import vdock
+    c = vdock.Context(0)
+    yield c
+    c.close()
+
+
 @pytest.fixture(scope="session")
 def host_ctx():
     import vdock

@@ -35,6 +35,13 @@ from helpers import compare_chain_distributions as compare
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(scope="module")
+def ctx(module_ctx):
+    """This module's own context (conftest.module_ctx): its random streams do not depend on the
+    tests that ran before it."""
+    return module_ctx
+
 SEEDS, CHAINS = 32, 512
 
 

@@ -19,6 +19,13 @@ from helpers import assert_parity, multi_ligand_parity
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(scope="module")
+def ctx(module_ctx):
+    """This module's own context (conftest.module_ctx): its random streams do not depend on the
+    tests that ran before it."""
+    return module_ctx
+
+
 def say(*a):
     print(*a, flush=True)
 

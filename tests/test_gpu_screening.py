@@ -23,6 +23,13 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
+def ctx(module_ctx):
+    """This module's own context (conftest.module_ctx): its random streams do not depend on the
+    tests that ran before it."""
+    return module_ctx
+
+
+@pytest.fixture(scope="module")
 def screening(ctx):
     import vdock
     from vdock import workloads as W
