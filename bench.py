@@ -383,6 +383,48 @@ def graph_leg(vdock, ctx, log, ids, chains, calls=3):
     return out
 
 
+def adjoint_leg(vdock, ctx, n=1 << 20):
+    """The adjoint microbenchmark on the resident maps: 2^20 random conformations (translation
+    uniform in the box, uniform orientation, torsions uniform) of a 40-atom, 12-torsion ligand,
+    generated and kept in HBM; energy and adjoint gradient of each (vd_eval on device pointers),
+    timed at 1, 2 and 4 chains per wavefront.  Algorithmic bytes per evaluation: the eight map
+    corners of each heavy atom (32 B), one pair-table quad per pair (16 B), the conformation in
+    (4 (7+T)), the energy and the gradient out (4 + 4 (6+T)).  Returns the record and a sample
+    for the oracle check."""
+    atext = vdock.synth_ligand(40, 12, 777)
+    lid = ctx.add_ligand(atext)
+    inf = ctx.ligand_info(lid)
+    T = inf.n_torsions
+    d_conf = ctx.alloc(n * (7 + T) * 4)
+    d_e = ctx.alloc(n * 4)
+    d_g = ctx.alloc(n * (6 + T) * 4)
+    ctx.random_confs(lid, n, 1, d_conf)
+    rates = {}
+    for cpw in (1, 2, 4):
+        if T + 7 > 64 // cpw:
+            continue
+        ctx.eval_device(lid, d_conf, d_e, d_g, n, chains_per_wave=cpw)
+        ms = [ctx.eval_device(lid, d_conf, d_e, d_g, n, chains_per_wave=cpw) for _ in range(3)]
+        rates[cpw] = float(np.median(ms))
+    best_cpw = min(rates, key=rates.get)
+    ctx.eval_device(lid, d_conf, d_e, d_g, n, chains_per_wave=best_cpw)
+    e_host = ctx.to_host(d_e, (n,))
+    sample = np.random.default_rng(5).choice(n, 256, replace=False)
+    conf_host = ctx.to_host(d_conf, (n, 7 + T))[sample]
+    g_host = ctx.to_host(d_g, (n, 6 + T))[sample]
+    for ptr in (d_conf, d_e, d_g):
+        ctx.free(ptr)
+    per_eval = 32 * inf.n_heavy + 16 * inf.n_pairs + 4 * (7 + T) + 4 + 4 * (6 + T)
+    ms = rates[best_cpw]
+    rec = {"conformations": n, "atoms": inf.n_atoms, "heavy_atoms": inf.n_heavy, "torsions": T, "pairs": inf.n_pairs,
+           "chains_per_wave": best_cpw, "kernel_ms": round(ms, 3), "evals_per_s": round(n / (ms / 1000.0), 1),
+           "ms_by_chains_per_wave": {str(k): round(v, 3) for k, v in rates.items()},
+           "algorithmic_bytes_per_eval": per_eval,
+           "algorithmic_GB_per_s": round(per_eval * n / (ms / 1000.0) / 1e9, 1),
+           "finite_fraction": float(np.isfinite(e_host).mean())}
+    return rec, dict(text=atext, maps=ctx.grid_download(), conf=conf_host, e=e_host[sample], g=g_host)
+
+
 def single_ligand_leg(vdock, ctx, log):
     """One 32-heavy-atom, 8-torsion ligand with 8,192 chains through the plain per-ligand launch
     (the drop-in for kernel_monte) on the screening scene's maps."""
@@ -540,6 +582,8 @@ def main():
     ap.add_argument("--no-post", action="store_true", help="skip the refinement + final-score leg")
     ap.add_argument("--grid-only", action="store_true",
                     help="only the 17-map precompute: screening scene x5, large-box scene x3 (for a kernel profile)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the adjoint microbenchmark on the screening scene's maps (for a kernel profile)")
     ap.add_argument("--classes-alone", action="store_true",
                     help="dock each lanes-per-chain class of the screening batch by itself (one launch each, "
                          "with the grid it has alone), print the kernel times and exit: the run that counter "
@@ -567,6 +611,10 @@ def main():
         return
     log("scene: receptor + maps")
     xyz, xs, box, _ = build_scene(vdock, ctx)
+    if args.adjoint_only:
+        rec, _ = adjoint_leg(vdock, ctx)
+        print(json.dumps({"adjoint_only": True, "adjoint": rec}), flush=True)
+        return
     log("ligands")
     texts = screening_ligands(vdock, args.ligands)
     ids = [ctx.add_ligand(t) for t in texts]
@@ -709,38 +757,7 @@ def main():
                        "terms": terms, "terms_per_s": round(terms / (g_ms / 1000.0), 1),
                        "terms_note": "(grid point, typed receptor atom) pairs within 8 A; each reads one 17-value "
                                      "table row and adds it to the point's 17 sums"}
-        # adjoint microbenchmark: 2^20 random conformations of a 40-atom, 12-torsion ligand,
-        # generated and kept in HBM; forward energy + adjoint gradient per conformation
-        atext = vdock.synth_ligand(40, 12, 777)
-        lid = ctx.add_ligand(atext)
-        inf = ctx.ligand_info(lid)
-        T = inf.n_torsions
-        n = 1 << 20
-        d_conf = ctx.alloc(n * (7 + T) * 4)
-        d_e = ctx.alloc(n * 4)
-        d_g = ctx.alloc(n * (6 + T) * 4)
-        ctx.random_confs(lid, n, 1, d_conf)
-        rates = {}
-        for cpw in (1, 2, 4):
-            if T + 7 > 64 // cpw:
-                continue
-            ctx.eval_device(lid, d_conf, d_e, d_g, n, chains_per_wave=cpw)
-            ms = [ctx.eval_device(lid, d_conf, d_e, d_g, n, chains_per_wave=cpw) for _ in range(3)]
-            rates[cpw] = float(np.median(ms))
-        best_cpw = min(rates, key=rates.get)
-        ctx.eval_device(lid, d_conf, d_e, d_g, n, chains_per_wave=best_cpw)
-        e_host = ctx.to_host(d_e, (n,))
-        sample = np.random.default_rng(5).choice(n, 256, replace=False)
-        conf_host = ctx.to_host(d_conf, (n, 7 + T))[sample]
-        g_host = ctx.to_host(d_g, (n, 6 + T))[sample]
-        for ptr in (d_conf, d_e, d_g):
-            ctx.free(ptr)
-        adjoint_sample = dict(text=atext, maps=ctx.grid_download(), conf=conf_host, e=e_host[sample], g=g_host)
-        out["adjoint"] = {"conformations": n, "atoms": inf.n_atoms, "torsions": T, "pairs": inf.n_pairs,
-                          "chains_per_wave": best_cpw, "kernel_ms": round(rates[best_cpw], 3),
-                          "evals_per_s": round(n / (rates[best_cpw] / 1000.0), 1),
-                          "ms_by_chains_per_wave": {str(k): round(v, 3) for k, v in rates.items()},
-                          "finite_fraction": float(np.isfinite(e_host).mean())}
+        out["adjoint"], adjoint_sample = adjoint_leg(vdock, ctx)
         out["single_ligand"] = single_ligand_leg(vdock, ctx, log)
         out["graph"] = graph_leg(vdock, ctx, log, ids, args.chains)
         # the screening step with dense BFGS (--rilc_bfgs 0) and its own heuristics
