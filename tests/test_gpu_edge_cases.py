@@ -10,7 +10,13 @@ path's own definition:
   - the same ligand twice in one batch: each slot's chains draw from the ligand's own random stream
     (keyed by the seed and the ligand id), so the two slots return the same chains bit for bit;
   - a ligand wider than the largest group (120 atoms, 24 torsions) docked through the plain path:
-    finite results, best chains confirmed by the oracle.
+    finite results, best chains confirmed by the oracle;
+  - the largest ligand the library takes (255 atoms, 8-bit atom fields of the pair words; 57
+    torsions, 7 + T numbers on 64 lanes): energies and gradients against the oracle, plain and
+    batched docking with best chains confirmed by the oracle, and FP64 refinement of docked poses
+    against the oracle's;
+  - one past each limit: 259 atoms and 58 torsions are both refused when the ligand is added,
+    with the library's error codes.
 """
 import numpy as np
 import pytest
@@ -98,3 +104,106 @@ def test_wide_ligand_plain_launch(ctx, scene):
     assert st.evals > 0
     worst = check_best_chains(ctx, scene["grid"], lig, lid, e, conf, heavy, scene["box"])
     say("120-atom ligand: worst |e - e_oracle|", worst)
+
+
+@pytest.fixture(scope="module")
+def big_scene(ctx):
+    """A receptor shell with a cavity of 14 A radius and a 30 A box, which the largest ligand
+    (radius 11-12 A about its centre) fits inside."""
+    import vdock
+    xyz, xs = vdock.synth_receptor(3000, 50.0, 14.0, 43)
+    box = vdock.box_from_center([0, 0, 0], [30.0] * 3, 0.375)
+    ctx.set_receptor(xyz, xs)
+    ctx.grid_populate(box)
+    maps = ctx.grid_download()
+    g = orc.Grid(orc.Lut(), *box.as_tuple())
+    g.set_maps(maps.astype(np.float64))
+    g.set_receptor(xyz.astype(np.float64), xs)
+    text = vdock.synth_ligand(236, 57, 6)
+    return dict(xyz=xyz, xs=xs, box=box, maps=maps, grid=g, text=text, lig=orc.Ligand(text))
+
+
+def largest_ligand(ctx, s):
+    ctx.set_receptor(s["xyz"], s["xs"])
+    ctx.grid_upload(s["box"], s["maps"])
+    lid = ctx.add_ligand(s["text"])
+    inf = ctx.ligand_info(lid)
+    assert (inf.n_atoms, inf.n_torsions) == (255, 57)
+    return lid
+
+
+def test_largest_ligand_eval(ctx, big_scene):
+    from helpers import intra_cutoff_bound, out_of_box_bound
+    s = big_scene
+    lid, lig = largest_ligand(ctx, s), s["lig"]
+    say("largest ligand: pairs", lig.n_pairs)
+    rng = np.random.default_rng(5)
+    n = 32
+    conf = np.zeros((n, 7 + lig.T))
+    conf[:, :3] = rng.uniform(-2.0, 2.0, (n, 3))
+    q = rng.normal(size=(n, 4))
+    conf[:, 3:7] = q / np.linalg.norm(q, axis=1, keepdims=True)
+    conf[:, 7:] = rng.uniform(-np.pi, np.pi, (n, lig.T))
+    conf = conf.astype(np.float32)
+    e_gpu, g_gpu = ctx.eval(lid, conf)   # 7 + 57 numbers: one chain per wavefront
+    worst_e = worst_g = 0.0
+    for i in range(n):
+        c = conf[i].astype(np.float64)
+        e_ref, g_ref, _, _ = orc.eval_deriv(s["grid"], lig, c)
+        tol_e = max(1e-3, 1e-3 * abs(e_ref)) + intra_cutoff_bound(lig, c) + out_of_box_bound(lig, s["box"], c)
+        assert abs(e_gpu[i] - e_ref) <= tol_e, (i, e_gpu[i], e_ref)
+        tol_g = 1e-3 * max(1.0, np.abs(g_ref).max()) + 1e-4
+        assert np.all(np.abs(g_gpu[i] - g_ref) <= tol_g), (i, np.abs(g_gpu[i] - g_ref).max())
+        worst_e = max(worst_e, abs(e_gpu[i] - e_ref) / max(1.0, abs(e_ref)))
+        worst_g = max(worst_g, np.abs(g_gpu[i] - g_ref).max() / max(1.0, np.abs(g_ref).max()))
+    say("largest ligand eval: worst rel e", worst_e, "worst rel g", worst_g)
+
+
+def test_largest_ligand_docks_plain_and_batched(ctx, big_scene):
+    import vdock
+    s = big_scene
+    lid = largest_ligand(ctx, s)
+    text_small = vdock.synth_ligand(20, 4, 61)
+    small = ctx.add_ligand(text_small)
+    p = vdock.default_params(chains=64, search_depth=2, bfgs_steps=6, seed=12)
+    e, conf, heavy, st = ctx.dock(lid, p)
+    assert st.evals > 64
+    worst = check_best_chains(ctx, s["grid"], s["lig"], lid, e, conf, heavy, s["box"])
+    say("largest ligand, plain: best", e.min(), "worst |e - e_oracle|", worst)
+    ctx.dock_batch([small, lid], p)
+    assert sorted(r.kernel.split("<")[1].split(",")[0] for r in ctx.last_launches()) == ["16", "64"]
+    for k, (i, lig) in enumerate([(small, orc.Ligand(text_small)), (lid, s["lig"])]):
+        e, conf, heavy = ctx.batch_results(k)
+        assert len(e) == 64
+        worst = check_best_chains(ctx, s["grid"], lig, i, e, conf, heavy, s["box"])
+        say("batched ligand", k, "best", e.min(), "worst |e - e_oracle|", worst)
+
+
+def test_largest_ligand_refines_like_the_oracle(ctx, big_scene):
+    import vdock
+    s = big_scene
+    lid, lig = largest_ligand(ctx, s), s["lig"]
+    en, conf, _, _ = ctx.dock(lid, vdock.default_params(chains=64, search_depth=2, bfgs_steps=6, seed=13))
+    starts = conf[np.argsort(en)[:4]].astype(np.float64)
+    co, out_xyz, res = ctx.refine(lid, starts)
+    de, dr = [], []
+    for i, c in enumerate(starts):
+        eo, _, xo = orc.refine(s["grid"], lig, c, orc.refine_steps(lig))
+        if res[i].inside:
+            de.append(abs(res[i].e - eo))
+            dr.append(float(np.sqrt(((out_xyz[i].astype(np.float64) - xo) ** 2).sum(-1).mean())))
+        else:
+            assert eo > 1e300
+    say("largest ligand refinement: |de|", np.round(de, 6), "rmsd", np.round(dr, 5))
+    assert len(de) >= 3
+    assert np.median(de) <= 1e-4 and np.median(dr) <= 1e-3 and max(de) <= 0.1
+
+
+def test_one_past_each_limit_is_refused(ctx, scene):
+    import vdock
+    with pytest.raises(vdock.VdockError) as err:
+        ctx.add_ligand(vdock.synth_ligand(250, 57, 7))   # 259 atoms
+    assert err.value.code == vdock.VD_ERR_PARSE and "limit is 255" in str(err.value)
+    with pytest.raises(vdock.VdockError) as err:
+        ctx.add_ligand(vdock.synth_ligand(100, 58, 8))   # 105 atoms, 58 torsions
+    assert err.value.code == vdock.VD_ERR_CAPACITY and "57 torsions" in str(err.value)
