@@ -13,8 +13,9 @@ path's own definition:
     finite results, best chains confirmed by the oracle;
   - the largest ligand the library takes (255 atoms, 8-bit atom fields of the pair words; 57
     torsions, 7 + T numbers on 64 lanes): energies and gradients against the oracle, plain and
-    batched docking with best chains confirmed by the oracle, and FP64 refinement of docked poses
-    against the oracle's;
+    batched docking with best chains confirmed by the oracle (also under the QuickVina2,
+    QuickVina-W and dense-BFGS modes, whose per-chain state grows with the torsions), and FP64
+    refinement of docked poses against the oracle's;
   - one past each limit: 259 atoms and 58 torsions are both refused when the ligand is added,
     with the library's error codes.
 """
@@ -168,15 +169,37 @@ def test_largest_ligand_docks_plain_and_batched(ctx, big_scene):
     p = vdock.default_params(chains=64, search_depth=2, bfgs_steps=6, seed=12)
     e, conf, heavy, st = ctx.dock(lid, p)
     assert st.evals > 64
-    worst = check_best_chains(ctx, s["grid"], s["lig"], lid, e, conf, heavy, s["box"])
-    say("largest ligand, plain: best", e.min(), "worst |e - e_oracle|", worst)
+    worst = check_best_chains(ctx, s["grid"], s["lig"], lid, e, conf, heavy, s["box"], rel=True)
+    say("largest ligand, plain: best", e.min(), "worst relative |e - e_oracle|", worst)
     ctx.dock_batch([small, lid], p)
     assert sorted(r.kernel.split("<")[1].split(",")[0] for r in ctx.last_launches()) == ["16", "64"]
     for k, (i, lig) in enumerate([(small, orc.Ligand(text_small)), (lid, s["lig"])]):
         e, conf, heavy = ctx.batch_results(k)
         assert len(e) == 64
-        worst = check_best_chains(ctx, s["grid"], lig, i, e, conf, heavy, s["box"])
-        say("batched ligand", k, "best", e.min(), "worst |e - e_oracle|", worst)
+        worst = check_best_chains(ctx, s["grid"], lig, i, e, conf, heavy, s["box"], rel=True)
+        say("batched ligand", k, "best", e.min(), "worst relative |e - e_oracle|", worst)
+
+
+@pytest.mark.parametrize("mode", ["quickvina2", "quickvina_w", "dense_bfgs"])
+def test_largest_ligand_other_search_modes(ctx, big_scene, mode):
+    """The modes whose per-chain state grows with the torsion count, at 57 torsions: the
+    QuickVina2 history (a ring of 630 entries of 64 numbers), QuickVina-W's individual ring and the
+    ligand's shared cell buffer, the dense BFGS matrix (63 x 63)."""
+    import vdock
+    s = big_scene
+    lid = largest_ligand(ctx, s)
+    kw = dict(quickvina2=dict(policy=vdock.VD_MC_QVINA2), quickvina_w=dict(policy=vdock.VD_MC_QVINAW),
+              dense_bfgs=dict(local_search=vdock.VD_LS_BFGS))[mode]
+    p = vdock.default_params(chains=64, search_depth=3, bfgs_steps=6, seed=14, **kw)
+    e, conf, heavy, st = ctx.dock(lid, p)
+    assert st.evals > 64
+    worst = check_best_chains(ctx, s["grid"], s["lig"], lid, e, conf, heavy, s["box"], rel=True)
+    say("largest ligand,", mode, "plain: best", e.min(), "worst relative |e - e_oracle|", worst)
+    if mode == "quickvina2":
+        ctx.dock_batch([lid], p)
+        e, conf, heavy = ctx.batch_results(0)
+        worst = check_best_chains(ctx, s["grid"], s["lig"], lid, e, conf, heavy, s["box"], rel=True)
+        say("largest ligand,", mode, "batched: best", e.min(), "worst relative |e - e_oracle|", worst)
 
 
 def test_largest_ligand_refines_like_the_oracle(ctx, big_scene):
