@@ -722,6 +722,7 @@ def main():
                        "evals_per_step": sum(x["evals"] for x in spans) // len(spans),
                        "avg_launch_ms_by_instantiation": ms_by_kernel, "largest_share_instantiation": name,
                        "algorithmic_bytes_by_instantiation": {k: round(v["bytes"] / v["n"]) for k, v in by_kernel.items()},
+                       "evals_by_instantiation": {k: round(v["evals"] / v["n"]) for k, v in by_kernel.items()},
                        "launches_by_instantiation": {k: v["n"] for k, v in by_kernel.items()}}
     if post is not None:
         out["post"] = post

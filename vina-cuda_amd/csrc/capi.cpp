@@ -176,7 +176,7 @@ struct vd_ctx {
   std::string err;
   double weights[6] = {-0.035579, -0.005156, 0.840245, -0.035069, -0.587439, 0.05846};
   vd::Precalculated pre;
-  DevBuf<float> lut_rows, lutq;
+  DevBuf<float> lut_rows, lutp;
   std::vector<double> rec_xyz;   // receptor coordinates as parsed (FP64)
   std::vector<int> rec_xs;
   // maps
@@ -285,13 +285,13 @@ int guard(vd_ctx* ctx, const std::function<void()>& f) {
 void upload_lut(vd_ctx* c) {
   c->pre.build(c->weights);
   if (c->host_only) return;
-  std::vector<float> rows, quads;
+  std::vector<float> rows, bins;
   c->pre.grid_rows(rows);
-  c->pre.pair_quads(quads);
+  c->pre.pair_bins(bins);
   c->lut_rows.ensure(rows.size());
-  c->lutq.ensure(quads.size());
+  c->lutp.ensure(bins.size());
   HIP_CHECK(hipMemcpy(c->lut_rows.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
-  HIP_CHECK(hipMemcpy(c->lutq.p, quads.data(), quads.size() * 4, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(c->lutp.p, bins.data(), bins.size() * 4, hipMemcpyHostToDevice));
   // FP64 quads (e_i, dor_i, e_i+1, dor_i+1) for the refinement after the search
   std::vector<double> qd((size_t)VD_NUM_TYPE_PAIRS * VD_LUT_BINS * 4);
   for (int tp = 0; tp < VD_NUM_TYPE_PAIRS; ++tp)
@@ -432,7 +432,7 @@ vd_mc_consts make_consts(vd_ctx* c, const vd_dock_params* p, int depth, int bfgs
 
 vdk::EvalK make_evalk(vd_ctx* c, float v_intra, float v_inter, float slope) {
   vdk::EvalK k;
-  k.corners = c->corners.p; k.gd = c->gd; k.lutq = reinterpret_cast<const float4*>(c->lutq.p);
+  k.corners = c->corners.p; k.gd = c->gd; k.lutp = reinterpret_cast<const float2*>(c->lutp.p);
   k.cd0 = c->gd.dim[0] - 1; k.cd01 = (c->gd.dim[0] - 1) * (c->gd.dim[1] - 1);
   k.cplane = (long long)k.cd01 * (c->gd.dim[2] - 1);
   k.v_intra = v_intra; k.v_inter = v_inter; k.slope = slope;
@@ -1078,7 +1078,7 @@ void vd_destroy(vd_ctx* c) {
   if (c->host_only) { delete c; return; }
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  c->lut_rows.release(); c->lutq.release(); c->maps.release(); c->corners.release(); c->grid_atoms.release();
+  c->lut_rows.release(); c->lutp.release(); c->maps.release(); c->corners.release(); c->grid_atoms.release();
   c->grid_cells.release();
   c->graphs.clear();
   for (hipStream_t& cs : c->cap)

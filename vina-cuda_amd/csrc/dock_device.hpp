@@ -174,8 +174,14 @@ struct EvalK {
   const float4* __restrict__ corners; vd_grid_desc gd;
   int cd0, cd01;                 // cells per x-row, per z-plane
   long long cplane;              // cells per type
-  const float4* __restrict__ lutq;
+  const float2* __restrict__ lutp;  // pair table, (e, dor) per bin: [tp][bin]
   float v_intra, v_inter, slope;
+  // the interpolation quad (e_i, dor_i, e_{i+1}, dor_{i+1}) of entry i: one 16-byte load at 8-byte
+  // alignment (global memory takes unaligned vector loads), half the table's footprint in the
+  // caches against a table that stores each quad whole
+  __device__ __forceinline__ float4 quad(uint32_t i) const {
+    return *reinterpret_cast<const float4*>(lutp + i);
+  }
 };
 struct Conf {                  // register view of one conformation
   float px, py, pz; Quat q; float tor;
@@ -365,7 +371,7 @@ __device__ float pair_terms_rows(const LigView& L, const Scratch& S, const EvalK
     }
     float4 q[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) q[u] = K.lutq[idx[u]];
+    for (int u = 0; u < U; ++u) q[u] = K.quad(idx[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       float e, dor;
@@ -427,7 +433,7 @@ __device__ float pair_terms_scatter(const LigView& L, const Scratch& S, const Ev
     }
     float4 q[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) q[u] = K.lutq[idx[u]];
+    for (int u = 0; u < U; ++u) q[u] = K.quad(idx[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       float e, dor;

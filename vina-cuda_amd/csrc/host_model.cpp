@@ -153,18 +153,16 @@ void Precalculated::grid_rows(std::vector<float>& out) const {
     }
 }
 
-void Precalculated::pair_quads(std::vector<float>& out) const {
-  // one extra all-zero row after the 153 type pairs, whose first quad is VD_ZERO_QUAD: empty pair
-  // slots and pairs beyond the cutoff read it and contribute nothing
-  out.assign(((size_t)VD_NUM_TYPE_PAIRS + 1) * VD_LUT_BINS * 4, 0.0f);
-  for (int tp = 0; tp < VD_NUM_TYPE_PAIRS; ++tp)
-    for (int b = 0; b < VD_LUT_BINS; ++b) {
-      const size_t s = (size_t)tp * VD_LUT_BINS;
-      const int b1 = std::min(b + 1, VD_LUT_BINS - 1);
-      float* q = &out[(s + b) * 4];
-      q[0] = (float)smooth_e[s + b]; q[1] = (float)smooth_dor[s + b];
-      q[2] = (float)smooth_e[s + b1]; q[3] = (float)smooth_dor[s + b1];
-    }
+void Precalculated::pair_bins(std::vector<float>& out) const {
+  // one extra all-zero row after the 153 type pairs, whose first entry is VD_ZERO_QUAD: empty pair
+  // slots and pairs beyond the cutoff read it and contribute nothing.  A pair inside the cutoff
+  // has r^2 < 64, so 32 r^2 < 2048 (scaling by 32 is exact) and its bin is at most 2047: the
+  // entry after it (bin + 1 <= 2048) lies in the same row.
+  out.assign(((size_t)VD_NUM_TYPE_PAIRS + 1) * VD_LUT_BINS * 2, 0.0f);
+  for (size_t i = 0; i < (size_t)VD_NUM_TYPE_PAIRS * VD_LUT_BINS; ++i) {
+    out[2 * i] = (float)smooth_e[i];
+    out[2 * i + 1] = (float)smooth_dor[i];
+  }
 }
 
 // ---------------------------------------------------------------------------------------

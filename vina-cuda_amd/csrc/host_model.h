@@ -38,9 +38,10 @@ struct Precalculated {
   void build(const double w[5]);
   // FP32 device layouts:
   //   grid rows  [t1][bin][VD_GRID_ROW]: fast value for t2 = 0..15, then the column [t1][bin] of t2 = 16
-  //   pair quads [tp][bin] = (e_i, dor_i, e_{i+1}, dor_{i+1})
+  //   pair bins  [tp][bin] = (e_i, dor_i); the 16 bytes at bin i are the interpolation quad
+  //              (e_i, dor_i, e_{i+1}, dor_{i+1})
   void grid_rows(std::vector<float>& out) const;
-  void pair_quads(std::vector<float>& out) const;
+  void pair_bins(std::vector<float>& out) const;
 };
 
 struct LigAtom {

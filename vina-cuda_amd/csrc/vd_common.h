@@ -10,7 +10,7 @@
 #define VD_LUT_BINS 2051             // int(32 * 64) + 3
 #define VD_LUT_FACTOR 32.0f
 #define VD_CUTOFF_SQR 64.0f
-#define VD_ZERO_QUAD (VD_NUM_TYPE_PAIRS * VD_LUT_BINS)   // first quad of the all-zero row after the pair table
+#define VD_ZERO_QUAD (VD_NUM_TYPE_PAIRS * VD_LUT_BINS)   // first entry of the all-zero row after the pair table
 #define VD_GRID_ROW 16               // per (t1, bin) row of the map LUT: the values of t2 = 0..15 (64 B,
                                      // four aligned 16-byte loads); the t2 = 16 values follow all rows
                                      // as a [t1][bin] column
