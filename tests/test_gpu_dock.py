@@ -132,6 +132,46 @@ def test_quickvina_w_policy_parity(ctx, scene, heavy, tors, lseed):
     assert abs(r_g - r_o) <= 0.08
 
 
+def test_quickvina_w_batched_parity(ctx, scene):
+    """QuickVina-W through the batched launch, with both lanes-per-chain classes running at once
+    (an 18-atom, 3-torsion ligand in the 16-lane class, a 36-atom, 11-torsion one in the 64-lane
+    class, each with its own shared buffer), against the oracle's runs of the same ligands and
+    streams: the 16-seed criteria of the Vina policy per ligand, and the gate saving evaluations in
+    about the same proportion on both sides (ratio to the Vina policy within 0.08)."""
+    import vdock
+    ctx.grid_upload(scene["box"], scene["maps"])
+    texts = [vdock.synth_ligand(18, 3, 71), vdock.synth_ligand(36, 11, 72)]
+    ids = [ctx.add_ligand(t) for t in texts]
+    ligs = [orc.Ligand(t) for t in texts]
+    chains, depth, bfgs = 256, 48, 12
+    b, e, _ = scene["box"].as_tuple()
+
+    def run(policy, seeds):
+        dev, ora = [[] for _ in ids], [[] for _ in ids]
+        gev = oev = 0
+        for seed in range(seeds):
+            st = ctx.dock_batch(ids, vdock.default_params(chains=chains, search_depth=depth, bfgs_steps=bfgs,
+                                                          seed=seed, policy=policy))
+            assert sorted(r.kernel.split("<")[1].split(",")[0] for r in ctx.last_launches()) == ["16", "64"]
+            gev += st.evals
+            for k, (lid, lig) in enumerate(zip(ids, ligs)):
+                eg, _, hg = ctx.batch_results(k)
+                prm = orc.mc_params(chains, depth, bfgs, b, e, seed=seed, stream=lid, threads=8, policy=policy)
+                eo, _, ho, ev = orc.mc_run(scene["grid"], lig, prm)
+                oev += ev
+                dev[k].append((float(eg.min()), hg[int(np.argmin(eg))].astype(np.float64)))
+                ora[k].append((float(eo.min()), ho[int(np.argmin(eo))]))
+        return dev, ora, gev, oev
+
+    dev, ora, gq, oq = run(2, 16)
+    _, _, gv, ov = run(0, 4)
+    r_g, r_o = (gq / 16) / (gv / 4), (oq / 16) / (ov / 4)
+    say("batched qvw eval ratio to vina: gpu", r_g, "oracle", r_o)
+    assert_parity(multi_ligand_parity(dev, ora, ["%d/%d" % (l.n_atoms, l.T) for l in ligs]))
+    assert r_o < 0.97
+    assert abs(r_g - r_o) <= 0.08
+
+
 def test_initial_conformations_match_oracle(ctx, scene):
     """With zero MC steps the device keeps each chain's initial conformation: it must equal the
     oracle's draw from the shared Philox stream (FP32 rounding aside)."""
