@@ -17,7 +17,7 @@ path's own definition:
     QuickVina-W and dense-BFGS modes, whose per-chain state grows with the torsions), and FP64
     refinement of docked poses against the oracle's;
   - the batched launch against plain per-ligand launches of the same ligands (one per class), Vina
-    and QuickVina2 policies: the same chains bit for bit;
+    and QuickVina2 policies, RILC-BFGS and dense BFGS: the same chains bit for bit;
   - one past each limit: 259 atoms and 58 torsions are both refused when the ligand is added,
     with the library's error codes.
 """
@@ -109,20 +109,22 @@ def test_wide_ligand_plain_launch(ctx, scene):
     say("120-atom ligand: worst |e - e_oracle|", worst)
 
 
-@pytest.mark.parametrize("policy", [0, 1])
-def test_batched_equals_plain_bit_for_bit(ctx, scene, policy):
+@pytest.mark.parametrize("policy,local_search", [(0, 1), (1, 1), (0, 0)])
+def test_batched_equals_plain_bit_for_bit(ctx, scene, policy, local_search):
     """A plain per-ligand launch given the lanes per chain of its ligand's class (a plain launch of
     few chains otherwise takes one chain per wavefront, for parallelism) runs the chains as the
     batched launch does: the same lanes per chain and register budget, and each chain drawing from
     the ligand's own random stream.  So the two return the same chains bit for bit -- for the Vina
-    policy and for QuickVina2, whose histories are per chain (depth 64: the gate skips searches).
+    policy and for QuickVina2, whose histories are per chain (depth 64: the gate skips searches),
+    with RILC-BFGS, and for the Vina policy with dense BFGS.
     One ligand in each class: 20 atoms / 2 torsions (16 lanes), 40 atoms / 12 torsions (64 lanes)."""
     import vdock
     ctx.grid_upload(scene["box"], scene["maps"])
     ids = [ctx.add_ligand(vdock.synth_ligand(20, 2, 81)), ctx.add_ligand(vdock.synth_ligand(40, 12, 82))]
-    p = vdock.default_params(chains=64, search_depth=64, bfgs_steps=8, seed=10, policy=policy)
-    plain = [ctx.dock(i, vdock.default_params(chains=64, search_depth=64, bfgs_steps=8, seed=10, policy=policy,
-                                              chains_per_wave=cpw)) for i, cpw in zip(ids, (4, 1))]
+    kw = dict(chains=64, search_depth=64 if local_search else 8, bfgs_steps=8, seed=10, policy=policy,
+              local_search=local_search)
+    p = vdock.default_params(**kw)
+    plain = [ctx.dock(i, vdock.default_params(chains_per_wave=cpw, **kw)) for i, cpw in zip(ids, (4, 1))]
     assert [r[3].chains_per_wave for r in plain] == [4, 1]
     st = ctx.dock_batch(ids, p)
     assert sorted(r.lanes_per_chain for r in ctx.last_launches()) == [16, 64]
