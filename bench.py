@@ -125,7 +125,8 @@ def large_box_leg(vdock, log, n_ligands=4096, chains=128):
             "evals_per_s": round(st.evals / (st.kernel_ms / 1000.0), 1), "mean_torsions": round(float(np.mean(tors)), 2),
             "mean_best_score": round(float(np.mean(ctx.batch_best())), 4),
             "launches": [{"kernel": r.kernel, "ligands": r.ligands, "blocks": r.blocks, "start_ms": round(r.start_ms, 2),
-                          "kernel_ms": round(r.kernel_ms, 2)} for r in ctx.last_launches()],
+                          "kernel_ms": round(r.kernel_ms, 2), "evals": int(r.evals), "algorithmic_bytes": round(r.eval_bytes)}
+                         for r in ctx.last_launches()],
             "grid": {"points": npts, "maps_MB": round(gbytes / 1e6, 1), "kernel_ms": round(g_ms, 3),
                      "GB_per_s": round(gbytes / (g_ms / 1000.0) / 1e9, 2)}}
 
@@ -582,6 +583,8 @@ def main():
     ap.add_argument("--no-post", action="store_true", help="skip the refinement + final-score leg")
     ap.add_argument("--grid-only", action="store_true",
                     help="only the 17-map precompute: screening scene x5, large-box scene x3 (for a kernel profile)")
+    ap.add_argument("--large-box-only", action="store_true",
+                    help="run only the large-box QuickVina2 batch (for profiling it)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the adjoint microbenchmark on the screening scene's maps (for a kernel profile)")
     ap.add_argument("--classes-alone", action="store_true",
@@ -608,6 +611,12 @@ def main():
         if args.copy_calibration:
             copy_calibration(ctx)
         print(json.dumps({"grid_only": True, "screening_ms": ms, "large_box_ms": ms2}), flush=True)
+        return
+    if args.large_box_only:
+        rec = large_box_leg(vdock, log)
+        if args.copy_calibration:
+            copy_calibration(ctx)
+        print(json.dumps({"large_box_only": True, "large_box": rec}), flush=True)
         return
     log("scene: receptor + maps")
     xyz, xs, box, _ = build_scene(vdock, ctx)
