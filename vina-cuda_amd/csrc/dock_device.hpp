@@ -644,9 +644,14 @@ template <int G> __device__ __forceinline__ unsigned long long group_ballot(bool
 }
 template <int G> __device__ __forceinline__ bool is_dim_lane(int l, int T) { return (l >= 1 && l <= T) || l >= G - 6; }
 
+// The gate functions below are compiled out of line (noinline).  Inlined into the search loop,
+// their temporaries raise the register pressure of the whole loop, and at three waves per SIMD the
+// kernels then spill inside it on every evaluation; called once per local search, the calls cost
+// little.
+
 // Adds (x, f, g) when on; every lane of the wave calls it.
 template <int G>
-__device__ void hist_add(History& h, const Conf& x, float f, float g, int l, int T, bool on) {
+__device__ __attribute__((noinline)) void hist_add(History& h, const Conf& x, float f, float g, int l, int T, bool on) {
   const bool dim = is_dim_lane<G>(l, T);
   const unsigned long long z = group_ballot<G>(dim && g == 0.0f), ps = group_ballot<G>(dim && g > 0.0f);
   const int slot = h.count < h.cap ? h.count : h.p;
@@ -663,7 +668,7 @@ __device__ void hist_add(History& h, const Conf& x, float f, float g, int l, int
 // true as soon as one does not explain (x, f, g), else false; checked = entries examined.
 // Groups with need == false examine nothing.  Every lane of the wave calls it.
 template <int G>
-__device__ bool hist_check(const HistView& h, const Conf& x, float f, float g, int l, int T, bool need,
+__device__ __attribute__((noinline)) bool hist_check(const HistView& h, const Conf& x, float f, float g, int l, int T, bool need,
                            int max_checks, int& checked) {
   checked = 0;
   if (!__any(need)) return false;
@@ -721,7 +726,7 @@ __device__ bool hist_check(const HistView& h, const Conf& x, float f, float g, i
 // QuickVina2's gate: true when (x, f, g) is worth a local search -- always until the history is
 // full, then unless each of the 6+T nearest entries explains it.  Every lane of the wave calls it.
 template <int G>
-__device__ bool hist_interesting(const History& h, const Conf& x, float f, float g, int l, int T, bool on) {
+__device__ __attribute__((noinline)) bool hist_interesting(const History& h, const Conf& x, float f, float g, int l, int T, bool on) {
   const bool need = on && h.count == h.cap;
   int checked;
   const bool found = hist_check<G>(HistView{h.X, h.F, h.Z, h.Pm, h.cap, h.cap}, x, f, g, l, T, need, 6 + T, checked);
@@ -762,7 +767,7 @@ struct GlobalHist {
 
 // Adds (x, f, g) to the global buffer cell of x's position when on.  Every lane calls it.
 template <int G>
-__device__ void glob_add(const GlobalHist& gh, const Conf& x, float f, float g, int l, int T, bool on) {
+__device__ __attribute__((noinline)) void glob_add(const GlobalHist& gh, const Conf& x, float f, float g, int l, int T, bool on) {
   const bool dim = is_dim_lane<G>(l, T);
   const unsigned long long z = group_ballot<G>(dim && g == 0.0f), ps = group_ballot<G>(dim && g > 0.0f);
   const int cell = gh.cell_of(x.px, x.py, x.pz);
@@ -786,7 +791,7 @@ __device__ void glob_add(const GlobalHist& gh, const Conf& x, float f, float g, 
 // examines up to 6+T nearest entries of the cell holding x; if none says interesting, the
 // individual buffer decides -- interesting until full, then by its 4(6+T) - (examined) nearest.
 template <int G>
-__device__ bool qvw_interesting(const History& h, const GlobalHist& gh, const Conf& x, float f, float g, int l, int T,
+__device__ __attribute__((noinline)) bool qvw_interesting(const History& h, const GlobalHist& gh, const Conf& x, float f, float g, int l, int T,
                                 bool on) {
   const int cell = gh.cell_of(x.px, x.py, x.pz);
   const int avail = on ? min(__atomic_load_n(gh.counter(cell), __ATOMIC_RELAXED), 8) : 0;
