@@ -811,14 +811,16 @@ __device__ __attribute__((noinline)) bool qvw_interesting(const History& h, cons
 template <int G>
 __device__ float dense_bfgs(const LigView& L, const Scratch& S, float* H, float* vbuf, const EvalK& K, Conf& x,
                             bool enabled, float x_e, int max_steps, int l, int& evals) {
-  // H is G x G, lane l owns row l; vbuf holds G floats for broadcasting a vector
+  // H is G x G, lane l owns row l, kept column-major (entry (l, j) at H[j G + l]) so that the
+  // lanes' accesses at one j are consecutive words: no LDS bank conflicts (row-major, the lanes of
+  // a group would all hit one bank); vbuf holds G floats for broadcasting a vector
 #pragma unroll 1
-  for (int j = 0; j < G; ++j) H[l * G + j] = (j == l) ? 1.0f : 0.0f;
+  for (int j = 0; j < G; ++j) H[j * G + l] = (j == l) ? 1.0f : 0.0f;
   auto matvec = [&](float v) {     // returns -(H v)_l
     __syncthreads(); vbuf[l] = v; __syncthreads();
     float acc = 0.0f;
 #pragma unroll 1
-    for (int j = 0; j < G; ++j) acc += H[l * G + j] * vbuf[j];
+    for (int j = 0; j < G; ++j) acc += H[j * G + l] * vbuf[j];
     __syncthreads();
     return -acc;
   };
@@ -853,7 +855,7 @@ __device__ float dense_bfgs(const LigView& L, const Scratch& S, float* H, float*
     if (step == 0 && upd && fabsf(yy) > kEpsFl) {
       const float diag = alpha * yp / yy;
 #pragma unroll 1
-      for (int j = 0; j < G; ++j) H[l * G + j] = (j == l) ? diag : H[l * G + j];
+      for (int j = 0; j < G; ++j) H[j * G + l] = (j == l) ? diag : H[j * G + l];
     }
     const float mhy = matvec(y);
     const float yhy = -gsum<G>(y * mhy);
@@ -867,7 +869,7 @@ __device__ float dense_bfgs(const LigView& L, const Scratch& S, float* H, float*
     if (do_upd)
 #pragma unroll 1
       for (int j = 0; j < G; ++j)
-        H[l * G + j] += alpha * r * (mhy * pb[j] + vbuf[j] * p) + alpha * alpha * (r * r * yhy + r) * p * pb[j];
+        H[j * G + l] += alpha * r * (mhy * pb[j] + vbuf[j] * p) + alpha * alpha * (r * r * yhy + r) * p * pb[j];
     __syncthreads();
   }
   if (enabled && !(f0 <= f_orig)) { f0 = f_orig; x = x_orig; }
