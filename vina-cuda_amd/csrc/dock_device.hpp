@@ -814,12 +814,12 @@ __device__ float dense_bfgs(const LigView& L, const Scratch& S, float* H, float*
   // H is G x G, lane l owns row l, kept column-major (entry (l, j) at H[j G + l]) so that the
   // lanes' accesses at one j are consecutive words: no LDS bank conflicts (row-major, the lanes of
   // a group would all hit one bank); vbuf holds G floats for broadcasting a vector
-#pragma unroll 1
+#pragma unroll 4
   for (int j = 0; j < G; ++j) H[j * G + l] = (j == l) ? 1.0f : 0.0f;
   auto matvec = [&](float v) {     // returns -(H v)_l
     __syncthreads(); vbuf[l] = v; __syncthreads();
     float acc = 0.0f;
-#pragma unroll 1
+#pragma unroll 4
     for (int j = 0; j < G; ++j) acc += H[j * G + l] * vbuf[j];
     __syncthreads();
     return -acc;
@@ -854,7 +854,7 @@ __device__ float dense_bfgs(const LigView& L, const Scratch& S, float* H, float*
     const float yy = gsum<G>(y * y), yp = gsum<G>(y * p);
     if (step == 0 && upd && fabsf(yy) > kEpsFl) {
       const float diag = alpha * yp / yy;
-#pragma unroll 1
+#pragma unroll 4
       for (int j = 0; j < G; ++j) H[j * G + l] = (j == l) ? diag : H[j * G + l];
     }
     const float mhy = matvec(y);
@@ -867,7 +867,7 @@ __device__ float dense_bfgs(const LigView& L, const Scratch& S, float* H, float*
     vbuf[l] = mhy; pb[l] = p;
     __syncthreads();
     if (do_upd)
-#pragma unroll 1
+#pragma unroll 4
       for (int j = 0; j < G; ++j)
         H[j * G + l] += alpha * r * (mhy * pb[j] + vbuf[j] * p) + alpha * alpha * (r * r * yhy + r) * p * pb[j];
     __syncthreads();
