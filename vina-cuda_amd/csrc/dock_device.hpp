@@ -808,72 +808,42 @@ __device__ __attribute__((noinline)) bool qvw_interesting(const History& h, cons
 // quasi_newton.cuh:1859-1916, 2391-2499).  H (n x n, n = 6+T <= G) lives in LDS, row i in the
 // lane that owns change component i.  The loops over a row stay rolled: unrolled G-long, they
 // kept G LDS values live at once and spilled up to 1.3 KB per lane.
+// Dense BFGS (bfgs.h:30-140 as the reference GPU code runs it, quasi_newton.cuh:2391-2499): the
+// matrix of each chain group.  H is G x G, lane l owns row l, kept column-major (entry (l, j) at
+// H[j G + l]) so that the lanes' accesses at one j are consecutive words: no LDS bank conflicts
+// (row-major, the lanes of a group would all hit one bank).  vbuf (2 G floats after H) broadcasts
+// vectors.  Every lane of the wavefront calls these (they fence); `on` selects the groups whose
+// matrix is written.
 template <int G>
-__device__ float dense_bfgs(const LigView& L, const Scratch& S, float* H, float* vbuf, const EvalK& K, Conf& x,
-                            bool enabled, float x_e, int max_steps, int l, int& evals) {
-  // H is G x G, lane l owns row l, kept column-major (entry (l, j) at H[j G + l]) so that the
-  // lanes' accesses at one j are consecutive words: no LDS bank conflicts (row-major, the lanes of
-  // a group would all hit one bank); vbuf holds G floats for broadcasting a vector
+__device__ __forceinline__ void dense_identity(float* H, int l, bool on) {
+  if (on) {
 #pragma unroll 4
-  for (int j = 0; j < G; ++j) H[j * G + l] = (j == l) ? 1.0f : 0.0f;
-  auto matvec = [&](float v) {     // returns -(H v)_l
-    __syncthreads(); vbuf[l] = v; __syncthreads();
-    float acc = 0.0f;
-#pragma unroll 4
-    for (int j = 0; j < G; ++j) acc += H[j * G + l] * vbuf[j];
-    __syncthreads();
-    return -acc;
-  };
-  float g;
-  float f0 = eval_deriv<G>(L, S, K, x, l, g);
-  evals += enabled ? 1 : 0;
-  const float f_orig = f0; const Conf x_orig = x;
-  bool active = enabled;
-  for (int step = 0; step < max_steps && __any(active); ++step) {
-    const float p = matvec(g);
-    const float pg = gsum<G>(p * g);
-    float alpha = 1.0f, f1 = f0, g_new = g; Conf x_new = x;
-    bool ls = active;
-    for (int trial = 0; trial < 10 && __any(ls); ++trial) {
-      const Conf xt = increment<G>(x, p, alpha, l, L.T);
-      float gt;
-      const float ft = eval_deriv<G>(L, S, K, csel(ls, xt, x), l, gt);
-      if (ls) {
-        x_new = xt; f1 = ft; g_new = gt; ++evals;
-        if (f1 - f0 < 1e-4f * alpha * pg) ls = false; else alpha *= 0.5f;
-      }
-    }
-    const float y = g_new - g;
-    const float gnorm = sqrtf(gsum<G>(g * g));
-    bool upd = false;
-    if (active) {
-      f0 = f1; x = x_new;
-      if (!(gnorm >= 1e-5f)) active = false;
-      else { g = g_new; upd = true; }
-    }
-    const float yy = gsum<G>(y * y), yp = gsum<G>(y * p);
-    if (step == 0 && upd && fabsf(yy) > kEpsFl) {
-      const float diag = alpha * yp / yy;
-#pragma unroll 4
-      for (int j = 0; j < G; ++j) H[j * G + l] = (j == l) ? diag : H[j * G + l];
-    }
-    const float mhy = matvec(y);
-    const float yhy = -gsum<G>(y * mhy);
-    const float r = 1.0f / (alpha * yp);
-    const bool do_upd = upd && !(alpha * yp < kEpsFl);
-    // H_lj += alpha r (mhy_l p_j + mhy_j p_l) + alpha^2 (r^2 yhy + r) p_l p_j
-    float* pb = vbuf + G;
-    __syncthreads();
-    vbuf[l] = mhy; pb[l] = p;
-    __syncthreads();
-    if (do_upd)
-#pragma unroll 4
-      for (int j = 0; j < G; ++j)
-        H[j * G + l] += alpha * r * (mhy * pb[j] + vbuf[j] * p) + alpha * alpha * (r * r * yhy + r) * p * pb[j];
-    __syncthreads();
+    for (int j = 0; j < G; ++j) H[j * G + l] = (j == l) ? 1.0f : 0.0f;
   }
-  if (enabled && !(f0 <= f_orig)) { f0 = f_orig; x = x_orig; }
-  return enabled ? f0 : x_e;
+}
+// -(H v)_l, summed in column order
+template <int G>
+__device__ __forceinline__ float dense_minus_hv(const float* H, float* vbuf, float v, int l) {
+  __syncthreads(); vbuf[l] = v; __syncthreads();
+  float acc = 0.0f;
+#pragma unroll 4
+  for (int j = 0; j < G; ++j) acc += H[j * G + l] * vbuf[j];
+  __syncthreads();
+  return -acc;
+}
+// H_lj += alpha r (mhy_l p_j + mhy_j p_l) + alpha^2 (r^2 yhy + r) p_l p_j  (mhy = -H y, r = 1/(alpha y.p))
+template <int G>
+__device__ __forceinline__ void dense_update(float* H, float* vbuf, float mhy, float p, float alpha, float r, float yhy,
+                                             bool on, int l) {
+  float* pb = vbuf + G;
+  __syncthreads();
+  vbuf[l] = mhy; pb[l] = p;
+  __syncthreads();
+  if (on)
+#pragma unroll 4
+    for (int j = 0; j < G; ++j)
+      H[j * G + l] += alpha * r * (mhy * pb[j] + vbuf[j] * p) + alpha * alpha * (r * r * yhy + r) * p * pb[j];
+  __syncthreads();
 }
 
 }  // namespace vdk

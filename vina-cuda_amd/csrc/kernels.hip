@@ -216,15 +216,6 @@ __device__ void write_chain(const LigView& L, const Scratch& S, const Conf& best
   }
 }
 
-// Dense-BFGS local search of the groups with enabled == true (the RILC-BFGS modes run inside
-// mc_chain_flat's state machine instead).
-template <int G>
-__device__ float local_search(const LigView& L, const Scratch& S, float* H, const EvalK& K, Conf& x,
-                              bool enabled, float x_e, int max_steps, int l, int& evals) {
-  if (!__any(enabled)) return x_e;
-  return dense_bfgs<G>(L, S, H, H + G * G, K, x, enabled, x_e, max_steps, l, evals);
-}
-
 // Initial conformation of a chain: the caller's (the role of kernel_monte's ric,
 // V11/inc/cuda/kernel2.cuh:27; the reference host draws them at main_procedure_cu.cpp:584-635), or
 // conf::randomize (V11/lib/conf.h:127-130, 214-217; random_orientation, V11/lib/quaternion.cpp:206-219)
@@ -264,95 +255,30 @@ __device__ Conf initial_conf(const vd_mc_consts& P, uint32_t stream, uint32_t ch
   return c;
 }
 
-// One MC chain per group with the reference GPU schedule and dense BFGS (--rilc_bfgs 0,
-// V11/inc/cuda/kernel2.cu:290-426 with bfgs_optimized, quasi_newton.cuh:2391-2499):
+// One MC chain per group with the reference GPU schedule (V11/inc/cuda/kernel2.cu:219-426):
 // mutate -> local search -> Metropolis(T) -> on accept, if better than the best so far: second
 // local search, keep the best.  Mutation as V11/lib/mutate.cpp:62-127 (see initial_conf for the
 // random stream); the orientation step uses the gyration radius of the current conformation.
 // ls_only: one local search from the starting conformation, no MC steps (vd_local_search).
-template <int G>
-__device__ void mc_chain(const LigView& L, const Scratch& S, float* H, const EvalK& K, const vd_mc_consts& P,
-                         int depth, int bfgs_steps, uint32_t stream, uint32_t chain, bool valid, int l, float* out_e,
-                         float* out_conf, float* out_heavy, int& evals, const float* given, bool ls_only) {
-  const uint32_t k0 = P.seed, k1 = stream;
-  Conf tmp = initial_conf<G>(P, stream, chain, l, L.T, given);
-  if (ls_only) {
-    const float e = local_search<G>(L, S, H, K, tmp, valid, 0.0f, bfgs_steps, l, evals);
-    set_coords<G>(L, S, tmp, l);
-    if (valid) write_chain<G>(L, S, tmp, e, chain, l, out_e, out_conf, out_heavy);
-    return;
-  }
-  float tmp_e = 0.0f;
-  set_coords<G>(L, S, tmp, l);
-  float rg = gyration_radius<G>(L, S, l);
-  float best_e = INFINITY;
-  stash_put<G>(S.stash, kSlotBest, tmp);
-  const int n_ent = P.mutation_quirk ? (1 + L.T) : (2 + L.T);
-  for (int step = 0; step < depth; ++step) {
-    const U4 w = philox(chain, (uint32_t)step, 0, 0, k0, k1);
-    const int which = min((int)(u01(w.x) * (float)n_ent), n_ent - 1);
-    const float u_metro = u01(w.y), tor = -kPi + 2.0f * kPi * u01(w.z);
-    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    bool need = which <= 1;
-    for (uint32_t a = 0; a < 64 && __any(need); ++a) {   // P(64 rejections) ~ 1e-20
-      const U4 v = philox(chain, (uint32_t)step, 1, a, k0, k1);
-      const float vx = 2.0f * u01(v.x) - 1.0f, vy = 2.0f * u01(v.y) - 1.0f, vz = 2.0f * u01(v.z) - 1.0f;
-      if (need && vx * vx + vy * vy + vz * vz < 1.0f) { sx = vx; sy = vy; sz = vz; need = false; }
-    }
-    Conf cand = tmp;
-    stash_put<G>(S.stash, kSlotTmp, tmp);
-    if (which == 0) { cand.px += P.amplitude * sx; cand.py += P.amplitude * sy; cand.pz += P.amplitude * sz; }
-    else if (which == 1) { if (rg > kEpsFl) { const float f = P.amplitude / rg; cand.q = qincrement(cand.q, f * sx, f * sy, f * sz); } }
-    else if (l == which - 1) cand.tor = tor;
-    const float ce = local_search<G>(L, S, H, K, cand, valid, 0.0f, bfgs_steps, l, evals);
-    const bool accept = valid && (step == 0 || ce < tmp_e || u_metro < expf((tmp_e - ce) / P.temperature));
-    if (accept) { tmp = cand; tmp_e = ce; } else tmp = stash_get<G>(S.stash, kSlotTmp);
-    const bool second = accept && tmp_e < best_e;
-    const float e2 = local_search<G>(L, S, H, K, tmp, second, tmp_e, bfgs_steps, l, evals);
-    if (second) { tmp_e = e2; if (tmp_e < best_e) { stash_put<G>(S.stash, kSlotBest, tmp); best_e = tmp_e; } }
-    if (__any(accept)) {
-      set_coords<G>(L, S, tmp, l);
-      const float r = gyration_radius<G>(L, S, l);
-      if (accept) rg = r;
-    }
-  }
-  const Conf best = stash_get<G>(S.stash, kSlotBest);
-  set_coords<G>(L, S, best, l);
-  if (valid) {
-    if (l == 0) {
-      out_e[chain] = best_e;
-      float* c = out_conf + (size_t)chain * (7 + L.T);
-      c[0] = best.px; c[1] = best.py; c[2] = best.pz; c[3] = best.q.w; c[4] = best.q.x; c[5] = best.q.y; c[6] = best.q.z;
-    }
-    if (l >= 1 && l <= L.T) out_conf[(size_t)chain * (7 + L.T) + 6 + l] = best.tor;
-    for (int i = l; i < L.A; i += G) {
-      const uint32_t meta = fbits(L.atom[i].w);
-      if ((meta >> 16) & 1u) {
-        const int slot = (meta >> 17) & 255;
-        float* o = out_heavy + ((size_t)chain * L.H + slot) * 3;
-        const float4 y = S.xyz[i];
-        o[0] = best.px + y.x; o[1] = best.py + y.y; o[2] = best.pz + y.z;
-      }
-    }
-  }
-}
-
-// The MC schedule of mc_chain with RILC-BFGS and the Lewis-Overton line search
-// (V11/inc/cuda/quasi_newton.cuh:1710-1806, 2172-2389), run as a flat state machine: every pass evaluates one conformation per chain group and each group then advances
-// its own state (start of a local search, line-search trial, end of a BFGS iteration, end of a
-// local search, Metropolis step, next mutation) as far as it can without another evaluation.
-// Chains of a wavefront thus never wait for each other's line searches or local searches; they
-// only wait at the end of the chain.  The gyration radius of each evaluated conformation comes
-// from the evaluation itself and travels with the conformation.
-// POL: 0 Vina; 1 QuickVina2 gate at the start of every local search (see History);
-// 2 QuickVina-W gate (qvw_interesting) with its records: the searched start point, each BFGS
-// iteration's end point and the point where the search stops go to the chain's own buffer, the
-// search's last point to the ligand's global buffer (QVW/inc/cuda/quasi_newton.cuh:2236-2410).
+// The local search is RILC-BFGS with the Lewis-Overton line search (quasi_newton.cuh:1710-1806,
+// 2172-2389) or, with POL = -1, dense BFGS with its backtracking line search (--rilc_bfgs 0,
+// kernel2.cu:427-563 with bfgs_optimized, quasi_newton.cuh:2391-2499; H is the group's matrix,
+// see dense_minus_hv).  It runs as a flat state machine: every pass evaluates one conformation
+// per chain group and each group then advances its own state (start of a local search,
+// line-search trial, end of a BFGS iteration, end of a local search, Metropolis step, next
+// mutation) as far as it can without another evaluation.  Chains of a wavefront thus never wait
+// for each other's line searches or local searches; they only wait at the end of the chain.  The
+// gyration radius of each evaluated conformation comes from the evaluation itself and travels
+// with the conformation.
+// POL: -1 dense BFGS; 0 Vina; 1 QuickVina2 gate at the start of every local search (see
+// History); 2 QuickVina-W gate (qvw_interesting) with its records: the searched start point, each
+// BFGS iteration's end point and the point where the search stops go to the chain's own buffer,
+// the search's last point to the ligand's global buffer (QVW/inc/cuda/quasi_newton.cuh:2236-2410).
 template <int G, int POL>
 __device__ void mc_chain_flat(const LigView& L, const Scratch& S, const EvalK& K, const vd_mc_consts& P, int depth,
                               int max_steps, uint32_t stream, uint32_t chain, bool valid, int l, float* out_e,
                               float* out_conf, float* out_heavy, int& evals, History* hist, const GlobalHist* gh,
-                              const float* given, bool ls_only) {
+                              const float* given, bool ls_only, float* H) {
   const uint32_t k0 = P.seed, k1 = stream;
   const int n_ent = P.mutation_quirk ? (1 + L.T) : (2 + L.T);
   enum { kMutate = 0, kStart = 1, kTrial = 2, kDone = 3 };
@@ -379,6 +305,8 @@ __device__ void mc_chain_flat(const LigView& L, const Scratch& S, const EvalK& K
   float stp = 0.0f, finit = 0.0f, dgtest = 0.0f, dstest = 0.0f, mu = 0.0f, nu = 0.0f;
   bool brackt = false;
   int count = 0, k = 0;
+  float dn_pg = 0.0f;   // dense BFGS: p.g of the iteration's direction
+  int dn_step = 0;      // dense BFGS: iterations done in this search
   while (__any(phase != kDone)) {
     // (1) MC step start: draws and mutation (mutate.cpp:62-127); the candidate is evaluated next
     const bool mut = phase == kMutate;
@@ -429,6 +357,38 @@ __device__ void mc_chain_flat(const LigView& L, const Scratch& S, const EvalK& K
       }
     }
     bool rec_own = false, rec_glob = false;   // QuickVina-W records of this pass
+    if constexpr (POL < 0) {   // dense BFGS (quasi_newton.cuh:2391-2499); x, fx, g hold the point evaluated
+      // a search starts from the identity matrix; no step limit -> the start point is the result
+      dense_identity<G>(H, l, start);
+      if (start) { dn_step = 0; if (max_steps > 0) begin = true; else ls_done = true; }
+      // backtracking: accept when f1 - f0 < 1e-4 alpha p.g, else halve alpha; after ten trials the
+      // last point is taken (alpha halved once more)
+      if (trial) {
+        const bool ok = fx - fxp < 1e-4f * stp * dn_pg;
+        if (!ok) stp *= 0.5f;
+        if (ok || count >= 10) iter_end = true;
+      }
+      if (__any(iter_end)) {   // end of an iteration: the step is taken; the update
+        const float y = g - gp;
+        const float gnorm = sqrtf(gsum<G>(gp * gp));
+        bool upd = false;
+        if (iter_end) { if (!(gnorm >= 1e-5f)) ls_done = true; else upd = true; }
+        const float yy = gsum<G>(y * y), yp = gsum<G>(y * d);
+        if (iter_end && dn_step == 0 && upd && fabsf(yy) > kEpsFl) H[l * G + l] = stp * yp / yy;   // first step: scaled diagonal
+        const float mhy = dense_minus_hv<G>(H, H + G * G, y, l);
+        const float yhy = -gsum<G>(y * mhy);
+        const float r = 1.0f / (stp * yp);
+        dense_update<G>(H, H + G * G, mhy, d, stp, r, yhy, iter_end && upd && !(stp * yp < kEpsFl), l);
+        if (iter_end && !ls_done) { ++dn_step; if (dn_step < max_steps) begin = true; else ls_done = true; }
+      }
+      if (__any(begin)) {      // an iteration starts at x: p = -H g
+        if (begin) { stash_put<G>(S.stash, kSlotPrev, x); gp = g; fxp = fx; prev_rg = x_rg; }
+        const float pn = dense_minus_hv<G>(H, H + G * G, g, l);
+        const float pg = gsum<G>(pn * g);
+        if (begin) { d = pn; dn_pg = pg; stp = 1.0f; count = 0; phase = kTrial; }
+      }
+    }
+    if constexpr (POL >= 0) {
     if (__any(start)) {
       const float gg = gsum<G>(g * g);
       if (start) {
@@ -481,6 +441,7 @@ __device__ void mc_chain_flat(const LigView& L, const Scratch& S, const EvalK& K
         }
       }
     }
+    }   // RILC-BFGS
     if constexpr (POL == 2) {
       if (__any(rec_own)) hist_add<G>(*hist, x, fx, g, l, L.T, rec_own);
       if (__any(rec_glob)) glob_add<G>(*gh, x, fx, g, l, L.T, rec_glob);
@@ -549,14 +510,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W, 8))) void
       gh.base = a.glob + a.out_off[wi.x].w; gh.per_axis = h.reserved; gh.T = h.n_torsions;
       for (int k = 0; k < 3; ++k) { gh.lo[k] = a.P.box_lo[k]; gh.hi[k] = a.P.box_hi[k]; }
     }
-    if constexpr (MODE == 0)
-      mc_chain<G>(L, S, H, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l,
-                  a.out_e + a.out_off[wi.x].x, a.out_conf + a.out_off[wi.x].y, a.out_heavy + a.out_off[wi.x].z, evals,
-                  given, a.ls_only != 0);
-    else
-      mc_chain_flat<G, MODE - 1>(L, S, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l,
-                                 a.out_e + a.out_off[wi.x].x, a.out_conf + a.out_off[wi.x].y,
-                                 a.out_heavy + a.out_off[wi.x].z, evals, &hist, &gh, given, a.ls_only != 0);
+    mc_chain_flat<G, MODE - 1>(L, S, a.K, a.P, h.search_depth, h.bfgs_steps, a.streams[wi.x], chain, valid, l,
+                               a.out_e + a.out_off[wi.x].x, a.out_conf + a.out_off[wi.x].y,
+                               a.out_heavy + a.out_off[wi.x].z, evals, &hist, &gh, given, a.ls_only != 0, H);
     if (a.slot_evals) {
       int sum = 0;
       for (int gi = 0; gi < CPW; ++gi) sum += __shfl(evals, gi * G);
