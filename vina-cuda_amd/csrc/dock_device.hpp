@@ -809,41 +809,63 @@ __device__ __attribute__((noinline)) bool qvw_interesting(const History& h, cons
 // lane that owns change component i.  The loops over a row stay rolled: unrolled G-long, they
 // kept G LDS values live at once and spilled up to 1.3 KB per lane.
 // Dense BFGS (bfgs.h:30-140 as the reference GPU code runs it, quasi_newton.cuh:2391-2499): the
-// matrix of each chain group.  H is G x G, lane l owns row l, kept column-major (entry (l, j) at
-// H[j G + l]) so that the lanes' accesses at one j are consecutive words: no LDS bank conflicts
-// (row-major, the lanes of a group would all hit one bank).  vbuf (2 G floats after H) broadcasts
-// vectors.  Every lane of the wavefront calls these (they fence); `on` selects the groups whose
-// matrix is written.
+// matrix of each chain group, over the n = 6 + T dimensions of the change vector (torsion k - 1 in
+// lane k, position and orientation in lanes G-6 .. G-1; see the lane layout above).  Dimension c
+// of lane l is dense_dim(l); the lanes of the other G - n lanes hold zeros in every vector, so
+// their rows and columns of a G x G matrix would stay zero and only ever meet zeros: the matrix
+// keeps the n x n block, in the group's G x G area.  Lane l owns row dense_dim(l), kept
+// column-major (entry (c, c') at H[c' n + c]) so that the lanes' accesses at one c' are
+// consecutive words: no LDS bank conflicts (row-major, the lanes would hit one bank).  Sums run
+// over the dimensions in lane order.  vbuf (2 G floats after the G x G area) broadcasts vectors.
+// Every lane of the wavefront calls these (they fence); `on` selects the groups whose matrix is
+// written.
+template <int G> __device__ __forceinline__ int dense_dim(int l, int T) {
+  return (l >= 1 && l <= T) ? l - 1 : (l >= G - 6 ? T + (l - (G - 6)) : -1);
+}
+template <int G> __device__ __forceinline__ int dense_lane(int c, int T) { return c < T ? c + 1 : (G - 6) + (c - T); }
 template <int G>
-__device__ __forceinline__ void dense_identity(float* H, int l, bool on) {
-  if (on) {
+__device__ __forceinline__ void dense_identity(float* H, int l, int T, bool on) {
+  const int n = 6 + T, c = dense_dim<G>(l, T);
+  if (on && c >= 0) {
 #pragma unroll 4
-    for (int j = 0; j < G; ++j) H[j * G + l] = (j == l) ? 1.0f : 0.0f;
+    for (int j = 0; j < n; ++j) H[j * n + c] = (j == c) ? 1.0f : 0.0f;
   }
 }
-// -(H v)_l, summed in column order
+// -(H v) for this lane's dimension (-0 for the other lanes)
 template <int G>
-__device__ __forceinline__ float dense_minus_hv(const float* H, float* vbuf, float v, int l) {
+__device__ __forceinline__ float dense_minus_hv(const float* H, float* vbuf, float v, int l, int T) {
+  const int n = 6 + T, c = dense_dim<G>(l, T);
   __syncthreads(); vbuf[l] = v; __syncthreads();
   float acc = 0.0f;
+  if (c >= 0) {
 #pragma unroll 4
-  for (int j = 0; j < G; ++j) acc += H[j * G + l] * vbuf[j];
+    for (int j = 0; j < n; ++j) acc += H[j * n + c] * vbuf[dense_lane<G>(j, T)];
+  }
   __syncthreads();
   return -acc;
 }
-// H_lj += alpha r (mhy_l p_j + mhy_j p_l) + alpha^2 (r^2 yhy + r) p_l p_j  (mhy = -H y, r = 1/(alpha y.p))
+// H_cj += alpha r (mhy_c p_j + mhy_j p_c) + alpha^2 (r^2 yhy + r) p_c p_j  (mhy = -H y, r = 1/(alpha y.p))
 template <int G>
 __device__ __forceinline__ void dense_update(float* H, float* vbuf, float mhy, float p, float alpha, float r, float yhy,
-                                             bool on, int l) {
+                                             bool on, int l, int T) {
+  const int n = 6 + T, c = dense_dim<G>(l, T);
   float* pb = vbuf + G;
   __syncthreads();
   vbuf[l] = mhy; pb[l] = p;
   __syncthreads();
-  if (on)
+  if (on && c >= 0)
 #pragma unroll 4
-    for (int j = 0; j < G; ++j)
-      H[j * G + l] += alpha * r * (mhy * pb[j] + vbuf[j] * p) + alpha * alpha * (r * r * yhy + r) * p * pb[j];
+    for (int j = 0; j < n; ++j) {
+      const int lj = dense_lane<G>(j, T);
+      H[j * n + c] += alpha * r * (mhy * pb[lj] + vbuf[lj] * p) + alpha * alpha * (r * r * yhy + r) * p * pb[lj];
+    }
   __syncthreads();
+}
+// the diagonal entry of this lane's dimension
+template <int G>
+__device__ __forceinline__ void dense_set_diag(float* H, float v, int l, int T, bool on) {
+  const int n = 6 + T, c = dense_dim<G>(l, T);
+  if (on && c >= 0) H[c * n + c] = v;
 }
 
 }  // namespace vdk

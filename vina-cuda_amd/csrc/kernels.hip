@@ -359,7 +359,7 @@ __device__ void mc_chain_flat(const LigView& L, const Scratch& S, const EvalK& K
     bool rec_own = false, rec_glob = false;   // QuickVina-W records of this pass
     if constexpr (POL < 0) {   // dense BFGS (quasi_newton.cuh:2391-2499); x, fx, g hold the point evaluated
       // a search starts from the identity matrix; no step limit -> the start point is the result
-      dense_identity<G>(H, l, start);
+      dense_identity<G>(H, l, L.T, start);
       if (start) { dn_step = 0; if (max_steps > 0) begin = true; else ls_done = true; }
       // backtracking: accept when f1 - f0 < 1e-4 alpha p.g, else halve alpha; after ten trials the
       // last point is taken (alpha halved once more)
@@ -374,16 +374,16 @@ __device__ void mc_chain_flat(const LigView& L, const Scratch& S, const EvalK& K
         bool upd = false;
         if (iter_end) { if (!(gnorm >= 1e-5f)) ls_done = true; else upd = true; }
         const float yy = gsum<G>(y * y), yp = gsum<G>(y * d);
-        if (iter_end && dn_step == 0 && upd && fabsf(yy) > kEpsFl) H[l * G + l] = stp * yp / yy;   // first step: scaled diagonal
-        const float mhy = dense_minus_hv<G>(H, H + G * G, y, l);
+        dense_set_diag<G>(H, stp * yp / yy, l, L.T, iter_end && dn_step == 0 && upd && fabsf(yy) > kEpsFl);   // first step: scaled diagonal
+        const float mhy = dense_minus_hv<G>(H, H + G * G, y, l, L.T);
         const float yhy = -gsum<G>(y * mhy);
         const float r = 1.0f / (stp * yp);
-        dense_update<G>(H, H + G * G, mhy, d, stp, r, yhy, iter_end && upd && !(stp * yp < kEpsFl), l);
+        dense_update<G>(H, H + G * G, mhy, d, stp, r, yhy, iter_end && upd && !(stp * yp < kEpsFl), l, L.T);
         if (iter_end && !ls_done) { ++dn_step; if (dn_step < max_steps) begin = true; else ls_done = true; }
       }
       if (__any(begin)) {      // an iteration starts at x: p = -H g
         if (begin) { stash_put<G>(S.stash, kSlotPrev, x); gp = g; fxp = fx; prev_rg = x_rg; }
-        const float pn = dense_minus_hv<G>(H, H + G * G, g, l);
+        const float pn = dense_minus_hv<G>(H, H + G * G, g, l, L.T);
         const float pg = gsum<G>(pn * g);
         if (begin) { d = pn; dn_pg = pg; stp = 1.0f; count = 0; phase = kTrial; }
       }
