@@ -6,7 +6,7 @@ path's own definition:
     chains per wavefront): the plain and the batched launch return the same 37 chains, and the
     best chains' energies are the oracle's evaluation of their conformations;
   - a rigid ligand (no torsion) and a ligand with many torsions, batched together with a typical
-    one: finite results, best chains confirmed by the oracle;
+    one, under RILC-BFGS and dense BFGS: finite results, best chains confirmed by the oracle;
   - the same ligand twice in one batch: each slot's chains draw from the ligand's own random stream
     (keyed by the seed and the ligand id), so the two slots return the same chains bit for bit;
   - a ligand wider than the largest group (120 atoms, 24 torsions) docked through the plain path:
@@ -68,14 +68,16 @@ def test_partial_chain_group(ctx, scene):
     say("37 chains: worst |e - e_oracle|", worst)
 
 
-def test_rigid_and_many_torsions_in_one_batch(ctx, scene):
+@pytest.mark.parametrize("local_search", [1, 0])
+def test_rigid_and_many_torsions_in_one_batch(ctx, scene, local_search):
+    """RILC-BFGS and dense BFGS (whose matrix spans the 6 + T dimensions: 6 for the rigid ligand)."""
     import vdock
     ctx.grid_upload(scene["box"], scene["maps"])
     texts = [vdock.synth_ligand(16, 0, 31), vdock.synth_ligand(45, 15, 32), vdock.synth_ligand(24, 5, 33)]
     ids = [ctx.add_ligand(t) for t in texts]
     assert [ctx.ligand_info(i).n_torsions for i in ids][0] == 0
     assert ctx.ligand_info(ids[1]).n_torsions >= 12
-    st = ctx.dock_batch(ids, vdock.default_params(chains=128, seed=4))
+    st = ctx.dock_batch(ids, vdock.default_params(chains=128, seed=4, local_search=local_search))
     assert st.evals > 0
     for k, (lid, t) in enumerate(zip(ids, texts)):
         e, conf, heavy = ctx.batch_results(k)
